@@ -1,0 +1,174 @@
+/*
+ * morphogen_hip.h - C ABI of the MI355X (gfx950) fused ChaCha8-DPF
+ * masked-XOR PIR scan.
+ *
+ * Plain C: opaque handles, pointers and sizes, int status codes, caller-owned
+ * host buffers. No call aborts or throws across this boundary; on failure a
+ * non-zero MG_* code is returned and mg_last_error() describes it (per
+ * calling thread).
+ *
+ * Each entry point names the reference interface it replaces
+ * (igor53627/morphogenesis, paths relative to the repository root). The
+ * reference's page leaves (4096 B) and 25-bit domain cap are generalised to
+ * one DPF leaf per W-byte row (W a multiple of 16) and d <= 32.
+ *
+ * Answer semantics for one key (kernel.rs:1243-1256, 1374-1383):
+ *   ans[j] = XOR over rows i < n_rows of rows[i][j] & mask(i)[j mod 16]
+ * where mask(i) is the 16-byte DPF leaf of key at index i (dpf.rs:103-142).
+ */
+#ifndef MORPHOGEN_HIP_H
+#define MORPHOGEN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MG_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+  MG_OK = 0,
+  MG_ERR_INVALID_ARG = 1,   /* null handle/pointer, bad size or option      */
+  MG_ERR_KEY_FORMAT = 2,    /* GpuDpfError::InvalidKeyFormat (dpf.rs:277-290) */
+  MG_ERR_DOMAIN = 3,        /* d > 32, or keys of one call disagree on d     */
+  MG_ERR_ROWS = 4,          /* n_rows > 2^d (kernel.rs:1283-1285), bad W,
+                               row range or delta row index out of bounds    */
+  MG_ERR_OUT_OF_MEMORY = 5, /* device allocation failed                     */
+  MG_ERR_DEVICE = 6,        /* HIP runtime error                            */
+  MG_ERR_NO_DEVICE = 7,     /* no such GPU ordinal                          */
+  MG_ERR_BUSY = 8,          /* server queue full / admission refused        */
+  MG_ERR_STATE = 9          /* call not valid in the object's current state */
+};
+
+typedef struct mg_device mg_device;   /* GpuScanner (kernel.rs:158-171)      */
+typedef struct mg_matrix mg_matrix;   /* GpuPageMatrix (storage.rs:18-22)    */
+typedef struct mg_delta mg_delta;     /* device copy of a DeltaBuffer
+                                         snapshot (delta.rs:55-307)          */
+typedef struct mg_server mg_server;   /* continuous-batching request loop    */
+
+/* Parsed key header (ChaChaKey fields, dpf.rs:91-99). */
+typedef struct {
+  uint32_t domain_bits;
+  uint32_t party;
+  uint32_t root_t;
+  uint32_t wire_len;
+} mg_key_info;
+
+/* Per-call timing (KernelTiming, kernel.rs:1213-1228), milliseconds, whole
+ * call (the reference averages per query). kernel_ms is measured with HIP
+ * events on the stream the scan kernels run on. */
+typedef struct {
+  double h2d_ms;
+  double kernel_ms;
+  double d2h_ms;
+  double total_ms;
+  double scan_kernel_ms; /* the fused scan kernel(s) alone                 */
+  uint32_t kernel;       /* MG_KERNEL_* actually used                      */
+  uint32_t lane_depth;   /* kl actually used                               */
+  uint32_t keys_per_wave;
+  uint32_t waves;        /* waves launched for the fused scan              */
+} mg_timing;
+
+enum {
+  MG_KERNEL_AUTO = 0,
+  MG_KERNEL_LANE_DFS = 1, /* per-lane subtree DFS, rows of <= 128 B       */
+  MG_KERNEL_STREAM = 2,   /* LDS-staged masks, coalesced wide rows         */
+  MG_KERNEL_POINT = 3     /* one full path walk per row (reference-style)  */
+};
+
+/* Scan options; zero-initialise for defaults. The tuning fields replace the
+ * reference's kernel flavour / tile flags of scan_batch_opts
+ * (kernel.rs:711-720). */
+typedef struct {
+  uint32_t kernel;          /* MG_KERNEL_*                                 */
+  uint32_t lane_depth;      /* kl: leaves per lane = 2^kl; 0 = tuned       */
+  uint32_t keys_per_wave;   /* stream kernel only, 1-4; 0 = tuned          */
+  uint32_t waves_per_key;   /* 0 = tuned                                   */
+  uint32_t waves_per_block; /* 1-4; 0 = tuned                              */
+  uint32_t use_graph;       /* replay the launch sequence as a HIP graph   */
+  uint64_t row_begin;       /* scan rows [row_begin, row_end) only; the    */
+  uint64_t row_end;         /* range must be tile-aligned unless it is the */
+                            /* whole matrix; row_end 0 = n_rows            */
+  const mg_delta *delta;    /* if set: answers ^= delta contribution
+                               (scan_delta_for_gpu, scan.rs:434-485)       */
+} mg_scan_opts;
+
+int mg_abi_version(void);
+const char *mg_last_error(void);
+
+/* GpuScanner::new(device_ord) (kernel.rs:258). */
+int mg_device_open(int ordinal, mg_device **out);
+int mg_device_close(mg_device *dev);
+/* Free/total HBM and compute-unit count of the device. */
+int mg_device_info(mg_device *dev, uint64_t *free_bytes, uint64_t *total_bytes,
+                   uint32_t *compute_units, char *name, size_t name_len);
+
+/* GpuPageMatrix::alloc_empty (storage.rs:89-101): zeroed rows. */
+int mg_matrix_create(mg_device *dev, uint64_t n_rows, uint32_t row_bytes,
+                     mg_matrix **out);
+/* GpuPageMatrix::update_pages (storage.rs:104-124) / new (storage.rs:27-43):
+ * copy n_rows host rows to rows [first_row, first_row + n_rows). */
+int mg_matrix_upload(mg_matrix *m, uint64_t first_row, const void *rows,
+                     uint64_t n_rows);
+int mg_matrix_download(mg_matrix *m, uint64_t first_row, void *rows,
+                       uint64_t n_rows);
+/* Seeded synthetic rows generated on the device: 64-bit word j of the
+ * row-major matrix = splitmix64 output j for `seed` (little-endian). */
+int mg_matrix_fill_seeded(mg_matrix *m, uint64_t seed);
+/* ChunkedMatrix::fill_with_pattern (chunked_matrix.rs:83-93), generated on
+ * the device with LCG jump-ahead. */
+int mg_matrix_fill_pattern(mg_matrix *m, uint64_t seed);
+int mg_matrix_info(const mg_matrix *m, uint64_t *n_rows, uint32_t *row_bytes);
+int mg_matrix_destroy(mg_matrix *m);
+
+/* ChaChaKey::from_bytes (dpf.rs:277-316): exact length 19 + 18d + 16, plus
+ * the evaluator's limit d <= 32 (ChaChaParams::new, dpf.rs:65-78). */
+int mg_key_parse(const uint8_t *bytes, size_t len, mg_key_info *out);
+
+/* generate_chacha_dpf_keys (dpf.rs:322-432), client side, host only: writes
+ * the two parties' wire keys (19 + 18d + 16 bytes each) for a point function
+ * at `target` over a d-bit domain. Root seeds are derived from `seed`
+ * (splitmix64 words 0-1 and 2-3) instead of the OS RNG. */
+int mg_keygen(uint32_t domain_bits, uint64_t target, uint64_t seed,
+              uint8_t *key0, uint8_t *key1);
+
+/* GpuScanner::scan / scan_batch / scan_batch_optimized(_tiled)(_with_graph)
+ * (kernel.rs:685-1162), flattened: n_keys keys of the same domain in one
+ * pass over the matrix. keys = concatenated wire keys, key_lens[i] their
+ * lengths. out = n_keys x row_bytes, caller-owned. timing may be NULL. */
+int mg_scan(mg_device *dev, mg_matrix *m, const uint8_t *keys,
+            const size_t *key_lens, uint32_t n_keys, uint8_t *out,
+            const mg_scan_opts *opts, mg_timing *timing);
+
+/* scan_batch_single_query_multistream_optimized(_with_graph)
+ * (kernel.rs:955-1097): one pass per query of keys_per_query keys, the
+ * reference's "plain" serving mode; same answers as mg_scan. */
+int mg_scan_per_query(mg_device *dev, mg_matrix *m, const uint8_t *keys,
+                      const size_t *key_lens, uint32_t n_keys,
+                      uint32_t keys_per_query, uint8_t *out,
+                      const mg_scan_opts *opts, mg_timing *timing);
+
+/* DeltaBuffer snapshot on the device (delta.rs:55-307): entries
+ * (row_idx[i], diffs[i * row_bytes ...]) XOR-patch rows. */
+int mg_delta_create(mg_device *dev, uint32_t row_bytes, uint64_t capacity,
+                    mg_delta **out);
+int mg_delta_set(mg_delta *dl, const uint64_t *row_idx, const uint8_t *diffs,
+                 uint64_t n_entries, uint64_t epoch);
+int mg_delta_info(const mg_delta *dl, uint64_t *n_entries, uint64_t *epoch);
+int mg_delta_destroy(mg_delta *dl);
+/* scan_delta_for_gpu (scan.rs:434-485), one leaf per row: out =
+ * n_keys x row_bytes answers of the delta entries alone. */
+int mg_scan_delta(mg_device *dev, const mg_delta *dl, const uint8_t *keys,
+                  const size_t *key_lens, uint32_t n_keys, uint8_t *out);
+/* Epoch merge (epoch.rs:118-181 XOR patch + update_pages): rows[idx] ^= diff
+ * for every entry; repeated indices compose by XOR. */
+int mg_matrix_apply_delta(mg_matrix *m, const mg_delta *dl);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

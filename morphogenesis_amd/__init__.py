@@ -1,0 +1,410 @@
+"""MI355X-native fused ChaCha8-DPF masked-XOR PIR scan.
+
+Python mirror of the reference's GPU scan interface
+(crates/morphogen-gpu-dpf/src/kernel.rs GpuScanner, src/storage.rs
+GpuPageMatrix, src/dpf.rs ChaChaKey wire format), driving the C ABI of
+libmorphogen_hip.so (include/morphogen_hip.h) through ctypes. Every scan runs
+on the GPU; there is no CPU fallback, and importing this package fails if the
+library has not been built.
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmorphogen_hip.so")
+
+KEYS_PER_QUERY = 3  # cuckoo locations per query (kernel.rs KEYS_PER_QUERY)
+
+MG_OK = 0
+MG_ERR_INVALID_ARG = 1
+MG_ERR_KEY_FORMAT = 2
+MG_ERR_DOMAIN = 3
+MG_ERR_ROWS = 4
+MG_ERR_OUT_OF_MEMORY = 5
+MG_ERR_DEVICE = 6
+MG_ERR_NO_DEVICE = 7
+MG_ERR_BUSY = 8
+MG_ERR_STATE = 9
+
+KERNEL_AUTO, KERNEL_LANE_DFS, KERNEL_STREAM, KERNEL_POINT = range(4)
+KERNEL_NAMES = {KERNEL_LANE_DFS: "lane_dfs", KERNEL_STREAM: "stream", KERNEL_POINT: "point"}
+
+
+class MgError(RuntimeError):
+    """Error returned through the C ABI (status code + mg_last_error text)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+class InvalidKeyFormat(MgError):
+    """GpuDpfError::InvalidKeyFormat (dpf.rs:14)."""
+
+
+class InvalidDomainBits(MgError):
+    """GpuDpfError::InvalidDomainBits (dpf.rs:11)."""
+
+
+class RowsError(MgError):
+    """More rows than 2^d, bad row width or out-of-range rows."""
+
+
+class NoDevice(MgError):
+    pass
+
+
+_ERRORS = {MG_ERR_KEY_FORMAT: InvalidKeyFormat, MG_ERR_DOMAIN: InvalidDomainBits,
+           MG_ERR_ROWS: RowsError, MG_ERR_NO_DEVICE: NoDevice}
+
+
+class KeyInfo(ctypes.Structure):
+    _fields_ = [("domain_bits", ctypes.c_uint32), ("party", ctypes.c_uint32),
+                ("root_t", ctypes.c_uint32), ("wire_len", ctypes.c_uint32)]
+
+
+class Timing(ctypes.Structure):
+    """KernelTiming (kernel.rs:1213-1228), per call, in milliseconds."""
+    _fields_ = [("h2d_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
+                ("d2h_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("scan_kernel_ms", ctypes.c_double), ("kernel", ctypes.c_uint32),
+                ("lane_depth", ctypes.c_uint32), ("keys_per_wave", ctypes.c_uint32),
+                ("waves", ctypes.c_uint32)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["kernel"] = KERNEL_NAMES.get(self.kernel, str(self.kernel))
+        return d
+
+
+class ScanOpts(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_uint32), ("lane_depth", ctypes.c_uint32),
+                ("keys_per_wave", ctypes.c_uint32), ("waves_per_key", ctypes.c_uint32),
+                ("waves_per_block", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
+                ("row_begin", ctypes.c_uint64), ("row_end", ctypes.c_uint64),
+                ("delta", ctypes.c_void_p)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C {_HERE}` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u8p, u64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint64)
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    sigs = {
+        "mg_abi_version": (ctypes.c_int, []),
+        "mg_last_error": (ctypes.c_char_p, []),
+        "mg_device_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+        "mg_device_close": (ctypes.c_int, [vp]),
+        "mg_device_info": (ctypes.c_int, [vp, u64p, u64p, ctypes.POINTER(ctypes.c_uint32),
+                                          ctypes.c_char_p, ctypes.c_size_t]),
+        "mg_matrix_create": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32,
+                                            ctypes.POINTER(vp)]),
+        "mg_matrix_upload": (ctypes.c_int, [vp, ctypes.c_uint64, vp, ctypes.c_uint64]),
+        "mg_matrix_download": (ctypes.c_int, [vp, ctypes.c_uint64, vp, ctypes.c_uint64]),
+        "mg_matrix_fill_seeded": (ctypes.c_int, [vp, ctypes.c_uint64]),
+        "mg_matrix_fill_pattern": (ctypes.c_int, [vp, ctypes.c_uint64]),
+        "mg_matrix_info": (ctypes.c_int, [vp, u64p, ctypes.POINTER(ctypes.c_uint32)]),
+        "mg_matrix_destroy": (ctypes.c_int, [vp]),
+        "mg_keygen": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, u8p,
+                                     u8p]),
+        "mg_key_parse": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.POINTER(KeyInfo)]),
+        "mg_scan": (ctypes.c_int, [vp, vp, u8p, szp, ctypes.c_uint32, u8p,
+                                   ctypes.POINTER(ScanOpts), ctypes.POINTER(Timing)]),
+        "mg_scan_per_query": (ctypes.c_int, [vp, vp, u8p, szp, ctypes.c_uint32, ctypes.c_uint32,
+                                             u8p, ctypes.POINTER(ScanOpts),
+                                             ctypes.POINTER(Timing)]),
+        "mg_delta_create": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint64,
+                                           ctypes.POINTER(vp)]),
+        "mg_delta_set": (ctypes.c_int, [vp, u64p, u8p, ctypes.c_uint64, ctypes.c_uint64]),
+        "mg_delta_info": (ctypes.c_int, [vp, u64p, u64p]),
+        "mg_delta_destroy": (ctypes.c_int, [vp]),
+        "mg_scan_delta": (ctypes.c_int, [vp, vp, u8p, szp, ctypes.c_uint32, u8p]),
+        "mg_matrix_apply_delta": (ctypes.c_int, [vp, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def _check(rc: int):
+    if rc != MG_OK:
+        msg = (lib.mg_last_error() or b"").decode(errors="replace")
+        raise _ERRORS.get(rc, MgError)(rc, msg)
+
+
+def _u8(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def _pack_keys(keys: Sequence[bytes]):
+    blob = np.frombuffer(b"".join(bytes(k) for k in keys) or b"\0", dtype=np.uint8).copy()
+    lens = (ctypes.c_size_t * max(1, len(keys)))(*[len(k) for k in keys])
+    return blob, lens
+
+
+def parse_key(data: bytes) -> KeyInfo:
+    """ChaChaKey::from_bytes validation (dpf.rs:277-316) through the C ABI."""
+    buf = np.frombuffer(bytes(data) or b"\0", dtype=np.uint8).copy()
+    info = KeyInfo()
+    _check(lib.mg_key_parse(_u8(buf), len(data), ctypes.byref(info)))
+    return info
+
+
+def generate_keys(domain_bits: int, target: int, seed: int):
+    """generate_chacha_dpf_keys (dpf.rs:322-432) with seeded root seeds:
+    returns the two parties' wire keys."""
+    n = 19 + 18 * domain_bits + 16 if 0 < domain_bits <= 32 else 1
+    k0 = np.zeros(n, dtype=np.uint8)
+    k1 = np.zeros(n, dtype=np.uint8)
+    rc = lib.mg_keygen(domain_bits, target, seed, _u8(k0), _u8(k1))
+    if rc != MG_OK:
+        raise _ERRORS.get(rc, MgError)(rc, f"keygen(d={domain_bits}, target={target})")
+    return k0.tobytes(), k1.tobytes()
+
+
+@dataclass
+class PirResult:
+    """PirResult (kernel.rs:1230-1240) generalised to W-byte rows: one answer
+    per key of the query; verif fields stay zero as in the reference build
+    (verifiable PIR compiled out)."""
+    pages: List[bytes]
+    verifs: List[bytes] = field(default_factory=list)
+    timing: Optional[dict] = None
+
+    @property
+    def page0(self):
+        return self.pages[0]
+
+    @property
+    def page1(self):
+        return self.pages[1]
+
+    @property
+    def page2(self):
+        return self.pages[2]
+
+
+def make_opts(kernel=KERNEL_AUTO, lane_depth=0, keys_per_wave=0, waves_per_key=0,
+              waves_per_block=0, use_graph=False, row_begin=0, row_end=0, delta=None):
+    o = ScanOpts()
+    o.kernel = kernel
+    o.lane_depth = lane_depth
+    o.keys_per_wave = keys_per_wave
+    o.waves_per_key = waves_per_key
+    o.waves_per_block = waves_per_block
+    o.use_graph = 1 if use_graph else 0
+    o.row_begin = row_begin
+    o.row_end = row_end
+    o.delta = delta._h if delta is not None else None
+    return o
+
+
+class GpuScanner:
+    """GpuScanner (kernel.rs:158-171): one gfx950 device, its stream and
+    workspace."""
+
+    def __init__(self, device_ord: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib.mg_device_open(device_ord, ctypes.byref(h)))
+        self._h = h
+        self.device_ord = device_ord
+
+    def close(self):
+        if self._h:
+            lib.mg_device_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        fr, tot = ctypes.c_uint64(), ctypes.c_uint64()
+        cus = ctypes.c_uint32()
+        name = ctypes.create_string_buffer(256)
+        _check(lib.mg_device_info(self._h, ctypes.byref(fr), ctypes.byref(tot), ctypes.byref(cus),
+                                  name, 256))
+        return {"free_bytes": fr.value, "total_bytes": tot.value, "compute_units": cus.value,
+                "name": name.value.decode()}
+
+    # -- flat key list (the C ABI's native form) --------------------------
+    def scan_keys(self, db: "GpuPageMatrix", keys: Sequence[bytes], opts: ScanOpts = None,
+                  with_timing=False):
+        """Answers of n keys (n x W uint8) from one pass over the matrix."""
+        n = len(keys)
+        out = np.zeros((max(n, 1), db.row_bytes), dtype=np.uint8)
+        blob, lens = _pack_keys(keys)
+        t = Timing()
+        _check(lib.mg_scan(self._h, db._h, _u8(blob), lens, n, _u8(out),
+                           ctypes.byref(opts) if opts is not None else None, ctypes.byref(t)))
+        out = out[:n]
+        return (out, t.as_dict()) if with_timing else out
+
+    def scan_keys_per_query(self, db, keys, keys_per_query=KEYS_PER_QUERY, opts=None,
+                            with_timing=False):
+        """One matrix pass per query (the reference's multistream mode)."""
+        n = len(keys)
+        out = np.zeros((max(n, 1), db.row_bytes), dtype=np.uint8)
+        blob, lens = _pack_keys(keys)
+        t = Timing()
+        _check(lib.mg_scan_per_query(self._h, db._h, _u8(blob), lens, n, keys_per_query,
+                                     _u8(out), ctypes.byref(opts) if opts is not None else None,
+                                     ctypes.byref(t)))
+        out = out[:n]
+        return (out, t.as_dict()) if with_timing else out
+
+    # -- reference-shaped entry points -----------------------------------
+    def scan(self, db, keys: Sequence[bytes], opts=None) -> PirResult:
+        """GpuScanner::scan (kernel.rs:927-935): one query of 3 keys."""
+        return self.scan_batch(db, [list(keys)], opts)[0]
+
+    def scan_batch(self, db, queries: Sequence[Sequence[bytes]], opts=None) -> List[PirResult]:
+        """GpuScanner::scan_batch (kernel.rs:685-700): all queries in one pass."""
+        flat = [k for q in queries for k in q]
+        if not flat:
+            return []
+        out, t = self.scan_keys(db, flat, opts, with_timing=True)
+        res, i = [], 0
+        for q in queries:
+            pages = [out[i + j].tobytes() for j in range(len(q))]
+            res.append(PirResult(pages, [bytes(16)] * len(q), t))
+            i += len(q)
+        return res
+
+    def scan_batch_optimized_tiled(self, db, queries, tile_size: int, use_graph=False):
+        """scan_batch_optimized_tiled(_with_graph) (kernel.rs:1124-1162): at
+        most tile_size queries per pass."""
+        tile = max(1, int(tile_size))
+        out = []
+        for i in range(0, len(queries), tile):
+            out += self.scan_batch(db, queries[i:i + tile], make_opts(use_graph=use_graph))
+        return out
+
+    def scan_batch_single_query(self, db, queries, use_graph=False):
+        """scan_batch_single_query_multistream_optimized(_with_graph)
+        (kernel.rs:955-1097): one pass per query."""
+        flat = [k for q in queries for k in q]
+        if not flat:
+            return []
+        per = len(queries[0])
+        out, t = self.scan_keys_per_query(db, flat, per, make_opts(use_graph=use_graph),
+                                          with_timing=True)
+        return [PirResult([out[i * per + j].tobytes() for j in range(per)], [bytes(16)] * per, t)
+                for i in range(len(queries))]
+
+    def scan_delta(self, delta: "DeltaBuffer", keys: Sequence[bytes]) -> np.ndarray:
+        """scan_delta_for_gpu (scan.rs:434-485) for one leaf per row."""
+        n = len(keys)
+        out = np.zeros((max(n, 1), delta.row_bytes), dtype=np.uint8)
+        blob, lens = _pack_keys(keys)
+        _check(lib.mg_scan_delta(self._h, delta._h, _u8(blob), lens, n, _u8(out)))
+        return out[:n]
+
+
+class GpuPageMatrix:
+    """GpuPageMatrix (storage.rs:18-145) with W-byte rows."""
+
+    def __init__(self, scanner: GpuScanner, n_rows: int, row_bytes: int):
+        h = ctypes.c_void_p()
+        _check(lib.mg_matrix_create(scanner._h, n_rows, row_bytes, ctypes.byref(h)))
+        self._h = h
+        self.scanner = scanner  # the device must outlive this matrix
+        self.n_rows = n_rows
+        self.row_bytes = row_bytes
+
+    @classmethod
+    def alloc_empty(cls, scanner, n_rows, row_bytes):
+        return cls(scanner, n_rows, row_bytes)
+
+    @classmethod
+    def new(cls, scanner, rows: np.ndarray):
+        rows = np.ascontiguousarray(rows, dtype=np.uint8)
+        m = cls(scanner, rows.shape[0], rows.shape[1])
+        m.update_rows(0, rows)
+        return m
+
+    def update_rows(self, start_row: int, rows: np.ndarray):
+        rows = np.ascontiguousarray(rows, dtype=np.uint8)
+        assert rows.ndim == 2 and rows.shape[1] == self.row_bytes
+        _check(lib.mg_matrix_upload(self._h, start_row, rows.ctypes.data, rows.shape[0]))
+
+    def download(self, start_row=0, n_rows=None) -> np.ndarray:
+        n = self.n_rows - start_row if n_rows is None else n_rows
+        out = np.zeros((n, self.row_bytes), dtype=np.uint8)
+        _check(lib.mg_matrix_download(self._h, start_row, out.ctypes.data, n))
+        return out
+
+    def fill_seeded(self, seed: int):
+        _check(lib.mg_matrix_fill_seeded(self._h, seed))
+
+    def fill_with_pattern(self, seed: int):
+        _check(lib.mg_matrix_fill_pattern(self._h, seed))
+
+    def apply_delta(self, delta: "DeltaBuffer"):
+        _check(lib.mg_matrix_apply_delta(self._h, delta._h))
+
+    def num_pages(self):
+        return self.n_rows
+
+    def size_bytes(self):
+        return self.n_rows * self.row_bytes
+
+    def close(self):
+        if self._h:
+            lib.mg_matrix_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeltaBuffer:
+    """Device copy of a DeltaBuffer snapshot (delta.rs:55-307): XOR patches."""
+
+    def __init__(self, scanner: GpuScanner, row_bytes: int, capacity: int):
+        h = ctypes.c_void_p()
+        _check(lib.mg_delta_create(scanner._h, row_bytes, capacity, ctypes.byref(h)))
+        self._h = h
+        self.scanner = scanner  # the device must outlive this buffer
+        self.row_bytes = row_bytes
+        self.capacity = capacity
+
+    def set(self, row_idx: np.ndarray, diffs: np.ndarray, epoch: int = 0):
+        idx = np.ascontiguousarray(row_idx, dtype=np.uint64)
+        dif = np.ascontiguousarray(diffs, dtype=np.uint8).reshape(len(idx), self.row_bytes)
+        _check(lib.mg_delta_set(self._h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                _u8(dif), len(idx), epoch))
+
+    def info(self):
+        n, e = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib.mg_delta_info(self._h, ctypes.byref(n), ctypes.byref(e)))
+        return n.value, e.value
+
+    def close(self):
+        if self._h:
+            lib.mg_delta_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

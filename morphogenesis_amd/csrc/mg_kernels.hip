@@ -1,0 +1,539 @@
+// gfx950 kernels of the fused ChaCha8-DPF masked-XOR scan.
+//
+// Answer for one key (kernel.rs:1243-1256, 1276-1383 generalised to W-byte
+// rows, one DPF leaf per row): ans = XOR_i rows[i] & tile16(mask(i)).
+//
+// scan_lane_dfs<C>   the hot kernel for rows of C 16-byte chunks (C <= 8).
+//                    Each wave owns one key and a contiguous range of tiles;
+//                    each lane walks its own 2^kl-leaf subtree depth-first,
+//                    so the GGM tree costs one PRG expansion per leaf pair
+//                    plus a short per-tile path walk.
+// scan_stream<S,G>   wide rows (W >= 256 B): same lane-DFS PRG for G keys per
+//                    wave; leaf masks go through LDS and each row is read
+//                    once, coalesced, for all G keys.
+// scan_point         point-eval fallback (d = 0, tiny inputs) and the delta
+//                    scan: one lane per row, full path walk per row.
+// reduce_partials    XOR of per-wave partial answers into the output.
+// fill_*             device-side seeded matrix fills.
+#include "mg_device.hpp"
+#include "mg_kernels.hpp"
+
+namespace mg {
+
+// ------------------------------------------------------------- helpers ----
+
+__device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v ^= __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint4 wave_xor4(uint4 v) {
+  return make_uint4(wave_xor(v.x), wave_xor(v.y), wave_xor(v.z), wave_xor(v.w));
+}
+
+// A path of wave-uniform nodes, one level per lane: lane L holds level L.
+// Reads use readlane with a uniform index; writes are a per-lane select.
+struct LanePath {
+  uint32_t s0, s1, s2, s3, t;
+  __device__ __forceinline__ Node get(int level) const {
+    Node n;
+    n.s0 = __builtin_amdgcn_readlane(s0, level);
+    n.s1 = __builtin_amdgcn_readlane(s1, level);
+    n.s2 = __builtin_amdgcn_readlane(s2, level);
+    n.s3 = __builtin_amdgcn_readlane(s3, level);
+    n.t = __builtin_amdgcn_readlane(t, level);
+    return n;
+  }
+  __device__ __forceinline__ void put(int lane, int level, const Node& n) {
+    const bool mine = lane == level;
+    s0 = mine ? n.s0 : s0;
+    s1 = mine ? n.s1 : s1;
+    s2 = mine ? n.s2 : s2;
+    s3 = mine ? n.s3 : s3;
+    t = mine ? n.t : t;
+  }
+};
+
+// Walks the wave-uniform tile prefix for `tile` (tb levels), reusing the
+// levels shared with the previous tile, then each lane's lb-level path to its
+// own subtree root. Returns the lane's subtree root (tree level tb + lb).
+__device__ __forceinline__ Node tile_lane_root(const DevKey* __restrict__ k,
+                                               LanePath& path, int lane,
+                                               uint64_t tile, uint64_t prev,
+                                               bool first, int tb, int lb) {
+  int start = 0;
+  if (!first) {
+    const uint64_t diff = tile ^ prev;  // non-zero, below bit tb
+    start = tb - (64 - __builtin_clzll(diff));
+  }
+  Node n = path.get(start);
+  for (int lvl = start; lvl < tb; ++lvl) {
+    n = ggm_step(k, lvl, n, (uint32_t)(tile >> (tb - 1 - lvl)) & 1u);
+    path.put(lane, lvl + 1, n);
+  }
+  for (int j = 0; j < lb; ++j)
+    n = ggm_step(k, tb + j, n, ((uint32_t)lane >> (lb - 1 - j)) & 1u);
+  return n;
+}
+
+// -------------------------------------------------------- lane-DFS scan ----
+
+template <int C>
+__global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
+  extern __shared__ uint4 lds[];
+  const int lane = threadIdx.x & 63;
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + wib;
+  if (w >= p.parts * p.n_keys) return;
+  const uint32_t key_i = w % p.n_keys;
+  const uint32_t part = w / p.n_keys;
+  const DevKey* __restrict__ k = p.keys + key_i;
+  const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
+
+  // Per-wave DFS stack: right children waiting at each subtree level.
+  uint4* st_seed = lds + (size_t)wib * kl * 80;  // [kl][64] seeds
+  uint32_t* st_t = reinterpret_cast<uint32_t*>(st_seed + kl * 64);  // [kl][64]
+
+  uint4 acc[C];
+#pragma unroll
+  for (int c = 0; c < C; c++) acc[c] = make_uint4(0, 0, 0, 0);
+
+  const uint64_t t_begin = p.tile_begin + (uint64_t)part * p.tiles_per_wave;
+  uint64_t t_end = t_begin + p.tiles_per_wave;
+  if (t_end > p.tile_end) t_end = p.tile_end;
+  const bool lane_active = lane < (1 << lb);
+  const uint32_t n_pairs = 1u << (kl - 1);
+  const int leaf_level = d - kl;
+
+  LanePath path;
+  {
+    const Node r = key_root(k);
+    path.s0 = r.s0; path.s1 = r.s1; path.s2 = r.s2; path.s3 = r.s3; path.t = r.t;
+  }
+  for (uint64_t tile = t_begin; tile < t_end; ++tile) {
+    Node node = tile_lane_root(k, path, lane, tile, tile - 1, tile == t_begin, tb, lb);
+    const uint64_t base = (tile << (kl + lb)) | ((uint64_t)lane << kl);
+
+    uint4 r0[C], r1[C];
+    auto load_pair = [&](uint32_t pair) {
+      const uint64_t x = base + 2 * (uint64_t)pair;
+      const bool v0 = lane_active && x < p.n_rows;
+      const bool v1 = lane_active && x + 1 < p.n_rows;
+      const uint4* src = p.rows + x * C;
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        r0[c] = v0 ? src[c] : make_uint4(0, 0, 0, 0);
+        r1[c] = v1 ? src[C + c] : make_uint4(0, 0, 0, 0);
+      }
+    };
+    load_pair(0);
+    int lvl = 0;
+    uint32_t pair = 0;
+    for (;;) {
+      Node L, R;
+      expand_node(k, leaf_level + lvl, node, L, R);
+      if (lvl == kl - 1) {
+        const uint4 m0 = leaf_mask(k, L), m1 = leaf_mask(k, R);
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+          xor_masked(acc[c], r0[c], m0);
+          xor_masked(acc[c], r1[c], m1);
+        }
+        if (++pair == n_pairs) break;
+        lvl = kl - 1 - __builtin_ctz(pair);
+        const uint4 sv = st_seed[lvl * 64 + lane];
+        node.s0 = sv.x; node.s1 = sv.y; node.s2 = sv.z; node.s3 = sv.w;
+        node.t = st_t[lvl * 64 + lane];
+        load_pair(pair);
+      } else {
+        ++lvl;
+        st_seed[lvl * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);
+        st_t[lvl * 64 + lane] = R.t;
+        node = L;
+      }
+    }
+  }
+
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    const uint4 v = wave_xor4(acc[c]);
+    if (lane == 0) p.partial[((size_t)part * p.n_keys + key_i) * C + c] = v;
+  }
+}
+
+// ------------------------------------------------------ point-eval scan ----
+// One lane per row; the row index is rows_idx[e] when given (delta entries),
+// else e itself. Per-wave XOR partials per key.
+__global__ __launch_bounds__(256) void scan_point(PointParams p) {
+  const int lane = threadIdx.x & 63;
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + wib;  // entry wave
+  if ((uint64_t)w * 64 >= p.n_entries) return;
+  const uint32_t key_i = blockIdx.y;
+  const DevKey* __restrict__ k = p.keys + key_i;
+  const uint64_t e = (uint64_t)w * 64 + lane;
+  const bool valid = e < p.n_entries;
+  const uint64_t x = valid ? (p.row_idx ? p.row_idx[e] : p.first_row + e) : 0;
+  const uint4 m = valid ? point_eval(k, x) : make_uint4(0, 0, 0, 0);
+  const uint4* src = p.rows + (p.row_idx ? e : x) * p.chunks;
+  for (uint32_t c = 0; c < p.chunks; c++) {
+    uint4 a = make_uint4(0, 0, 0, 0);
+    if (valid) xor_masked(a, src[c], m);
+    a = wave_xor4(a);
+    if (lane == 0) p.partial[((size_t)w * p.n_keys + key_i) * p.chunks + c] = a;
+  }
+}
+
+// ------------------------------------------------------- stream (wide) ----
+// G keys per wave, rows of C = 64*S chunks or fewer. Every leaf-pair step of
+// the lane DFS leaves 2 masks per lane per key in LDS; the wave then reads
+// those 128 rows once each, coalesced (lane = chunk), and folds them into G
+// accumulators. Accumulator slot j of lane l holds chunk l + 64*j.
+template <int S, int G>
+__global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
+  extern __shared__ uint4 lds[];
+  const int lane = threadIdx.x & 63;
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + wib;
+  const uint32_t n_groups = (p.n_keys + G - 1) / G;
+  if (w >= p.parts * n_groups) return;
+  const uint32_t grp = w % n_groups;
+  const uint32_t part = w / n_groups;
+  const uint32_t key0 = grp * G;
+  const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
+  const uint32_t C = p.chunks;
+
+  // LDS per wave: G stacks [kl][64] seeds + [kl][64] t, then masks [G][128].
+  const size_t per_wave = (size_t)G * kl * 80 + (size_t)G * 128;
+  uint4* base_lds = lds + (size_t)wib * per_wave;
+  uint4* masks = base_lds + (size_t)G * kl * 80;
+
+  const DevKey* kp[G];
+  bool key_live[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    key_live[g] = key0 + g < p.n_keys;
+    kp[g] = p.keys + (key_live[g] ? key0 + g : key0);
+  }
+
+  uint4 acc[G][S];
+#pragma unroll
+  for (int g = 0; g < G; g++)
+#pragma unroll
+    for (int j = 0; j < S; j++) acc[g][j] = make_uint4(0, 0, 0, 0);
+
+  const uint64_t t_begin = p.tile_begin + (uint64_t)part * p.tiles_per_wave;
+  uint64_t t_end = t_begin + p.tiles_per_wave;
+  if (t_end > p.tile_end) t_end = p.tile_end;
+  const uint32_t n_pairs = 1u << (kl - 1);
+  const int leaf_level = d - kl;
+  const int lanes_live = 1 << lb;
+
+  LanePath path[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const Node r = key_root(kp[g]);
+    path[g].s0 = r.s0; path[g].s1 = r.s1; path[g].s2 = r.s2;
+    path[g].s3 = r.s3; path[g].t = r.t;
+  }
+
+  for (uint64_t tile = t_begin; tile < t_end; ++tile) {
+    Node node[G];
+#pragma unroll
+    for (int g = 0; g < G; g++)
+      node[g] = tile_lane_root(kp[g], path[g], lane, tile, tile - 1, tile == t_begin, tb, lb);
+    const uint64_t tile_base = tile << (kl + lb);
+    int lvl = 0;
+    uint32_t pair = 0;
+    for (;;) {
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        uint4* st_seed = base_lds + (size_t)g * kl * 80;
+        uint32_t* st_t = reinterpret_cast<uint32_t*>(st_seed + kl * 64);
+        Node L, R;
+        expand_node(kp[g], leaf_level + lvl, node[g], L, R);
+        if (lvl == kl - 1) {
+          masks[g * 128 + 2 * lane] = leaf_mask(kp[g], L);
+          masks[g * 128 + 2 * lane + 1] = leaf_mask(kp[g], R);
+        } else {
+          st_seed[(lvl + 1) * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);
+          st_t[(lvl + 1) * 64 + lane] = R.t;
+          node[g] = L;
+        }
+      }
+      if (lvl < kl - 1) {
+        ++lvl;
+        continue;
+      }
+      // 2 rows per live source lane s: tile_base + s*2^kl + 2*pair + {0,1};
+      // row r of the step uses mask slot r. Loads go out U rows at a time.
+      constexpr int U = 8;
+      const uint32_t step_rows = 2u * (uint32_t)lanes_live;
+      for (uint32_t r0 = 0; r0 < step_rows; r0 += U) {
+        uint4 v[U][S];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const uint32_t r = r0 + u;
+          const uint64_t x = tile_base + ((uint64_t)(r >> 1) << kl) + 2 * (uint64_t)pair + (r & 1);
+          const bool ok = r < step_rows && x < p.n_rows;
+          const uint4* src = p.rows + x * C;
+#pragma unroll
+          for (int j = 0; j < S; j++) {
+            const uint32_t c = lane + 64 * j;
+            v[u][j] = (ok && c < C) ? load_nt(src + c) : make_uint4(0, 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const uint32_t r = (r0 + u) & 127u;
+#pragma unroll
+          for (int g = 0; g < G; g++) {
+            const uint4 m = masks[g * 128 + r];
+#pragma unroll
+            for (int j = 0; j < S; j++) xor_masked(acc[g][j], v[u][j], m);
+          }
+        }
+      }
+      if (++pair == n_pairs) break;
+      lvl = kl - 1 - __builtin_ctz(pair);
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        uint4* st_seed = base_lds + (size_t)g * kl * 80;
+        uint32_t* st_t = reinterpret_cast<uint32_t*>(st_seed + kl * 64);
+        const uint4 sv = st_seed[lvl * 64 + lane];
+        node[g].s0 = sv.x; node[g].s1 = sv.y; node[g].s2 = sv.z; node[g].s3 = sv.w;
+        node[g].t = st_t[lvl * 64 + lane];
+      }
+    }
+  }
+
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    if (!key_live[g]) continue;
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+      const uint32_t c = lane + 64 * j;
+      if (c < C) p.partial[((size_t)part * p.n_keys + key0 + g) * C + c] = acc[g][j];
+    }
+  }
+}
+
+// ------------------------------------------------------------ reductions --
+// out[key][c] (^)= XOR over parts of partial[part][key][c]. One block per
+// (key, chunk group of 4); 256 threads stride over parts.
+__global__ __launch_bounds__(256) void reduce_partials(const uint4* __restrict__ partial,
+                                                       uint32_t parts, uint32_t n_keys,
+                                                       uint32_t chunks, uint4* __restrict__ out,
+                                                       int accumulate) {
+  __shared__ uint4 red[256];
+  const uint32_t key = blockIdx.y;
+  const uint32_t c = blockIdx.x;
+  uint4 a = make_uint4(0, 0, 0, 0);
+  for (uint32_t q = threadIdx.x; q < parts; q += blockDim.x)
+    a = xor4(a, partial[((size_t)q * n_keys + key) * chunks + c]);
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = xor4(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    uint4* o = out + (size_t)key * chunks + c;
+    *o = accumulate ? xor4(*o, red[0]) : red[0];
+  }
+}
+
+// ----------------------------------------------------------------- fills --
+// Word j of the matrix = splitmix64 output j for `seed`; 2 words per lane.
+__global__ __launch_bounds__(256) void fill_splitmix(uint4* __restrict__ out, uint64_t first_pair,
+                                                     uint64_t n_pairs, uint64_t seed) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t v[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const uint64_t j = 2 * (first_pair + i) + h;
+      uint64_t z = seed + (j + 1) * 0x9E3779B97F4A7C15ull;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      v[h] = z ^ (z >> 31);
+    }
+    out[i] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1],
+                        (uint32_t)(v[1] >> 32));
+  }
+}
+
+// chunked_matrix.rs:83-93 byte LCG. Each lane jumps to its 16-byte block.
+__device__ __forceinline__ uint64_t lcg_jump(uint64_t seed, uint64_t steps) {
+  uint64_t mul = 1, add = 0, cm = 6364136223846793005ull, ca = 1;
+  while (steps) {
+    if (steps & 1) {
+      mul *= cm;
+      add = add * cm + ca;
+    }
+    ca = ca * cm + ca;
+    cm *= cm;
+    steps >>= 1;
+  }
+  return mul * seed + add;
+}
+
+__global__ __launch_bounds__(256) void fill_lcg(uint4* __restrict__ out, uint64_t n_blocks16,
+                                                uint64_t seed) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_blocks16;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t st = lcg_jump(seed, 16 * i);
+    uint32_t wv[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      uint32_t word = 0;
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        st = st * 6364136223846793005ull + 1;
+        word |= (uint32_t)(st >> 56) << (8 * b);
+      }
+      wv[q] = word;
+    }
+    out[i] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+  }
+}
+
+__global__ __launch_bounds__(256) void zero_u4(uint4* __restrict__ out, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = make_uint4(0, 0, 0, 0);
+}
+
+// rows[idx[e]] ^= diffs[e]; one lane per 16-byte chunk. Duplicate indices
+// are applied in one pass each by the host (see apply_delta), so no atomics.
+__global__ __launch_bounds__(256) void xor_rows(uint4* __restrict__ rows, const uint64_t* __restrict__ idx,
+                                                const uint4* __restrict__ diffs, uint64_t n_entries,
+                                                uint32_t chunks) {
+  const uint64_t total = n_entries * chunks;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t e = i / chunks;
+    const uint32_t c = (uint32_t)(i % chunks);
+    uint4* dst = rows + idx[e] * chunks + c;
+    *dst = xor4(*dst, diffs[e * chunks + c]);
+  }
+}
+
+// ------------------------------------------------------------- launchers --
+
+static int grid_for(uint64_t items, int block) {
+  uint64_t g = (items + block - 1) / block;
+  if (g > 65536 * 4) g = 65536 * 4;
+  if (g == 0) g = 1;
+  return (int)g;
+}
+
+hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_block,
+                           hipStream_t s) {
+  const uint32_t total = p.parts * p.n_keys;
+  const dim3 block(64 * waves_per_block);
+  const dim3 grid((total + waves_per_block - 1) / waves_per_block);
+  const size_t lds = (size_t)waves_per_block * p.kl * 80 * sizeof(uint4);
+  switch (chunks) {
+#define MG_CASE(C)                                                            \
+  case C:                                                                     \
+    hipLaunchKernelGGL(scan_lane_dfs<C>, grid, block, lds, s, p);             \
+    break;
+    MG_CASE(1) MG_CASE(2) MG_CASE(3) MG_CASE(4) MG_CASE(5) MG_CASE(6) MG_CASE(7) MG_CASE(8)
+#undef MG_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block) {
+  return (size_t)waves_per_block * ((size_t)G * kl * 80 + (size_t)G * 128) * sizeof(uint4);
+}
+
+hipError_t launch_stream(int S, int G, const StreamParams& p, int waves_per_block,
+                         hipStream_t s) {
+  const uint32_t n_groups = (p.n_keys + G - 1) / G;
+  const uint32_t total = p.parts * n_groups;
+  const dim3 block(64 * waves_per_block);
+  const dim3 grid((total + waves_per_block - 1) / waves_per_block);
+  const size_t lds = stream_lds_bytes(G, p.kl, waves_per_block);
+#define MG_SG(SS, GG)                                                         \
+  if (S == SS && G == GG) {                                                   \
+    hipLaunchKernelGGL((scan_stream<SS, GG>), grid, block, lds, s, p);        \
+    return hipGetLastError();                                                 \
+  }
+  MG_SG(1, 1) MG_SG(1, 2) MG_SG(1, 3) MG_SG(1, 4)
+  MG_SG(2, 1) MG_SG(2, 2) MG_SG(2, 3)
+  MG_SG(4, 1) MG_SG(4, 2)
+#undef MG_SG
+  return hipErrorInvalidValue;
+}
+
+hipError_t lane_dfs_blocks_per_cu(int chunks, int block_threads, size_t lds, int* blocks) {
+  switch (chunks) {
+#define MG_CASE(C)                                                                          \
+  case C:                                                                                   \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_lane_dfs<C>,           \
+                                                        block_threads, lds);
+    MG_CASE(1) MG_CASE(2) MG_CASE(3) MG_CASE(4) MG_CASE(5) MG_CASE(6) MG_CASE(7) MG_CASE(8)
+#undef MG_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t stream_blocks_per_cu(int S, int G, int block_threads, size_t lds, int* blocks) {
+#define MG_SG(SS, GG)                                                                       \
+  if (S == SS && G == GG)                                                                   \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_stream<SS, GG>,        \
+                                                        block_threads, lds);
+  MG_SG(1, 1) MG_SG(1, 2) MG_SG(1, 3) MG_SG(1, 4)
+  MG_SG(2, 1) MG_SG(2, 2) MG_SG(2, 3)
+  MG_SG(4, 1) MG_SG(4, 2)
+#undef MG_SG
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_point(const PointParams& p, hipStream_t s) {
+  const uint32_t entry_waves = (uint32_t)((p.n_entries + 63) / 64);
+  const dim3 block(256);
+  const dim3 grid((entry_waves + 3) / 4, p.n_keys);
+  hipLaunchKernelGGL(scan_point, grid, block, 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const uint4* partial, uint32_t parts, uint32_t n_keys, uint32_t chunks,
+                         uint4* out, bool accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_partials, dim3(chunks, n_keys), dim3(256), 0, s, partial, parts,
+                     n_keys, chunks, out, accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_splitmix(uint4* out, uint64_t first_pair, uint64_t n_pairs, uint64_t seed,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(fill_splitmix, dim3(grid_for(n_pairs, 256)), dim3(256), 0, s, out,
+                     first_pair, n_pairs, seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_lcg(uint4* out, uint64_t n_blocks16, uint64_t seed, hipStream_t s) {
+  hipLaunchKernelGGL(fill_lcg, dim3(grid_for(n_blocks16, 256)), dim3(256), 0, s, out, n_blocks16,
+                     seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(uint4* out, uint64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u4, dim3(grid_for(n, 256)), dim3(256), 0, s, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_xor_rows(uint4* rows, const uint64_t* idx, const uint4* diffs,
+                           uint64_t n_entries, uint32_t chunks, hipStream_t s) {
+  hipLaunchKernelGGL(xor_rows, dim3(grid_for(n_entries * chunks, 256)), dim3(256), 0, s, rows,
+                     idx, diffs, n_entries, chunks);
+  return hipGetLastError();
+}
+
+}  // namespace mg

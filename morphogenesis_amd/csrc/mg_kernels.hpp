@@ -1,0 +1,74 @@
+// Kernel parameter blocks and launchers shared by the kernels and the host
+// runtime. Host code sees only these declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace mg {
+
+struct DevKey;
+
+// Tree levels of a d-bit key are split as d = tb + lb + kl:
+//   tb  wave-uniform tile prefix (cached across consecutive tiles),
+//   lb  per-lane walk to the lane's subtree root (lanes >= 2^lb idle),
+//   kl  depth of each lane's subtree, expanded depth-first (kl >= 1).
+// A tile is 2^(lb + kl) consecutive rows.
+struct LaneDfsParams {
+  const uint4* rows;
+  uint64_t n_rows;
+  const DevKey* keys;
+  uint32_t n_keys;
+  uint32_t d, kl, lb, tb;
+  uint64_t tile_begin;  // tiles [tile_begin, tile_end) are scanned; rows
+  uint64_t tile_end;    // at or past n_rows are skipped
+  uint64_t tiles_per_wave;
+  uint32_t parts;  // waves per key; wave w serves key w % n_keys
+  uint4* partial;  // [parts][n_keys][chunks]
+};
+
+struct StreamParams {
+  const uint4* rows;
+  uint64_t n_rows;
+  uint32_t chunks;
+  const DevKey* keys;
+  uint32_t n_keys;
+  uint32_t d, kl, lb, tb;
+  uint64_t tile_begin;
+  uint64_t tile_end;
+  uint64_t tiles_per_wave;
+  uint32_t parts;  // waves per key group
+  uint4* partial;  // [parts][n_keys][chunks]
+};
+
+struct PointParams {
+  const uint4* rows;        // matrix rows, or delta diffs when row_idx != 0
+  const uint64_t* row_idx;  // delta row indices (nullptr for a matrix scan)
+  uint64_t first_row;       // matrix scan: entry e is row first_row + e
+  uint64_t n_entries;
+  uint32_t chunks;
+  const DevKey* keys;
+  uint32_t n_keys;
+  uint4* partial;  // [ceil(n_entries/64)][n_keys][chunks]
+};
+
+hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_block,
+                           hipStream_t s);
+hipError_t launch_stream(int S, int G, const StreamParams& p, int waves_per_block,
+                         hipStream_t s);
+size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block);
+// Resident blocks per CU for the given block size and dynamic LDS.
+hipError_t lane_dfs_blocks_per_cu(int chunks, int block_threads, size_t lds, int* blocks);
+hipError_t stream_blocks_per_cu(int S, int G, int block_threads, size_t lds, int* blocks);
+hipError_t launch_point(const PointParams& p, hipStream_t s);
+hipError_t launch_reduce(const uint4* partial, uint32_t parts, uint32_t n_keys, uint32_t chunks,
+                         uint4* out, bool accumulate, hipStream_t s);
+hipError_t launch_fill_splitmix(uint4* out, uint64_t first_pair, uint64_t n_pairs, uint64_t seed,
+                                hipStream_t s);
+hipError_t launch_fill_lcg(uint4* out, uint64_t n_blocks16, uint64_t seed, hipStream_t s);
+hipError_t launch_zero(uint4* out, uint64_t n, hipStream_t s);
+hipError_t launch_xor_rows(uint4* rows, const uint64_t* idx, const uint4* diffs,
+                           uint64_t n_entries, uint32_t chunks, hipStream_t s);
+
+}  // namespace mg

@@ -1,0 +1,905 @@
+// Host runtime behind include/morphogen_hip.h: device, matrix, key, delta and
+// scan entry points. It plays the role of the reference's GpuScanner /
+// GpuPageMatrix host driver (crates/morphogen-gpu-dpf/src/kernel.rs:156-1211,
+// src/storage.rs:18-145) on one gfx950 GPU: one non-blocking HIP stream per
+// device, a grow-only workspace, pinned staging buffers, HIP-event timing and
+// an optional HIP-graph replay of the launch sequence.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/morphogen_hip.h"
+#include "mg_kernels.hpp"
+#include "mg_key.hpp"
+
+using mg::DevKey;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? MG_ERR_OUT_OF_MEMORY : MG_ERR_DEVICE, "%s: %s (%d)",
+              what, hipGetErrorString(e), (int)e);
+}
+
+#define MG_HIP(call)                                   \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return hip_fail(_e, #call);  \
+  } while (0)
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Device buffer that only grows.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t bytes) {
+    if (bytes <= cap) return MG_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)4096);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return hip_fail(e, "hipMalloc(workspace)");
+    }
+    cap = want;
+    return MG_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t bytes) {
+    if (bytes <= cap) return MG_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)4096);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return hip_fail(e, "hipHostMalloc(staging)");
+    }
+    cap = want;
+    return MG_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Launch plan of one scan call; also the graph-cache signature (compared
+// bytewise, so the constructor clears padding too).
+struct Plan {
+  uint32_t kernel;
+  uint32_t chunks;
+  uint32_t n_keys;
+  uint32_t d, kl, lb, tb;
+  uint32_t keys_per_wave;
+  uint32_t waves_per_block;
+  uint32_t parts;
+  uint64_t tile_begin, tile_end, tiles_per_wave;
+  uint64_t row_begin, row_end;
+  uint64_t n_delta;
+  const void* rows;
+  const void* delta_idx;
+  const void* delta_rows;
+  uint32_t delta_parts;
+  // Workspace addresses baked into a captured graph.
+  const void* ws_keys;
+  const void* ws_out;
+  const void* ws_partial;
+  const void* ws_partial_delta;
+  Plan() {
+    memset(this, 0, sizeof *this);
+    keys_per_wave = 1;
+    waves_per_block = 4;
+  }
+  bool operator==(const Plan& o) const { return memcmp(this, &o, sizeof(Plan)) == 0; }
+};
+
+}  // namespace
+
+struct mg_device {
+  int ordinal = 0;
+  int cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[5] = {};
+  std::mutex mu;
+  DevBuf keys, partial, partial_delta, out;
+  HostBuf h_keys, h_out;
+  // HIP graph of the kernel sequence for the last plan.
+  bool graph_valid = false;
+  Plan graph_plan;
+  hipGraphExec_t graph_exec = nullptr;
+  std::map<std::pair<uint32_t, uint32_t>, int> occupancy;  // (kernel id, lds) -> blocks/CU
+};
+
+struct mg_matrix {
+  mg_device* dev = nullptr;
+  uint4* rows = nullptr;
+  uint64_t n_rows = 0;
+  uint32_t row_bytes = 0;
+};
+
+struct mg_delta {
+  mg_device* dev = nullptr;
+  uint32_t row_bytes = 0;
+  uint64_t capacity = 0;
+  uint64_t n = 0;
+  uint64_t epoch = 0;
+  uint64_t max_idx = 0;
+  uint64_t* d_idx = nullptr;
+  uint4* d_diffs = nullptr;
+  // Duplicate-free copy for the merge: each row index once, diffs XOR-ed.
+  uint64_t n_unique = 0;
+  uint64_t* d_uidx = nullptr;
+  uint4* d_udiffs = nullptr;
+};
+
+namespace {
+
+int set_device(mg_device* dev) {
+  hipError_t e = hipSetDevice(dev->ordinal);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return MG_OK;
+}
+
+int parse_keys(const uint8_t* keys, const size_t* key_lens, uint32_t n_keys, DevKey* out,
+               uint32_t* common_d) {
+  size_t off = 0;
+  for (uint32_t i = 0; i < n_keys; i++) {
+    uint32_t party = 0;
+    switch (mg::parse_wire_key(keys + off, key_lens[i], &out[i], &party)) {
+      case mg::kParseOk:
+        break;
+      case mg::kParseFormat:
+        return fail(MG_ERR_KEY_FORMAT, "key %u: invalid key format (length %zu)", i, key_lens[i]);
+      default:
+        return fail(MG_ERR_DOMAIN, "key %u: domain bits %u > %d", i, (unsigned)keys[off + 18],
+                    mg::kMaxDomainBits);
+    }
+    if (i == 0) *common_d = out[i].domain_bits;
+    if (out[i].domain_bits != *common_d)
+      return fail(MG_ERR_DOMAIN, "key %u has domain %u, key 0 has %u; group keys by domain", i,
+                  out[i].domain_bits, *common_d);
+    off += key_lens[i];
+  }
+  return MG_OK;
+}
+
+// Resident waves the device holds for a kernel variant (cached by `kind`).
+int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int chunks,
+                   int waves_per_block, size_t lds, int* out) {
+  auto key = std::make_pair(kind * 64 + (uint32_t)waves_per_block, (uint32_t)lds);
+  auto it = dev->occupancy.find(key);
+  int blocks = 0;
+  if (it != dev->occupancy.end()) {
+    blocks = it->second;
+  } else {
+    if (kind == MG_KERNEL_LANE_DFS * 100 + (uint32_t)chunks)
+      MG_HIP(mg::lane_dfs_blocks_per_cu(chunks, 64 * waves_per_block, lds, &blocks));
+    else
+      MG_HIP(mg::stream_blocks_per_cu(S, G, 64 * waves_per_block, lds, &blocks));
+    dev->occupancy[key] = blocks;
+  }
+  *out = std::max(1, blocks) * waves_per_block * dev->cus;
+  return MG_OK;
+}
+
+}  // namespace
+
+namespace {
+
+int max_keys_per_wave(int S) { return S == 1 ? 4 : (S == 2 ? 3 : 2); }
+
+// Splits a d-bit domain as tb + lb + kl (see LaneDfsParams).
+void split_levels(uint32_t d, uint32_t kl_req, Plan* pl) {
+  uint32_t kl = std::min(std::max(kl_req, 1u), d);
+  uint32_t lb = std::min(6u, d - kl);
+  pl->kl = kl;
+  pl->lb = lb;
+  pl->tb = d - kl - lb;
+}
+
+int make_plan(mg_device* dev, const mg_matrix* m, uint32_t n_keys, uint32_t d,
+              const mg_scan_opts* o, Plan* pl) {
+  mg_scan_opts def{};
+  if (!o) o = &def;
+  pl->chunks = m->row_bytes / 16;
+  pl->n_keys = n_keys;
+  pl->d = d;
+  pl->rows = m->rows;
+  pl->row_begin = o->row_begin;
+  pl->row_end = o->row_end ? o->row_end : m->n_rows;
+  if (pl->row_end > m->n_rows || pl->row_begin > pl->row_end)
+    return fail(MG_ERR_ROWS, "row range [%llu, %llu) outside the %llu-row matrix",
+                (unsigned long long)pl->row_begin, (unsigned long long)pl->row_end,
+                (unsigned long long)m->n_rows);
+  if (d < 64 && (m->n_rows > (1ull << d)))
+    return fail(MG_ERR_ROWS, "%llu rows exceed the 2^%u-leaf key domain",
+                (unsigned long long)m->n_rows, d);
+  if (o->waves_per_block > 4) return fail(MG_ERR_INVALID_ARG, "waves_per_block must be 0-4");
+  if (o->keys_per_wave > 4) return fail(MG_ERR_INVALID_ARG, "keys_per_wave must be 0-4");
+  if (o->lane_depth > 16) return fail(MG_ERR_INVALID_ARG, "lane_depth must be 0-16");
+
+  uint32_t kernel = o->kernel;
+  if (kernel > MG_KERNEL_POINT) return fail(MG_ERR_INVALID_ARG, "unknown kernel %u", kernel);
+  if (kernel == MG_KERNEL_AUTO) {
+    if (d == 0) kernel = MG_KERNEL_POINT;
+    else if (pl->chunks <= 8) kernel = MG_KERNEL_LANE_DFS;
+    else if (pl->chunks <= 256) kernel = MG_KERNEL_STREAM;
+    else kernel = MG_KERNEL_POINT;
+  }
+  if (kernel != MG_KERNEL_POINT && d == 0)
+    return fail(MG_ERR_INVALID_ARG, "a 0-bit domain needs the point kernel");
+  if (kernel == MG_KERNEL_LANE_DFS && pl->chunks > 8)
+    return fail(MG_ERR_INVALID_ARG, "lane-DFS kernel takes rows of at most 128 bytes");
+  if (kernel == MG_KERNEL_STREAM && pl->chunks > 256)
+    return fail(MG_ERR_INVALID_ARG, "stream kernel takes rows of at most 4096 bytes");
+  pl->kernel = kernel;
+  pl->waves_per_block = o->waves_per_block ? o->waves_per_block : 4;
+
+  if (kernel == MG_KERNEL_POINT) {
+    pl->parts = (uint32_t)((pl->row_end - pl->row_begin + 63) / 64);
+    return MG_OK;
+  }
+  split_levels(d, o->lane_depth ? o->lane_depth : (kernel == MG_KERNEL_LANE_DFS ? 8 : 6), pl);
+  const uint64_t tile_rows = 1ull << (pl->kl + pl->lb);
+  if (pl->row_begin % tile_rows != 0)
+    return fail(MG_ERR_ROWS, "row_begin %llu is not a multiple of the %llu-row tile",
+                (unsigned long long)pl->row_begin, (unsigned long long)tile_rows);
+  pl->tile_begin = pl->row_begin / tile_rows;
+  pl->tile_end = (pl->row_end + tile_rows - 1) / tile_rows;
+  const uint64_t n_tiles = pl->tile_end - pl->tile_begin;
+
+  if (kernel == MG_KERNEL_LANE_DFS && !o->waves_per_block)
+    while (pl->waves_per_block > 1 && (size_t)pl->waves_per_block * pl->kl * 80 * 16 > 64 * 1024)
+      pl->waves_per_block--;
+  uint32_t groups = n_keys;
+  int resident = 0;
+  int rc;
+  if (kernel == MG_KERNEL_LANE_DFS) {
+    const size_t lds = (size_t)pl->waves_per_block * pl->kl * 80 * 16;
+    rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl->chunks, 0, 0, (int)pl->chunks,
+                        (int)pl->waves_per_block, lds, &resident);
+  } else {
+    const int S = pl->chunks <= 64 ? 1 : (pl->chunks <= 128 ? 2 : 4);
+    int G = o->keys_per_wave ? (int)o->keys_per_wave : std::min<int>(3, (int)n_keys);
+    G = std::max(1, std::min(G, max_keys_per_wave(S)));
+    pl->keys_per_wave = (uint32_t)G;
+    groups = (n_keys + G - 1) / G;
+    if (!o->waves_per_block) pl->waves_per_block = 1;
+    const size_t lds = mg::stream_lds_bytes(G, pl->kl, pl->waves_per_block);
+    if (lds > 160 * 1024) return fail(MG_ERR_INVALID_ARG, "stream plan needs %zu B of LDS", lds);
+    rc = resident_waves(dev, MG_KERNEL_STREAM * 100 + (uint32_t)(S * 8 + G), S, G,
+                        (int)pl->chunks, (int)pl->waves_per_block, lds, &resident);
+  }
+  if (rc) return rc;
+  uint64_t parts = o->waves_per_key;
+  if (!parts) parts = std::max<uint64_t>(1, (uint64_t)resident / groups);
+  parts = std::min<uint64_t>(parts, n_tiles ? n_tiles : 1);
+  parts = std::min<uint64_t>(parts, 1u << 20);
+  pl->tiles_per_wave = (n_tiles + parts - 1) / parts;
+  if (pl->tiles_per_wave == 0) pl->tiles_per_wave = 1;
+  pl->parts = (uint32_t)((n_tiles + pl->tiles_per_wave - 1) / pl->tiles_per_wave);
+  if (pl->parts == 0) pl->parts = 1;
+  if ((uint64_t)pl->parts * groups > (1ull << 30))
+    return fail(MG_ERR_INVALID_ARG, "%u waves per key x %u key groups is too many", pl->parts,
+                groups);
+  return MG_OK;
+}
+
+// Enqueues the kernels of a plan on the device stream: scan, delta,
+// reductions into dev->out.
+int enqueue_plan(mg_device* dev, const Plan& pl, hipEvent_t ev_scan0, hipEvent_t ev_scan1) {
+  const DevKey* keys = (const DevKey*)dev->keys.p;
+  uint4* out = (uint4*)dev->out.p;
+  uint4* partial = (uint4*)dev->partial.p;
+  hipStream_t s = dev->stream;
+  if (ev_scan0) MG_HIP(hipEventRecord(ev_scan0, s));
+  if (pl.row_end > pl.row_begin) {
+    if (pl.kernel == MG_KERNEL_LANE_DFS) {
+      mg::LaneDfsParams p{};
+      p.rows = (const uint4*)pl.rows;
+      p.n_rows = pl.row_end;
+      p.keys = keys;
+      p.n_keys = pl.n_keys;
+      p.d = pl.d; p.kl = pl.kl; p.lb = pl.lb; p.tb = pl.tb;
+      p.tile_begin = pl.tile_begin;
+      p.tile_end = pl.tile_end;
+      p.tiles_per_wave = pl.tiles_per_wave;
+      p.parts = pl.parts;
+      p.partial = partial;
+      MG_HIP(mg::launch_lane_dfs((int)pl.chunks, p, (int)pl.waves_per_block, s));
+    } else if (pl.kernel == MG_KERNEL_STREAM) {
+      mg::StreamParams p{};
+      p.rows = (const uint4*)pl.rows;
+      p.n_rows = pl.row_end;
+      p.chunks = pl.chunks;
+      p.keys = keys;
+      p.n_keys = pl.n_keys;
+      p.d = pl.d; p.kl = pl.kl; p.lb = pl.lb; p.tb = pl.tb;
+      p.tile_begin = pl.tile_begin;
+      p.tile_end = pl.tile_end;
+      p.tiles_per_wave = pl.tiles_per_wave;
+      p.parts = pl.parts;
+      p.partial = partial;
+      const int S = pl.chunks <= 64 ? 1 : (pl.chunks <= 128 ? 2 : 4);
+      MG_HIP(mg::launch_stream(S, (int)pl.keys_per_wave, p, (int)pl.waves_per_block, s));
+    } else {
+      mg::PointParams p{};
+      p.rows = (const uint4*)pl.rows;
+      p.row_idx = nullptr;
+      p.first_row = pl.row_begin;
+      p.n_entries = pl.row_end - pl.row_begin;
+      p.chunks = pl.chunks;
+      p.keys = keys;
+      p.n_keys = pl.n_keys;
+      p.partial = partial;
+      MG_HIP(mg::launch_point(p, s));
+    }
+  }
+  if (ev_scan1) MG_HIP(hipEventRecord(ev_scan1, s));
+  if (pl.row_end > pl.row_begin) {
+    MG_HIP(mg::launch_reduce(partial, pl.parts, pl.n_keys, pl.chunks, out, false, s));
+  } else {
+    MG_HIP(mg::launch_zero(out, (uint64_t)pl.n_keys * pl.chunks, s));
+  }
+  if (pl.n_delta) {
+    mg::PointParams p{};
+    p.rows = (const uint4*)pl.delta_rows;
+    p.row_idx = (const uint64_t*)pl.delta_idx;
+    p.n_entries = pl.n_delta;
+    p.chunks = pl.chunks;
+    p.keys = keys;
+    p.n_keys = pl.n_keys;
+    p.partial = (uint4*)dev->partial_delta.p;
+    MG_HIP(mg::launch_point(p, s));
+    MG_HIP(mg::launch_reduce((const uint4*)dev->partial_delta.p, pl.delta_parts, pl.n_keys,
+                             pl.chunks, out, true, s));
+  }
+  return MG_OK;
+}
+
+int run_plan_graph(mg_device* dev, const Plan& pl) {
+  if (!(dev->graph_valid && dev->graph_plan == pl)) {
+    if (dev->graph_exec) (void)hipGraphExecDestroy(dev->graph_exec);
+    dev->graph_exec = nullptr;
+    dev->graph_valid = false;
+    MG_HIP(hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_plan(dev, pl, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(dev->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&dev->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    dev->graph_plan = pl;
+    dev->graph_valid = true;
+  }
+  MG_HIP(hipGraphLaunch(dev->graph_exec, dev->stream));
+  return MG_OK;
+}
+
+// One scan of n_keys parsed keys (already in dev->h_keys) into `out`.
+int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8_t* out,
+                const mg_scan_opts* opts, mg_timing* t) {
+  const double t0 = now_ms();
+  Plan pl;
+  int rc = make_plan(dev, m, n_keys, d, opts, &pl);
+  if (rc) return rc;
+  const mg_delta* dl = opts ? opts->delta : nullptr;
+  if (dl) {
+    if (dl->row_bytes != m->row_bytes)
+      return fail(MG_ERR_INVALID_ARG, "delta rows are %u B, matrix rows %u B", dl->row_bytes,
+                  m->row_bytes);
+    if (dl->dev != dev) return fail(MG_ERR_INVALID_ARG, "delta belongs to another device");
+    if (dl->n && dl->max_idx >= m->n_rows)
+      return fail(MG_ERR_ROWS, "delta row %llu outside the %llu-row matrix",
+                  (unsigned long long)dl->max_idx, (unsigned long long)m->n_rows);
+    pl.n_delta = dl->n;
+    pl.delta_idx = dl->d_idx;
+    pl.delta_rows = dl->d_diffs;
+    pl.delta_parts = (uint32_t)((dl->n + 63) / 64);
+  }
+  const size_t out_bytes = (size_t)n_keys * m->row_bytes;
+  if ((rc = dev->keys.reserve(sizeof(DevKey) * n_keys))) return rc;
+  if ((rc = dev->out.reserve(out_bytes))) return rc;
+  if ((rc = dev->partial.reserve((size_t)pl.parts * n_keys * m->row_bytes))) return rc;
+  if (pl.n_delta &&
+      (rc = dev->partial_delta.reserve((size_t)pl.delta_parts * n_keys * m->row_bytes)))
+    return rc;
+  if ((rc = dev->h_out.reserve(out_bytes))) return rc;
+  pl.ws_keys = dev->keys.p;
+  pl.ws_out = dev->out.p;
+  pl.ws_partial = dev->partial.p;
+  pl.ws_partial_delta = pl.n_delta ? dev->partial_delta.p : nullptr;
+
+  hipStream_t s = dev->stream;
+  MG_HIP(hipEventRecord(dev->ev[0], s));
+  MG_HIP(hipMemcpyAsync(dev->keys.p, dev->h_keys.p, sizeof(DevKey) * n_keys,
+                        hipMemcpyHostToDevice, s));
+  if (opts && opts->use_graph) {
+    MG_HIP(hipEventRecord(dev->ev[1], s));
+    if ((rc = run_plan_graph(dev, pl))) return rc;
+    MG_HIP(hipEventRecord(dev->ev[2], s));
+  } else {
+    if ((rc = enqueue_plan(dev, pl, dev->ev[1], dev->ev[2]))) return rc;
+  }
+  MG_HIP(hipEventRecord(dev->ev[3], s));
+  MG_HIP(hipMemcpyAsync(dev->h_out.p, dev->out.p, out_bytes, hipMemcpyDeviceToHost, s));
+  MG_HIP(hipEventRecord(dev->ev[4], s));
+  MG_HIP(hipStreamSynchronize(s));
+  memcpy(out, dev->h_out.p, out_bytes);
+  if (t) {
+    float h2d = 0, scan = 0, kern = 0, d2h = 0;
+    MG_HIP(hipEventElapsedTime(&h2d, dev->ev[0], dev->ev[1]));
+    MG_HIP(hipEventElapsedTime(&scan, dev->ev[1], dev->ev[2]));
+    MG_HIP(hipEventElapsedTime(&kern, dev->ev[1], dev->ev[3]));
+    MG_HIP(hipEventElapsedTime(&d2h, dev->ev[3], dev->ev[4]));
+    t->h2d_ms = h2d;
+    t->scan_kernel_ms = scan;
+    t->kernel_ms = kern;
+    t->d2h_ms = d2h;
+    t->total_ms = now_ms() - t0;
+    t->kernel = pl.kernel;
+    t->lane_depth = pl.kl;
+    t->keys_per_wave = pl.kernel == MG_KERNEL_STREAM ? pl.keys_per_wave : 1;
+    t->waves = pl.kernel == MG_KERNEL_POINT
+                   ? pl.parts * n_keys
+                   : pl.parts * (pl.kernel == MG_KERNEL_STREAM
+                                     ? (n_keys + pl.keys_per_wave - 1) / pl.keys_per_wave
+                                     : n_keys);
+  }
+  return MG_OK;
+}
+
+int stage_keys(mg_device* dev, const uint8_t* keys, const size_t* key_lens, uint32_t n_keys,
+               uint32_t* d) {
+  int rc = dev->h_keys.reserve(sizeof(DevKey) * std::max<uint32_t>(n_keys, 1));
+  if (rc) return rc;
+  return parse_keys(keys, key_lens, n_keys, (DevKey*)dev->h_keys.p, d);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- ABI --
+
+extern "C" {
+
+int mg_abi_version(void) { return MG_ABI_VERSION; }
+
+const char* mg_last_error(void) { return g_err.c_str(); }
+
+int mg_device_open(int ordinal, mg_device** out) {
+  if (!out) return fail(MG_ERR_INVALID_ARG, "out is null");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(MG_ERR_NO_DEVICE, "no HIP device available");
+  if (ordinal < 0 || ordinal >= count)
+    return fail(MG_ERR_NO_DEVICE, "device %d not present (%d devices)", ordinal, count);
+  MG_HIP(hipSetDevice(ordinal));
+  hipDeviceProp_t prop;
+  MG_HIP(hipGetDeviceProperties(&prop, ordinal));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(MG_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", ordinal,
+                prop.gcnArchName);
+  mg_device* dev = new mg_device();
+  dev->ordinal = ordinal;
+  dev->cus = prop.multiProcessorCount;
+  hipError_t se = hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking);
+  if (se != hipSuccess) {
+    delete dev;
+    return hip_fail(se, "hipStreamCreateWithFlags");
+  }
+  for (auto& ev : dev->ev) {
+    hipError_t ee = hipEventCreate(&ev);
+    if (ee != hipSuccess) {
+      mg_device_close(dev);
+      return hip_fail(ee, "hipEventCreate");
+    }
+  }
+  *out = dev;
+  return MG_OK;
+}
+
+int mg_device_close(mg_device* dev) {
+  if (!dev) return MG_OK;
+  (void)hipSetDevice(dev->ordinal);
+  if (dev->stream) (void)hipStreamSynchronize(dev->stream);
+  if (dev->graph_exec) (void)hipGraphExecDestroy(dev->graph_exec);
+  dev->keys.release();
+  dev->partial.release();
+  dev->partial_delta.release();
+  dev->out.release();
+  dev->h_keys.release();
+  dev->h_out.release();
+  for (auto& ev : dev->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (dev->stream) (void)hipStreamDestroy(dev->stream);
+  delete dev;
+  return MG_OK;
+}
+
+int mg_device_info(mg_device* dev, uint64_t* free_bytes, uint64_t* total_bytes,
+                   uint32_t* compute_units, char* name, size_t name_len) {
+  if (!dev) return fail(MG_ERR_INVALID_ARG, "device is null");
+  int rc = set_device(dev);
+  if (rc) return rc;
+  size_t fr = 0, tot = 0;
+  MG_HIP(hipMemGetInfo(&fr, &tot));
+  if (free_bytes) *free_bytes = fr;
+  if (total_bytes) *total_bytes = tot;
+  if (compute_units) *compute_units = (uint32_t)dev->cus;
+  if (name && name_len) {
+    hipDeviceProp_t prop;
+    MG_HIP(hipGetDeviceProperties(&prop, dev->ordinal));
+    snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  return MG_OK;
+}
+
+int mg_matrix_create(mg_device* dev, uint64_t n_rows, uint32_t row_bytes, mg_matrix** out) {
+  if (!dev || !out) return fail(MG_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (row_bytes == 0 || row_bytes % 16 != 0)
+    return fail(MG_ERR_ROWS, "row width %u is not a positive multiple of 16", row_bytes);
+  if (n_rows > (1ull << 32)) return fail(MG_ERR_ROWS, "more than 2^32 rows");
+  std::lock_guard<std::mutex> lk(dev->mu);
+  int rc = set_device(dev);
+  if (rc) return rc;
+  mg_matrix* m = new mg_matrix();
+  m->dev = dev;
+  m->n_rows = n_rows;
+  m->row_bytes = row_bytes;
+  const size_t bytes = std::max<size_t>((size_t)n_rows * row_bytes, 16);
+  hipError_t e = hipMalloc((void**)&m->rows, bytes);
+  if (e != hipSuccess) {
+    delete m;
+    return hip_fail(e, "hipMalloc(matrix)");
+  }
+  rc = MG_OK;
+  if (mg::launch_zero(m->rows, bytes / 16, dev->stream) != hipSuccess ||
+      hipStreamSynchronize(dev->stream) != hipSuccess) {
+    (void)hipFree(m->rows);
+    delete m;
+    return fail(MG_ERR_DEVICE, "zeroing the matrix failed");
+  }
+  *out = m;
+  return MG_OK;
+}
+
+int mg_matrix_upload(mg_matrix* m, uint64_t first_row, const void* rows, uint64_t n_rows) {
+  if (!m || (!rows && n_rows)) return fail(MG_ERR_INVALID_ARG, "null argument");
+  if (first_row > m->n_rows || n_rows > m->n_rows - first_row)
+    return fail(MG_ERR_ROWS, "rows [%llu, +%llu) exceed the %llu-row matrix",
+                (unsigned long long)first_row, (unsigned long long)n_rows,
+                (unsigned long long)m->n_rows);
+  std::lock_guard<std::mutex> lk(m->dev->mu);
+  int rc = set_device(m->dev);
+  if (rc) return rc;
+  MG_HIP(hipMemcpyAsync((uint8_t*)m->rows + first_row * m->row_bytes, rows,
+                        (size_t)n_rows * m->row_bytes, hipMemcpyHostToDevice, m->dev->stream));
+  MG_HIP(hipStreamSynchronize(m->dev->stream));
+  return MG_OK;
+}
+
+int mg_matrix_download(mg_matrix* m, uint64_t first_row, void* rows, uint64_t n_rows) {
+  if (!m || (!rows && n_rows)) return fail(MG_ERR_INVALID_ARG, "null argument");
+  if (first_row > m->n_rows || n_rows > m->n_rows - first_row)
+    return fail(MG_ERR_ROWS, "rows [%llu, +%llu) exceed the %llu-row matrix",
+                (unsigned long long)first_row, (unsigned long long)n_rows,
+                (unsigned long long)m->n_rows);
+  std::lock_guard<std::mutex> lk(m->dev->mu);
+  int rc = set_device(m->dev);
+  if (rc) return rc;
+  MG_HIP(hipMemcpyAsync(rows, (const uint8_t*)m->rows + first_row * m->row_bytes,
+                        (size_t)n_rows * m->row_bytes, hipMemcpyDeviceToHost, m->dev->stream));
+  MG_HIP(hipStreamSynchronize(m->dev->stream));
+  return MG_OK;
+}
+
+int mg_matrix_fill_seeded(mg_matrix* m, uint64_t seed) {
+  if (!m) return fail(MG_ERR_INVALID_ARG, "matrix is null");
+  std::lock_guard<std::mutex> lk(m->dev->mu);
+  int rc = set_device(m->dev);
+  if (rc) return rc;
+  const uint64_t pairs = (uint64_t)m->n_rows * m->row_bytes / 16;
+  if (pairs) MG_HIP(mg::launch_fill_splitmix(m->rows, 0, pairs, seed, m->dev->stream));
+  MG_HIP(hipStreamSynchronize(m->dev->stream));
+  return MG_OK;
+}
+
+int mg_matrix_fill_pattern(mg_matrix* m, uint64_t seed) {
+  if (!m) return fail(MG_ERR_INVALID_ARG, "matrix is null");
+  std::lock_guard<std::mutex> lk(m->dev->mu);
+  int rc = set_device(m->dev);
+  if (rc) return rc;
+  const uint64_t blocks = (uint64_t)m->n_rows * m->row_bytes / 16;
+  if (blocks) MG_HIP(mg::launch_fill_lcg(m->rows, blocks, seed, m->dev->stream));
+  MG_HIP(hipStreamSynchronize(m->dev->stream));
+  return MG_OK;
+}
+
+int mg_matrix_info(const mg_matrix* m, uint64_t* n_rows, uint32_t* row_bytes) {
+  if (!m) return fail(MG_ERR_INVALID_ARG, "matrix is null");
+  if (n_rows) *n_rows = m->n_rows;
+  if (row_bytes) *row_bytes = m->row_bytes;
+  return MG_OK;
+}
+
+int mg_matrix_destroy(mg_matrix* m) {
+  if (!m) return MG_OK;
+  std::lock_guard<std::mutex> lk(m->dev->mu);
+  set_device(m->dev);
+  (void)hipStreamSynchronize(m->dev->stream);
+  if (m->dev->graph_exec) {
+    (void)hipGraphExecDestroy(m->dev->graph_exec);
+    m->dev->graph_exec = nullptr;
+    m->dev->graph_valid = false;
+  }
+  (void)hipFree(m->rows);
+  delete m;
+  return MG_OK;
+}
+
+int mg_key_parse(const uint8_t* bytes, size_t len, mg_key_info* out) {
+  if (!bytes || !out) return fail(MG_ERR_INVALID_ARG, "null argument");
+  DevKey k;
+  uint32_t party = 0;
+  switch (mg::parse_wire_key(bytes, len, &k, &party)) {
+    case mg::kParseOk:
+      break;
+    case mg::kParseFormat:
+      return fail(MG_ERR_KEY_FORMAT, "invalid key format (length %zu)", len);
+    default:
+      return fail(MG_ERR_DOMAIN, "domain bits %u > %d", (unsigned)bytes[18], mg::kMaxDomainBits);
+  }
+  out->domain_bits = k.domain_bits;
+  out->party = party;
+  out->root_t = k.root_t;
+  out->wire_len = (uint32_t)len;
+  return MG_OK;
+}
+
+int mg_scan(mg_device* dev, mg_matrix* m, const uint8_t* keys, const size_t* key_lens,
+            uint32_t n_keys, uint8_t* out, const mg_scan_opts* opts, mg_timing* timing) {
+  if (!dev || !m) return fail(MG_ERR_INVALID_ARG, "null device or matrix");
+  if (m->dev != dev) return fail(MG_ERR_INVALID_ARG, "matrix belongs to another device");
+  if (n_keys == 0) return MG_OK;
+  if (!keys || !key_lens || !out) return fail(MG_ERR_INVALID_ARG, "null keys or output");
+  std::lock_guard<std::mutex> lk(dev->mu);
+  int rc = set_device(dev);
+  if (rc) return rc;
+  uint32_t d = 0;
+  if ((rc = stage_keys(dev, keys, key_lens, n_keys, &d))) return rc;
+  return scan_locked(dev, m, n_keys, d, out, opts, timing);
+}
+
+int mg_scan_per_query(mg_device* dev, mg_matrix* m, const uint8_t* keys, const size_t* key_lens,
+                      uint32_t n_keys, uint32_t keys_per_query, uint8_t* out,
+                      const mg_scan_opts* opts, mg_timing* timing) {
+  if (keys_per_query == 0) return fail(MG_ERR_INVALID_ARG, "keys_per_query is 0");
+  if (!key_lens && n_keys) return fail(MG_ERR_INVALID_ARG, "null key lengths");
+  mg_timing acc{};
+  size_t off = 0;
+  for (uint32_t q = 0; q < n_keys; q += keys_per_query) {
+    const uint32_t nk = std::min(keys_per_query, n_keys - q);
+    mg_timing t{};
+    int rc = mg_scan(dev, m, keys + off, key_lens + q, nk, out + (size_t)q * m->row_bytes, opts,
+                     &t);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < nk; i++) off += key_lens[q + i];
+    acc.h2d_ms += t.h2d_ms;
+    acc.kernel_ms += t.kernel_ms;
+    acc.d2h_ms += t.d2h_ms;
+    acc.total_ms += t.total_ms;
+    acc.scan_kernel_ms += t.scan_kernel_ms;
+    acc.kernel = t.kernel;
+    acc.lane_depth = t.lane_depth;
+    acc.keys_per_wave = t.keys_per_wave;
+    acc.waves = std::max(acc.waves, t.waves);
+  }
+  if (timing) *timing = acc;
+  return MG_OK;
+}
+
+int mg_delta_create(mg_device* dev, uint32_t row_bytes, uint64_t capacity, mg_delta** out) {
+  if (!dev || !out) return fail(MG_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (row_bytes == 0 || row_bytes % 16 != 0)
+    return fail(MG_ERR_ROWS, "row width %u is not a positive multiple of 16", row_bytes);
+  std::lock_guard<std::mutex> lk(dev->mu);
+  int rc = set_device(dev);
+  if (rc) return rc;
+  mg_delta* dl = new mg_delta();
+  dl->dev = dev;
+  dl->row_bytes = row_bytes;
+  dl->capacity = capacity;
+  const size_t cap = std::max<uint64_t>(capacity, 1);
+  hipError_t e = hipMalloc((void**)&dl->d_idx, cap * 8);
+  if (e == hipSuccess) e = hipMalloc((void**)&dl->d_diffs, cap * row_bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&dl->d_uidx, cap * 8);
+  if (e == hipSuccess) e = hipMalloc((void**)&dl->d_udiffs, cap * row_bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(dl->d_idx);
+    (void)hipFree(dl->d_diffs);
+    (void)hipFree(dl->d_uidx);
+    (void)hipFree(dl->d_udiffs);
+    delete dl;
+    return hip_fail(e, "hipMalloc(delta)");
+  }
+  *out = dl;
+  return MG_OK;
+}
+
+int mg_delta_set(mg_delta* dl, const uint64_t* row_idx, const uint8_t* diffs, uint64_t n_entries,
+                 uint64_t epoch) {
+  if (!dl || (n_entries && (!row_idx || !diffs))) return fail(MG_ERR_INVALID_ARG, "null argument");
+  if (n_entries > dl->capacity)
+    return fail(MG_ERR_INVALID_ARG, "%llu entries exceed the delta capacity %llu",
+                (unsigned long long)n_entries, (unsigned long long)dl->capacity);
+  // Duplicate-free copy for merging: row -> XOR of its diffs.
+  std::map<uint64_t, std::vector<uint8_t>> uniq;
+  uint64_t mx = 0;
+  for (uint64_t i = 0; i < n_entries; i++) {
+    mx = std::max(mx, row_idx[i]);
+    auto& v = uniq[row_idx[i]];
+    if (v.empty()) v.assign(dl->row_bytes, 0);
+    const uint8_t* src = diffs + i * dl->row_bytes;
+    for (uint32_t b = 0; b < dl->row_bytes; b++) v[b] ^= src[b];
+  }
+  std::vector<uint64_t> uidx;
+  std::vector<uint8_t> udiff;
+  uidx.reserve(uniq.size());
+  udiff.reserve(uniq.size() * dl->row_bytes);
+  for (auto& kv : uniq) {
+    uidx.push_back(kv.first);
+    udiff.insert(udiff.end(), kv.second.begin(), kv.second.end());
+  }
+  std::lock_guard<std::mutex> lk(dl->dev->mu);
+  int rc = set_device(dl->dev);
+  if (rc) return rc;
+  hipStream_t s = dl->dev->stream;
+  if (n_entries) {
+    MG_HIP(hipMemcpyAsync(dl->d_idx, row_idx, n_entries * 8, hipMemcpyHostToDevice, s));
+    MG_HIP(hipMemcpyAsync(dl->d_diffs, diffs, n_entries * dl->row_bytes, hipMemcpyHostToDevice, s));
+    MG_HIP(hipMemcpyAsync(dl->d_uidx, uidx.data(), uidx.size() * 8, hipMemcpyHostToDevice, s));
+    MG_HIP(hipMemcpyAsync(dl->d_udiffs, udiff.data(), udiff.size(), hipMemcpyHostToDevice, s));
+  }
+  MG_HIP(hipStreamSynchronize(s));
+  dl->n = n_entries;
+  dl->n_unique = uidx.size();
+  dl->max_idx = mx;
+  dl->epoch = epoch;
+  return MG_OK;
+}
+
+int mg_delta_info(const mg_delta* dl, uint64_t* n_entries, uint64_t* epoch) {
+  if (!dl) return fail(MG_ERR_INVALID_ARG, "delta is null");
+  if (n_entries) *n_entries = dl->n;
+  if (epoch) *epoch = dl->epoch;
+  return MG_OK;
+}
+
+int mg_delta_destroy(mg_delta* dl) {
+  if (!dl) return MG_OK;
+  std::lock_guard<std::mutex> lk(dl->dev->mu);
+  set_device(dl->dev);
+  (void)hipStreamSynchronize(dl->dev->stream);
+  if (dl->dev->graph_exec) {
+    (void)hipGraphExecDestroy(dl->dev->graph_exec);
+    dl->dev->graph_exec = nullptr;
+    dl->dev->graph_valid = false;
+  }
+  (void)hipFree(dl->d_idx);
+  (void)hipFree(dl->d_diffs);
+  (void)hipFree(dl->d_uidx);
+  (void)hipFree(dl->d_udiffs);
+  delete dl;
+  return MG_OK;
+}
+
+int mg_scan_delta(mg_device* dev, const mg_delta* dl, const uint8_t* keys, const size_t* key_lens,
+                  uint32_t n_keys, uint8_t* out) {
+  if (!dev || !dl) return fail(MG_ERR_INVALID_ARG, "null device or delta");
+  if (dl->dev != dev) return fail(MG_ERR_INVALID_ARG, "delta belongs to another device");
+  if (n_keys == 0) return MG_OK;
+  if (!keys || !key_lens || !out) return fail(MG_ERR_INVALID_ARG, "null keys or output");
+  std::lock_guard<std::mutex> lk(dev->mu);
+  int rc = set_device(dev);
+  if (rc) return rc;
+  uint32_t d = 0;
+  if ((rc = stage_keys(dev, keys, key_lens, n_keys, &d))) return rc;
+  if (d < 64 && dl->n && (dl->max_idx >> d) != 0)
+    return fail(MG_ERR_ROWS, "delta row %llu outside the 2^%u-leaf domain",
+                (unsigned long long)dl->max_idx, d);
+  const uint32_t chunks = dl->row_bytes / 16;
+  const size_t out_bytes = (size_t)n_keys * dl->row_bytes;
+  const uint32_t parts = (uint32_t)((dl->n + 63) / 64);
+  if ((rc = dev->keys.reserve(sizeof(DevKey) * n_keys))) return rc;
+  if ((rc = dev->out.reserve(out_bytes))) return rc;
+  if ((rc = dev->partial_delta.reserve((size_t)std::max(parts, 1u) * out_bytes))) return rc;
+  if ((rc = dev->h_out.reserve(out_bytes))) return rc;
+  hipStream_t s = dev->stream;
+  MG_HIP(hipMemcpyAsync(dev->keys.p, dev->h_keys.p, sizeof(DevKey) * n_keys,
+                        hipMemcpyHostToDevice, s));
+  MG_HIP(mg::launch_zero((uint4*)dev->out.p, (uint64_t)n_keys * chunks, s));
+  if (dl->n) {
+    mg::PointParams p{};
+    p.rows = dl->d_diffs;
+    p.row_idx = dl->d_idx;
+    p.n_entries = dl->n;
+    p.chunks = chunks;
+    p.keys = (const DevKey*)dev->keys.p;
+    p.n_keys = n_keys;
+    p.partial = (uint4*)dev->partial_delta.p;
+    MG_HIP(mg::launch_point(p, s));
+    MG_HIP(mg::launch_reduce((const uint4*)dev->partial_delta.p, parts, n_keys, chunks,
+                             (uint4*)dev->out.p, false, s));
+  }
+  MG_HIP(hipMemcpyAsync(dev->h_out.p, dev->out.p, out_bytes, hipMemcpyDeviceToHost, s));
+  MG_HIP(hipStreamSynchronize(s));
+  memcpy(out, dev->h_out.p, out_bytes);
+  return MG_OK;
+}
+
+int mg_matrix_apply_delta(mg_matrix* m, const mg_delta* dl) {
+  if (!m || !dl) return fail(MG_ERR_INVALID_ARG, "null argument");
+  if (dl->dev != m->dev) return fail(MG_ERR_INVALID_ARG, "delta belongs to another device");
+  if (dl->row_bytes != m->row_bytes)
+    return fail(MG_ERR_INVALID_ARG, "delta rows are %u B, matrix rows %u B", dl->row_bytes,
+                m->row_bytes);
+  if (dl->n && dl->max_idx >= m->n_rows)
+    return fail(MG_ERR_ROWS, "delta row %llu outside the %llu-row matrix",
+                (unsigned long long)dl->max_idx, (unsigned long long)m->n_rows);
+  std::lock_guard<std::mutex> lk(m->dev->mu);
+  int rc = set_device(m->dev);
+  if (rc) return rc;
+  if (dl->n_unique)
+    MG_HIP(mg::launch_xor_rows(m->rows, dl->d_uidx, dl->d_udiffs, dl->n_unique,
+                               m->row_bytes / 16, m->dev->stream));
+  MG_HIP(hipStreamSynchronize(m->dev->stream));
+  return MG_OK;
+}
+
+}  // extern "C"
