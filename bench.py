@@ -59,7 +59,7 @@ def cpu_baseline(rows_log2, row_bytes, domain_bits, sample_rows_log2, seed):
     keys = [oracle.keygen_bytes(domain_bits, (i * 2654435761) % n, seed=100 + i)[0]
             for i in range(KEYS_PER_QUERY)]
     times = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         oracle.scan_tiled(keys, rows, threads)
         times.append(time.perf_counter() - t0)
@@ -74,7 +74,7 @@ def cpu_baseline(rows_log2, row_bytes, domain_bits, sample_rows_log2, seed):
         pass
     return {"value": q_per_s, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"oracle scan_tiled, 3 keys (1 query) over the first 2^{sample_rows_log2} of "
-                      f"2^{rows_log2} rows x {row_bytes} B, median of 3 ({t:.2f} s), scaled "
+                      f"2^{rows_log2} rows x {row_bytes} B, median of 5 ({t:.2f} s each), scaled "
                       f"x{full_scale:.0f} to the full matrix; cpu: {model}"}
 
 
@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--lane-depth", type=int, default=0)
     ap.add_argument("--waves-per-key", type=int, default=0)
     ap.add_argument("--use-graph", action="store_true")
-    ap.add_argument("--cpu-sample-log2", type=int, default=25)
+    ap.add_argument("--cpu-sample-log2", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=31)
     args = ap.parse_args()

@@ -277,3 +277,29 @@ def test_c2_shape_two_party_recovery(mg, scanner):
     got = scanner.scan_keys(m, [k0, k1], mg.make_opts(row_end=sub))
     assert np.array_equal(got, oracle.scan_tiled([k0, k1], rows, THREADS))
     m.close()
+
+
+def test_golden_fixtures(mg, scanner):
+    import json
+    with open(os.path.join(os.path.dirname(__file__), "golden", "scan_fixtures.json")) as f:
+        fx = json.load(f)
+    for c in fx["scan_cases"]:
+        print("fixture", c["n_rows"], c["row_bytes"], c["domain_bits"], flush=True)
+        m = mg.GpuPageMatrix(scanner, c["n_rows"], c["row_bytes"])
+        m.fill_seeded(c["data_seed"])
+        keys = [bytes.fromhex(k) for k in c["keys"]]
+        got = scanner.scan_keys(m, keys)
+        assert [a.tobytes().hex() for a in got] == c["answers"]
+        m.close()
+    for c in fx["delta_cases"]:
+        m = mg.GpuPageMatrix(scanner, c["n_rows"], c["row_bytes"])
+        m.fill_seeded(c["data_seed"])
+        keys = [bytes.fromhex(k) for k in c["keys"]]
+        diffs = np.frombuffer(bytes.fromhex(c["diffs"]), np.uint8).reshape(-1, c["row_bytes"])
+        dl = mg.DeltaBuffer(scanner, c["row_bytes"], len(c["row_idx"]))
+        dl.set(np.array(c["row_idx"], dtype=np.uint64), diffs)
+        assert [a.tobytes().hex() for a in scanner.scan_delta(dl, keys)] == c["answers"]
+        got = scanner.scan_keys(m, keys, mg.make_opts(delta=dl))
+        assert [a.tobytes().hex() for a in got] == c["merged_answers"]
+        dl.close()
+        m.close()
