@@ -46,7 +46,7 @@ def percentile(xs, p):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-def cpu_baseline(rows_log2, row_bytes, domain_bits, sample_rows_log2, seed):
+def cpu_baseline(n_rows, row_bytes, domain_bits, sample_rows_log2, seed):
     """Oracle tiled scan (C restatement of eval_fused_3dpf_cpu) on a prefix
     of the same matrix; throughput scaled to the full row count."""
     import numpy as np
@@ -64,7 +64,7 @@ def cpu_baseline(rows_log2, row_bytes, domain_bits, sample_rows_log2, seed):
         oracle.scan_tiled(keys, rows, threads)
         times.append(time.perf_counter() - t0)
     t = statistics.median(times)
-    full_scale = (1 << rows_log2) / n
+    full_scale = n_rows / n
     q_per_s = 1.0 / (t * full_scale)
     model = ""
     try:
@@ -74,8 +74,8 @@ def cpu_baseline(rows_log2, row_bytes, domain_bits, sample_rows_log2, seed):
         pass
     return {"value": q_per_s, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"oracle scan_tiled, 3 keys (1 query) over the first 2^{sample_rows_log2} of "
-                      f"2^{rows_log2} rows x {row_bytes} B, median of 5 ({t:.2f} s each), scaled "
-                      f"x{full_scale:.0f} to the full matrix; cpu: {model}"}
+                      f"{n_rows} rows x {row_bytes} B, median of 5 ({t:.2f} s each), scaled "
+                      f"x{full_scale:.1f} to the full matrix; cpu: {model}"}
 
 
 def main():
@@ -84,6 +84,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rows-log2", type=int, default=31)
+    ap.add_argument("--rows", type=int, default=0, help="explicit row count (overrides --rows-log2)")
+    ap.add_argument("--kernel", choices=["auto", "lane_dfs", "stream", "point"], default="auto")
+    ap.add_argument("--keys-per-wave", type=int, default=0)
+    ap.add_argument("--waves-per-block", type=int, default=0)
     ap.add_argument("--row-bytes", type=int, default=32)
     ap.add_argument("--domain-bits", type=int, default=0, help="0 = rows-log2")
     ap.add_argument("--keys", type=int, default=24, help="keys per batched pass (even)")
@@ -98,7 +102,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the scan shards by row "
+        print(json.dumps({"error": "this benchmark runs on one GPU only; the scan shards by row "
                                    "range across GPUs by design but no multi-GPU path is built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
@@ -107,9 +111,9 @@ def main():
 
     import morphogenesis_amd as mg
 
-    n_rows = 1 << args.rows_log2
+    n_rows = args.rows or (1 << args.rows_log2)
     w = args.row_bytes
-    d = args.domain_bits or args.rows_log2
+    d = args.domain_bits or max(1, (n_rows - 1).bit_length())
     q = max(2, args.keys - args.keys % 2)
     scanner = mg.GpuScanner(0)
     info = scanner.info()
@@ -126,8 +130,11 @@ def main():
     pairs = [mg.generate_keys(d, t, seed=1000 + i) for i, t in enumerate(targets)]
     keys = [k for p in pairs for k in p]
     target_rows = [m.download(t, 1)[0] for t in targets]
-    opts = mg.make_opts(lane_depth=args.lane_depth, waves_per_key=args.waves_per_key,
-                        use_graph=args.use_graph)
+    kernel = {"auto": mg.KERNEL_AUTO, "lane_dfs": mg.KERNEL_LANE_DFS,
+              "stream": mg.KERNEL_STREAM, "point": mg.KERNEL_POINT}[args.kernel]
+    opts = mg.make_opts(kernel=kernel, lane_depth=args.lane_depth,
+                        keys_per_wave=args.keys_per_wave, waves_per_key=args.waves_per_key,
+                        waves_per_block=args.waves_per_block, use_graph=args.use_graph)
 
     def check(ans):
         for i, t in enumerate(targets):
@@ -174,15 +181,17 @@ def main():
     if os.path.exists(tfile):
         try:
             with open(tfile) as f:
-                traffic = json.load(f).get(f"{args.rows_log2}x{w}x{q}")
+                traffic = json.load(f).get(f"{n_rows}x{w}x{q}")
         except Exception:
             traffic = None
 
+    rows_label = f"2^{n_rows.bit_length() - 1}" if n_rows & (n_rows - 1) == 0 else str(n_rows)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(args.rows_log2, w, d, min(args.cpu_sample_log2, args.rows_log2),
-                           args.seed)
+        # Sample at most 4 GiB of host rows.
+        sample_log2 = min(args.cpu_sample_log2, n_rows.bit_length() - 1, 32 - (w - 1).bit_length())
+        cpu = cpu_baseline(n_rows, w, d, sample_log2, args.seed)
         log(f"cpu baseline {cpu['value']:.4f} queries/s on {cpu['cores']} threads")
 
     out = {
@@ -199,7 +208,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic: splitmix64-seeded uniform rows generated in HBM, seeded DPF keygen, "
                 "uniform targets",
-        "config": {"workload": f"2^{args.rows_log2} rows x {w} B, {q} DPF keys per pass over a "
+        "config": {"workload": f"{rows_label} rows x {w} B, {q} DPF keys per pass over a "
                                f"{d}-bit domain ({q // 2} two-party pairs)",
                    "rows": n_rows, "row_bytes": w, "domain_bits": d, "keys_per_pass": q,
                    "kernel": timing["kernel"], "lane_depth": timing["lane_depth"],

@@ -167,6 +167,64 @@ int mg_scan_delta(mg_device *dev, const mg_delta *dl, const uint8_t *keys,
  * for every entry; repeated indices compose by XOR. */
 int mg_matrix_apply_delta(mg_matrix *m, const mg_delta *dl);
 
+/* ---------------------------------------------------------------------------
+ * Continuous-batching request loop (no reference counterpart as one object:
+ * it replaces the server's GPU batch policy and dispatch,
+ * crates/morphogen-server/src/network/gpu_batch.rs:129-149 and gpu_scan.rs:36-76,
+ * plus the epoch-consistent scan, scan.rs:260-299).
+ *
+ * A host thread sweeps the matrix segment by segment. Each sweep step runs
+ * the fused scan over one segment for every active key at once. A request is
+ * admitted as soon as enough key slots are free, starts at the next segment,
+ * and completes when its keys have seen every segment; XOR is order-free, so
+ * its answers equal a whole-matrix scan. Each key answers for the matrix
+ * state (base rows XOR every update applied before its admission), so
+ * updates never tear an in-flight answer.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  uint32_t segments;          /* sweep steps per full pass; 0 = 64          */
+  uint32_t lane_depth;        /* kl of the fused kernel; 0 = 10             */
+  uint32_t max_active_keys;   /* 0 = bounded by free HBM (at most 65536)    */
+  uint32_t domain_bits;       /* key domain served; 0 = ceil(log2 n_rows)   */
+  uint64_t hbm_reserve_bytes; /* HBM left free for other allocations       */
+  uint64_t max_delta_entries; /* update rows kept before a rebase; 0 = 2^20 */
+  uint32_t double_buffer;     /* keep a second matrix copy so a rebase never
+                                 waits for in-flight keys                   */
+} mg_server_opts;
+
+typedef struct {
+  double queue_ms;    /* submit -> admitted                                 */
+  double service_ms;  /* admitted -> answers on the host                    */
+  double latency_ms;  /* submit -> answers on the host                      */
+  uint64_t epoch;     /* update epoch the answers reflect                   */
+  uint32_t n_keys;
+} mg_request_info;
+
+typedef struct {
+  uint64_t submitted, completed, keys_completed, sweep_steps, epoch;
+  uint64_t delta_entries;     /* updates not yet merged into a base copy   */
+  uint64_t rebases;
+  uint32_t active_keys, queued_requests, max_active_keys, segments;
+  double busy_ms;             /* time spent in sweep steps                 */
+} mg_server_stats;
+
+int mg_server_create(mg_device *dev, mg_matrix *m, const mg_server_opts *opts,
+                     mg_server **out);
+/* Queues one request of n_keys keys (typically 3 cuckoo keys x k lookups).
+ * MG_ERR_BUSY if it could never fit the key slots. */
+int mg_server_submit(mg_server *s, const uint8_t *keys, const size_t *key_lens,
+                     uint32_t n_keys, uint64_t *ticket);
+/* Waits up to timeout_ms (0 = forever) for a request; out = n_keys x W.
+ * MG_ERR_BUSY on timeout (the request stays queued). */
+int mg_server_wait(mg_server *s, uint64_t ticket, uint8_t *out,
+                   uint32_t timeout_ms, mg_request_info *info);
+/* Epoch update: rows[row_idx[i]] ^= diffs[i] for every request admitted
+ * after this call (EpochManager::try_advance, epoch.rs:465-549). */
+int mg_server_update(mg_server *s, const uint64_t *row_idx,
+                     const uint8_t *diffs, uint64_t n_entries, uint64_t epoch);
+int mg_server_stats_get(mg_server *s, mg_server_stats *out);
+int mg_server_destroy(mg_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,34 @@ class ScanOpts(ctypes.Structure):
                 ("delta", ctypes.c_void_p)]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("segments", ctypes.c_uint32), ("lane_depth", ctypes.c_uint32),
+                ("max_active_keys", ctypes.c_uint32), ("domain_bits", ctypes.c_uint32),
+                ("hbm_reserve_bytes", ctypes.c_uint64), ("max_delta_entries", ctypes.c_uint64),
+                ("double_buffer", ctypes.c_uint32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("queue_ms", ctypes.c_double), ("service_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double), ("epoch", ctypes.c_uint64),
+                ("n_keys", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("keys_completed", ctypes.c_uint64), ("sweep_steps", ctypes.c_uint64),
+                ("epoch", ctypes.c_uint64), ("delta_entries", ctypes.c_uint64),
+                ("rebases", ctypes.c_uint64), ("active_keys", ctypes.c_uint32),
+                ("queued_requests", ctypes.c_uint32), ("max_active_keys", ctypes.c_uint32),
+                ("segments", ctypes.c_uint32), ("busy_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -128,6 +156,14 @@ def _load():
         "mg_delta_destroy": (ctypes.c_int, [vp]),
         "mg_scan_delta": (ctypes.c_int, [vp, vp, u8p, szp, ctypes.c_uint32, u8p]),
         "mg_matrix_apply_delta": (ctypes.c_int, [vp, vp]),
+        "mg_server_create": (ctypes.c_int, [vp, vp, ctypes.POINTER(ServerOpts),
+                                            ctypes.POINTER(vp)]),
+        "mg_server_submit": (ctypes.c_int, [vp, u8p, szp, ctypes.c_uint32, u64p]),
+        "mg_server_wait": (ctypes.c_int, [vp, ctypes.c_uint64, u8p, ctypes.c_uint32,
+                                          ctypes.POINTER(RequestInfo)]),
+        "mg_server_update": (ctypes.c_int, [vp, u64p, u8p, ctypes.c_uint64, ctypes.c_uint64]),
+        "mg_server_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
+        "mg_server_destroy": (ctypes.c_int, [vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -401,6 +437,60 @@ class DeltaBuffer:
     def close(self):
         if self._h:
             lib.mg_delta_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuServer:
+    """Continuous-batching request loop over a device-resident matrix (C ABI
+    mg_server_*). Requests of any key count are admitted as key slots free
+    up and share every sweep step; answers equal a whole-matrix scan of the
+    matrix state at admission (base rows XOR the updates applied before)."""
+
+    def __init__(self, scanner: GpuScanner, matrix: GpuPageMatrix, segments=0, lane_depth=0,
+                 max_active_keys=0, domain_bits=0, hbm_reserve_bytes=0, max_delta_entries=0,
+                 double_buffer=False):
+        o = ServerOpts(segments, lane_depth, max_active_keys, domain_bits, hbm_reserve_bytes,
+                       max_delta_entries, 1 if double_buffer else 0)
+        h = ctypes.c_void_p()
+        _check(lib.mg_server_create(scanner._h, matrix._h, ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self.scanner = scanner  # device and matrix must outlive the server
+        self.matrix = matrix
+        self.row_bytes = matrix.row_bytes
+
+    def submit(self, keys: Sequence[bytes]) -> int:
+        blob, lens = _pack_keys(keys)
+        t = ctypes.c_uint64()
+        _check(lib.mg_server_submit(self._h, _u8(blob), lens, len(keys), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, n_keys: int, timeout_ms: int = 0):
+        """(answers n_keys x W, info dict); MgError(MG_ERR_BUSY) on timeout."""
+        out = np.zeros((max(n_keys, 1), self.row_bytes), dtype=np.uint8)
+        info = RequestInfo()
+        _check(lib.mg_server_wait(self._h, ticket, _u8(out), timeout_ms, ctypes.byref(info)))
+        return out[:n_keys], info.as_dict()
+
+    def update(self, row_idx: np.ndarray, diffs: np.ndarray, epoch: int):
+        idx = np.ascontiguousarray(row_idx, dtype=np.uint64)
+        dif = np.ascontiguousarray(diffs, dtype=np.uint8).reshape(len(idx), self.row_bytes)
+        _check(lib.mg_server_update(self._h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                    _u8(dif), len(idx), epoch))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib.mg_server_stats_get(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib.mg_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -1,0 +1,141 @@
+"""Launch-parameter autotuner for the fused scan.
+
+Times the fused scan kernel (HIP events on its stream) over a grid of
+(kernel, lane depth kl, keys per wave, waves per block, waves per key) for one
+matrix shape and key count, and keeps the fastest bit-exact setting. Every
+candidate's answers are compared with the first candidate's, so a setting
+that changes a single byte is rejected.
+
+Results are stored in a JSON table keyed by (arch, row width, log2 rows,
+keys-per-pass bucket); `lookup` returns the stored options for a shape so a
+caller can pass them to GpuScanner.scan_keys.
+
+CLI: python -m morphogenesis_amd.autotune --rows-log2 31 --row-bytes 32 --keys 24
+"""
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import (KERNEL_LANE_DFS, KERNEL_STREAM, GpuPageMatrix, GpuScanner, generate_keys,
+               make_opts)
+
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
+
+
+def keys_bucket(n_keys: int) -> int:
+    b = 1
+    while b < n_keys:
+        b *= 2
+    return b
+
+
+def table_key(arch: str, row_bytes: int, n_rows: int, n_keys: int) -> str:
+    return f"{arch}|W{row_bytes}|R{max(0, int(n_rows - 1).bit_length())}|Q{keys_bucket(n_keys)}"
+
+
+def load_table(path=TABLE):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def lookup(arch, row_bytes, n_rows, n_keys, path=TABLE):
+    """Stored options (a dict for make_opts) for this shape, or None."""
+    return load_table(path).get(table_key(arch, row_bytes, n_rows, n_keys))
+
+
+def candidates(row_bytes: int, n_keys: int):
+    chunks = row_bytes // 16
+    out = []
+    if chunks <= 8:
+        for kl, wpb in itertools.product((7, 8, 9, 10, 11, 12), (1, 2, 4)):
+            out.append(dict(kernel=KERNEL_LANE_DFS, lane_depth=kl, waves_per_block=wpb))
+    if chunks <= 256:
+        gmax = 4 if chunks <= 64 else (3 if chunks <= 128 else 2)
+        for kl, g, wpb in itertools.product((4, 5, 6, 7), range(1, gmax + 1), (1, 2)):
+            if g > n_keys:
+                continue
+            out.append(dict(kernel=KERNEL_STREAM, lane_depth=kl, keys_per_wave=g,
+                            waves_per_block=wpb))
+    return out
+
+
+def tune(scanner, matrix, keys, reps=2, grid=None, log=print):
+    """Times each candidate; returns (best option dict, result rows)."""
+    grid = grid or candidates(matrix.row_bytes, len(keys))
+    ref = None
+    rows = []
+    for c in grid:
+        try:
+            opts = make_opts(**c)
+            ans, t = scanner.scan_keys(matrix, keys, opts, with_timing=True)
+            times = [t["scan_kernel_ms"]]
+            for _ in range(reps - 1):
+                ans2, t2 = scanner.scan_keys(matrix, keys, opts, with_timing=True)
+                times.append(t2["scan_kernel_ms"])
+                if not np.array_equal(ans, ans2):
+                    raise RuntimeError("non-deterministic answers")
+        except Exception as e:  # a setting the device cannot launch is skipped
+            log(f"skip {c}: {e}")
+            continue
+        if ref is None:
+            ref = ans
+        exact = bool(np.array_equal(ans, ref))
+        ms = min(times)
+        rows.append(dict(c, ms=ms, exact=exact, waves=t["waves"]))
+        log(f"{c} -> {ms:.3f} ms{'' if exact else '  MISMATCH'}")
+    good = [r for r in rows if r["exact"]]
+    best = min(good, key=lambda r: r["ms"]) if good else None
+    return best, rows
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rows-log2", type=int, default=27)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--row-bytes", type=int, default=32)
+    ap.add_argument("--domain-bits", type=int, default=0)
+    ap.add_argument("--keys", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--kernel", choices=["all", "lane_dfs", "stream"], default="all")
+    ap.add_argument("--save", action="store_true", help="store the winner in tuned.json")
+    args = ap.parse_args()
+    n_rows = args.rows or (1 << args.rows_log2)
+    d = args.domain_bits or max(1, (n_rows - 1).bit_length())
+    scanner = GpuScanner(0)
+    arch = scanner.info()["name"].split("(")[-1].split(":")[0]
+    m = GpuPageMatrix(scanner, n_rows, args.row_bytes)
+    m.fill_seeded(5)
+    rng = np.random.default_rng(1)
+    keys = [generate_keys(d, int(t), seed=i)[i % 2]
+            for i, t in enumerate(rng.integers(0, n_rows, size=args.keys))]
+    grid = candidates(args.row_bytes, args.keys)
+    if args.kernel != "all":
+        kid = KERNEL_LANE_DFS if args.kernel == "lane_dfs" else KERNEL_STREAM
+        grid = [c for c in grid if c["kernel"] == kid]
+    t0 = time.time()
+    best, rows = tune(scanner, m, keys, args.reps, grid,
+                      log=lambda s: print(s, file=sys.stderr, flush=True))
+    result = {"shape": {"rows": n_rows, "row_bytes": args.row_bytes, "domain_bits": d,
+                        "keys": args.keys, "arch": arch},
+              "best": best, "rows": rows, "seconds": time.time() - t0}
+    print(json.dumps(result))
+    if args.save and best:
+        table = load_table()
+        entry = {k: best[k] for k in ("kernel", "lane_depth", "keys_per_wave", "waves_per_block")
+                 if k in best}
+        table[table_key(arch, args.row_bytes, n_rows, args.keys)] = entry
+        with open(TABLE, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+    m.close()
+    scanner.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,148 @@
+// Host-side internals shared by the C ABI runtime (mg_runtime.cpp) and the
+// request loop (mg_server.cpp): opaque handle layouts, error reporting,
+// workspace buffers, launch planning and kernel enqueueing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/morphogen_hip.h"
+#include "mg_kernels.hpp"
+#include "mg_key.hpp"
+
+namespace mg {
+
+// ---------------------------------------------------------------- errors --
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+const char* last_error();
+
+#define MG_HIP(call)                                       \
+  do {                                                     \
+    hipError_t _e = (call);                                \
+    if (_e != hipSuccess) return ::mg::hip_fail(_e, #call); \
+  } while (0)
+
+double now_ms();
+
+// ------------------------------------------------------------ workspaces --
+// Device buffer that only grows.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t bytes);
+  void release();
+};
+
+// Pinned host buffer that only grows.
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t bytes);
+  void release();
+};
+
+// Launch plan of one scan; also the graph-cache signature (compared
+// bytewise, so the constructor clears padding too).
+struct Plan {
+  uint32_t kernel;
+  uint32_t chunks;
+  uint32_t n_keys;
+  uint32_t d, kl, lb, tb;
+  uint32_t keys_per_wave;
+  uint32_t waves_per_block;
+  uint32_t parts;
+  uint64_t tile_begin, tile_end, tiles_per_wave;
+  uint64_t row_begin, row_end;
+  uint64_t n_delta;
+  const void* rows;
+  const void* delta_idx;
+  const void* delta_rows;
+  uint32_t delta_parts;
+  // Buffers baked into a captured graph.
+  const void* ws_keys;
+  const void* ws_out;
+  const void* ws_partial;
+  const void* ws_partial_delta;
+  Plan();
+  bool operator==(const Plan& o) const;
+};
+
+// Device buffers one enqueue_scan reads and writes.
+struct ScanBuffers {
+  const DevKey* keys = nullptr;
+  const uint32_t* key_idx = nullptr;  // launch key i -> keys[key_idx[i]]
+  uint4* partial = nullptr;           // >= plan.parts * n_keys * chunks
+  uint4* partial_delta = nullptr;     // >= plan.delta_parts * n_keys * chunks
+  uint4* out = nullptr;
+  const uint32_t* out_idx = nullptr;  // launch key i -> out row out_idx[i]
+  bool accumulate = false;            // out ^= scan instead of out = scan
+};
+
+}  // namespace mg
+
+struct mg_device {
+  int ordinal = 0;
+  int cus = 0;
+  size_t max_lds = 65536;  // dynamic LDS bytes one block may use
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[5] = {};
+  std::mutex mu;  // serialises every use of `stream` and the workspace
+  mg::DevBuf keys, partial, partial_delta, out;
+  mg::HostBuf h_keys, h_out;
+  // HIP graph of the kernel sequence for the last plan.
+  bool graph_valid = false;
+  mg::Plan graph_plan;
+  hipGraphExec_t graph_exec = nullptr;
+  std::map<std::pair<uint32_t, uint32_t>, int> occupancy;  // (variant, lds) -> blocks/CU
+};
+
+struct mg_matrix {
+  mg_device* dev = nullptr;
+  uint4* rows = nullptr;
+  uint64_t n_rows = 0;
+  uint32_t row_bytes = 0;
+};
+
+struct mg_delta {
+  mg_device* dev = nullptr;
+  uint32_t row_bytes = 0;
+  uint64_t capacity = 0;
+  uint64_t n = 0;
+  uint64_t epoch = 0;
+  uint64_t max_idx = 0;
+  uint64_t* d_idx = nullptr;
+  uint4* d_diffs = nullptr;
+  // Duplicate-free copy for the merge: each row index once, diffs XOR-ed.
+  uint64_t n_unique = 0;
+  uint64_t* d_uidx = nullptr;
+  uint4* d_udiffs = nullptr;
+};
+
+namespace mg {
+
+int set_device(mg_device* dev);
+
+// Parses n_keys concatenated wire keys into `out`; all must share one domain.
+int parse_keys(const uint8_t* keys, const size_t* key_lens, uint32_t n_keys, DevKey* out,
+               uint32_t* common_d);
+
+// Plans a scan of rows [o->row_begin, o->row_end) of a matrix at `rows`.
+int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_bytes,
+              uint32_t n_keys, uint32_t d, const mg_scan_opts* o, Plan* pl);
+
+// Enqueues scan kernel, optional delta kernel and reductions on dev->stream.
+// Events (may be null) bracket the fused scan kernel.
+int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_t ev_scan0,
+                 hipEvent_t ev_scan1);
+
+// Collapses repeated row indices by XOR-ing their diffs (for the merge kernel).
+void dedupe_delta(const uint64_t* idx, const uint8_t* diffs, uint64_t n, uint32_t row_bytes,
+                  std::vector<uint64_t>* uidx, std::vector<uint8_t>* udiffs);
+
+}  // namespace mg

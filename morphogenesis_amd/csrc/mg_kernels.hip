@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   if (w >= p.parts * p.n_keys) return;
   const uint32_t key_i = w % p.n_keys;
   const uint32_t part = w / p.n_keys;
-  const DevKey* __restrict__ k = p.keys + key_i;
+  const DevKey* __restrict__ k = p.keys + (p.key_idx ? p.key_idx[key_i] : key_i);
   const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
 
   // Per-wave DFS stack: right children waiting at each subtree level.
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void scan_point(PointParams p) {
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + wib;  // entry wave
   if ((uint64_t)w * 64 >= p.n_entries) return;
   const uint32_t key_i = blockIdx.y;
-  const DevKey* __restrict__ k = p.keys + key_i;
+  const DevKey* __restrict__ k = p.keys + (p.key_idx ? p.key_idx[key_i] : key_i);
   const uint64_t e = (uint64_t)w * 64 + lane;
   const bool valid = e < p.n_entries;
   const uint64_t x = valid ? (p.row_idx ? p.row_idx[e] : p.first_row + e) : 0;
@@ -214,7 +214,8 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
 #pragma unroll
   for (int g = 0; g < G; g++) {
     key_live[g] = key0 + g < p.n_keys;
-    kp[g] = p.keys + (key_live[g] ? key0 + g : key0);
+    const uint32_t ki = key_live[g] ? key0 + g : key0;
+    kp[g] = p.keys + (p.key_idx ? p.key_idx[ki] : ki);
   }
 
   uint4 acc[G][S];
@@ -325,6 +326,7 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
 __global__ __launch_bounds__(256) void reduce_partials(const uint4* __restrict__ partial,
                                                        uint32_t parts, uint32_t n_keys,
                                                        uint32_t chunks, uint4* __restrict__ out,
+                                                       const uint32_t* __restrict__ out_idx,
                                                        int accumulate) {
   __shared__ uint4 red[256];
   const uint32_t key = blockIdx.y;
@@ -339,7 +341,7 @@ __global__ __launch_bounds__(256) void reduce_partials(const uint4* __restrict__
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    uint4* o = out + (size_t)key * chunks + c;
+    uint4* o = out + (size_t)(out_idx ? out_idx[key] : key) * chunks + c;
     *o = accumulate ? xor4(*o, red[0]) : red[0];
   }
 }
@@ -417,6 +419,19 @@ __global__ __launch_bounds__(256) void xor_rows(uint4* __restrict__ rows, const 
     const uint32_t c = (uint32_t)(i % chunks);
     uint4* dst = rows + idx[e] * chunks + c;
     *dst = xor4(*dst, diffs[e * chunks + c]);
+  }
+}
+
+__global__ __launch_bounds__(256) void scatter_u4(uint4* __restrict__ dst,
+                                                  const uint4* __restrict__ src,
+                                                  const uint32_t* __restrict__ idx, uint32_t n,
+                                                  uint32_t per) {
+  const uint64_t total = (uint64_t)n * per;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t e = i / per;
+    const uint32_t j = (uint32_t)(i % per);
+    dst[(uint64_t)idx[e] * per + j] = src[i];
   }
 }
 
@@ -505,9 +520,9 @@ hipError_t launch_point(const PointParams& p, hipStream_t s) {
 }
 
 hipError_t launch_reduce(const uint4* partial, uint32_t parts, uint32_t n_keys, uint32_t chunks,
-                         uint4* out, bool accumulate, hipStream_t s) {
+                         uint4* out, const uint32_t* out_idx, bool accumulate, hipStream_t s) {
   hipLaunchKernelGGL(reduce_partials, dim3(chunks, n_keys), dim3(256), 0, s, partial, parts,
-                     n_keys, chunks, out, accumulate ? 1 : 0);
+                     n_keys, chunks, out, out_idx, accumulate ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -526,6 +541,13 @@ hipError_t launch_fill_lcg(uint4* out, uint64_t n_blocks16, uint64_t seed, hipSt
 
 hipError_t launch_zero(uint4* out, uint64_t n, hipStream_t s) {
   hipLaunchKernelGGL(zero_u4, dim3(grid_for(n, 256)), dim3(256), 0, s, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_u4(uint4* dst, const uint4* src, const uint32_t* idx, uint32_t n,
+                             uint32_t per, hipStream_t s) {
+  hipLaunchKernelGGL(scatter_u4, dim3(grid_for((uint64_t)n * per, 256)), dim3(256), 0, s, dst, src,
+                     idx, n, per);
   return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@ struct LaneDfsParams {
   const uint4* rows;
   uint64_t n_rows;
   const DevKey* keys;
+  const uint32_t* key_idx;  // key i of the launch is keys[key_idx[i]] (identity if null)
   uint32_t n_keys;
   uint32_t d, kl, lb, tb;
   uint64_t tile_begin;  // tiles [tile_begin, tile_end) are scanned; rows
@@ -33,6 +34,7 @@ struct StreamParams {
   uint64_t n_rows;
   uint32_t chunks;
   const DevKey* keys;
+  const uint32_t* key_idx;
   uint32_t n_keys;
   uint32_t d, kl, lb, tb;
   uint64_t tile_begin;
@@ -49,6 +51,7 @@ struct PointParams {
   uint64_t n_entries;
   uint32_t chunks;
   const DevKey* keys;
+  const uint32_t* key_idx;
   uint32_t n_keys;
   uint4* partial;  // [ceil(n_entries/64)][n_keys][chunks]
 };
@@ -62,12 +65,16 @@ size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block);
 hipError_t lane_dfs_blocks_per_cu(int chunks, int block_threads, size_t lds, int* blocks);
 hipError_t stream_blocks_per_cu(int S, int G, int block_threads, size_t lds, int* blocks);
 hipError_t launch_point(const PointParams& p, hipStream_t s);
+// out[out_idx ? out_idx[k] : k] (^)= XOR of the parts' partials of key k.
 hipError_t launch_reduce(const uint4* partial, uint32_t parts, uint32_t n_keys, uint32_t chunks,
-                         uint4* out, bool accumulate, hipStream_t s);
+                         uint4* out, const uint32_t* out_idx, bool accumulate, hipStream_t s);
 hipError_t launch_fill_splitmix(uint4* out, uint64_t first_pair, uint64_t n_pairs, uint64_t seed,
                                 hipStream_t s);
 hipError_t launch_fill_lcg(uint4* out, uint64_t n_blocks16, uint64_t seed, hipStream_t s);
 hipError_t launch_zero(uint4* out, uint64_t n, hipStream_t s);
+// dst[idx[e] * per + j] = src[e * per + j] for e < n, j < per (16-byte units).
+hipError_t launch_scatter_u4(uint4* dst, const uint4* src, const uint32_t* idx, uint32_t n,
+                             uint32_t per, hipStream_t s);
 hipError_t launch_xor_rows(uint4* rows, const uint64_t* idx, const uint4* diffs,
                            uint64_t n_entries, uint32_t chunks, hipStream_t s);
 

@@ -1,30 +1,22 @@
 // Host runtime behind include/morphogen_hip.h: device, matrix, key, delta and
 // scan entry points. It plays the role of the reference's GpuScanner /
-// GpuPageMatrix host driver (crates/morphogen-gpu-dpf/src/kernel.rs:156-1211,
+// GpuPageMatrix host side (crates/morphogen-gpu-dpf/src/kernel.rs:156-1211,
 // src/storage.rs:18-145) on one gfx950 GPU: one non-blocking HIP stream per
 // device, a grow-only workspace, pinned staging buffers, HIP-event timing and
 // an optional HIP-graph replay of the launch sequence.
-#include <hip/hip_runtime.h>
-
 #include <algorithm>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
-#include <map>
-#include <mutex>
-#include <string>
-#include <vector>
 
-#include "../../include/morphogen_hip.h"
-#include "mg_kernels.hpp"
-#include "mg_key.hpp"
+#include "mg_internal.hpp"
 
-using mg::DevKey;
+namespace mg {
 
 namespace {
-
 thread_local std::string g_err;
+}
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -41,136 +33,62 @@ int hip_fail(hipError_t e, const char* what) {
               what, hipGetErrorString(e), (int)e);
 }
 
-#define MG_HIP(call)                                   \
-  do {                                                 \
-    hipError_t _e = (call);                            \
-    if (_e != hipSuccess) return hip_fail(_e, #call);  \
-  } while (0)
+const char* last_error() { return g_err.c_str(); }
 
 double now_ms() {
   using namespace std::chrono;
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-// Device buffer that only grows.
-struct DevBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-  int reserve(size_t bytes) {
-    if (bytes <= cap) return MG_OK;
-    if (p) (void)hipFree(p);
+int DevBuf::reserve(size_t bytes) {
+  if (bytes <= cap) return MG_OK;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  const size_t want = std::max(bytes, (size_t)4096);
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
     p = nullptr;
-    cap = 0;
-    size_t want = std::max(bytes, (size_t)4096);
-    hipError_t e = hipMalloc(&p, want);
-    if (e != hipSuccess) {
-      p = nullptr;
-      return hip_fail(e, "hipMalloc(workspace)");
-    }
-    cap = want;
-    return MG_OK;
+    return hip_fail(e, "hipMalloc(workspace)");
   }
-  void release() {
-    if (p) (void)hipFree(p);
+  cap = want;
+  return MG_OK;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
+int HostBuf::reserve(size_t bytes) {
+  if (bytes <= cap) return MG_OK;
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+  const size_t want = std::max(bytes, (size_t)4096);
+  hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
     p = nullptr;
-    cap = 0;
+    return hip_fail(e, "hipHostMalloc(staging)");
   }
-};
+  cap = want;
+  return MG_OK;
+}
 
-struct HostBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-  int reserve(size_t bytes) {
-    if (bytes <= cap) return MG_OK;
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    cap = 0;
-    size_t want = std::max(bytes, (size_t)4096);
-    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
-    if (e != hipSuccess) {
-      p = nullptr;
-      return hip_fail(e, "hipHostMalloc(staging)");
-    }
-    cap = want;
-    return MG_OK;
-  }
-  void release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-};
+void HostBuf::release() {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+}
 
-// Launch plan of one scan call; also the graph-cache signature (compared
-// bytewise, so the constructor clears padding too).
-struct Plan {
-  uint32_t kernel;
-  uint32_t chunks;
-  uint32_t n_keys;
-  uint32_t d, kl, lb, tb;
-  uint32_t keys_per_wave;
-  uint32_t waves_per_block;
-  uint32_t parts;
-  uint64_t tile_begin, tile_end, tiles_per_wave;
-  uint64_t row_begin, row_end;
-  uint64_t n_delta;
-  const void* rows;
-  const void* delta_idx;
-  const void* delta_rows;
-  uint32_t delta_parts;
-  // Workspace addresses baked into a captured graph.
-  const void* ws_keys;
-  const void* ws_out;
-  const void* ws_partial;
-  const void* ws_partial_delta;
-  Plan() {
-    memset(this, 0, sizeof *this);
-    keys_per_wave = 1;
-    waves_per_block = 4;
-  }
-  bool operator==(const Plan& o) const { return memcmp(this, &o, sizeof(Plan)) == 0; }
-};
+Plan::Plan() {
+  memset(this, 0, sizeof *this);
+  keys_per_wave = 1;
+  waves_per_block = 4;
+}
 
-}  // namespace
-
-struct mg_device {
-  int ordinal = 0;
-  int cus = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[5] = {};
-  std::mutex mu;
-  DevBuf keys, partial, partial_delta, out;
-  HostBuf h_keys, h_out;
-  // HIP graph of the kernel sequence for the last plan.
-  bool graph_valid = false;
-  Plan graph_plan;
-  hipGraphExec_t graph_exec = nullptr;
-  std::map<std::pair<uint32_t, uint32_t>, int> occupancy;  // (kernel id, lds) -> blocks/CU
-};
-
-struct mg_matrix {
-  mg_device* dev = nullptr;
-  uint4* rows = nullptr;
-  uint64_t n_rows = 0;
-  uint32_t row_bytes = 0;
-};
-
-struct mg_delta {
-  mg_device* dev = nullptr;
-  uint32_t row_bytes = 0;
-  uint64_t capacity = 0;
-  uint64_t n = 0;
-  uint64_t epoch = 0;
-  uint64_t max_idx = 0;
-  uint64_t* d_idx = nullptr;
-  uint4* d_diffs = nullptr;
-  // Duplicate-free copy for the merge: each row index once, diffs XOR-ed.
-  uint64_t n_unique = 0;
-  uint64_t* d_uidx = nullptr;
-  uint4* d_udiffs = nullptr;
-};
-
-namespace {
+bool Plan::operator==(const Plan& o) const { return memcmp(this, &o, sizeof(Plan)) == 0; }
 
 int set_device(mg_device* dev) {
   hipError_t e = hipSetDevice(dev->ordinal);
@@ -183,14 +101,14 @@ int parse_keys(const uint8_t* keys, const size_t* key_lens, uint32_t n_keys, Dev
   size_t off = 0;
   for (uint32_t i = 0; i < n_keys; i++) {
     uint32_t party = 0;
-    switch (mg::parse_wire_key(keys + off, key_lens[i], &out[i], &party)) {
-      case mg::kParseOk:
+    switch (parse_wire_key(keys + off, key_lens[i], &out[i], &party)) {
+      case kParseOk:
         break;
-      case mg::kParseFormat:
+      case kParseFormat:
         return fail(MG_ERR_KEY_FORMAT, "key %u: invalid key format (length %zu)", i, key_lens[i]);
       default:
         return fail(MG_ERR_DOMAIN, "key %u: domain bits %u > %d", i, (unsigned)keys[off + 18],
-                    mg::kMaxDomainBits);
+                    kMaxDomainBits);
     }
     if (i == 0) *common_d = out[i].domain_bits;
     if (out[i].domain_bits != *common_d)
@@ -201,9 +119,30 @@ int parse_keys(const uint8_t* keys, const size_t* key_lens, uint32_t n_keys, Dev
   return MG_OK;
 }
 
-// Resident waves the device holds for a kernel variant (cached by `kind`).
-int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int chunks,
-                   int waves_per_block, size_t lds, int* out) {
+void dedupe_delta(const uint64_t* idx, const uint8_t* diffs, uint64_t n, uint32_t row_bytes,
+                  std::vector<uint64_t>* uidx, std::vector<uint8_t>* udiffs) {
+  std::map<uint64_t, std::vector<uint8_t>> uniq;
+  for (uint64_t i = 0; i < n; i++) {
+    auto& v = uniq[idx[i]];
+    if (v.empty()) v.assign(row_bytes, 0);
+    const uint8_t* src = diffs + i * row_bytes;
+    for (uint32_t b = 0; b < row_bytes; b++) v[b] ^= src[b];
+  }
+  uidx->clear();
+  udiffs->clear();
+  uidx->reserve(uniq.size());
+  udiffs->reserve(uniq.size() * row_bytes);
+  for (auto& kv : uniq) {
+    uidx->push_back(kv.first);
+    udiffs->insert(udiffs->end(), kv.second.begin(), kv.second.end());
+  }
+}
+
+namespace {
+
+// Resident waves the device holds for a kernel variant (cached per variant).
+int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int chunks, int waves_per_block,
+                   size_t lds, int* out) {
   auto key = std::make_pair(kind * 64 + (uint32_t)waves_per_block, (uint32_t)lds);
   auto it = dev->occupancy.find(key);
   int blocks = 0;
@@ -211,47 +150,47 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int chunks,
     blocks = it->second;
   } else {
     if (kind == MG_KERNEL_LANE_DFS * 100 + (uint32_t)chunks)
-      MG_HIP(mg::lane_dfs_blocks_per_cu(chunks, 64 * waves_per_block, lds, &blocks));
+      MG_HIP(lane_dfs_blocks_per_cu(chunks, 64 * waves_per_block, lds, &blocks));
     else
-      MG_HIP(mg::stream_blocks_per_cu(S, G, 64 * waves_per_block, lds, &blocks));
+      MG_HIP(stream_blocks_per_cu(S, G, 64 * waves_per_block, lds, &blocks));
     dev->occupancy[key] = blocks;
   }
   *out = std::max(1, blocks) * waves_per_block * dev->cus;
   return MG_OK;
 }
 
-}  // namespace
-
-namespace {
-
 int max_keys_per_wave(int S) { return S == 1 ? 4 : (S == 2 ? 3 : 2); }
+
+int stream_s(uint32_t chunks) { return chunks <= 64 ? 1 : (chunks <= 128 ? 2 : 4); }
 
 // Splits a d-bit domain as tb + lb + kl (see LaneDfsParams).
 void split_levels(uint32_t d, uint32_t kl_req, Plan* pl) {
-  uint32_t kl = std::min(std::max(kl_req, 1u), d);
-  uint32_t lb = std::min(6u, d - kl);
+  const uint32_t kl = std::min(std::max(kl_req, 1u), d);
+  const uint32_t lb = std::min(6u, d - kl);
   pl->kl = kl;
   pl->lb = lb;
   pl->tb = d - kl - lb;
 }
 
-int make_plan(mg_device* dev, const mg_matrix* m, uint32_t n_keys, uint32_t d,
-              const mg_scan_opts* o, Plan* pl) {
+}  // namespace
+
+int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_bytes,
+              uint32_t n_keys, uint32_t d, const mg_scan_opts* o, Plan* pl) {
   mg_scan_opts def{};
   if (!o) o = &def;
-  pl->chunks = m->row_bytes / 16;
+  pl->chunks = row_bytes / 16;
   pl->n_keys = n_keys;
   pl->d = d;
-  pl->rows = m->rows;
+  pl->rows = rows;
   pl->row_begin = o->row_begin;
-  pl->row_end = o->row_end ? o->row_end : m->n_rows;
-  if (pl->row_end > m->n_rows || pl->row_begin > pl->row_end)
+  pl->row_end = o->row_end ? o->row_end : n_rows;
+  if (pl->row_end > n_rows || pl->row_begin > pl->row_end)
     return fail(MG_ERR_ROWS, "row range [%llu, %llu) outside the %llu-row matrix",
                 (unsigned long long)pl->row_begin, (unsigned long long)pl->row_end,
-                (unsigned long long)m->n_rows);
-  if (d < 64 && (m->n_rows > (1ull << d)))
+                (unsigned long long)n_rows);
+  if (d < 64 && (n_rows > (1ull << d)))
     return fail(MG_ERR_ROWS, "%llu rows exceed the 2^%u-leaf key domain",
-                (unsigned long long)m->n_rows, d);
+                (unsigned long long)n_rows, d);
   if (o->waves_per_block > 4) return fail(MG_ERR_INVALID_ARG, "waves_per_block must be 0-4");
   if (o->keys_per_wave > 4) return fail(MG_ERR_INVALID_ARG, "keys_per_wave must be 0-4");
   if (o->lane_depth > 16) return fail(MG_ERR_INVALID_ARG, "lane_depth must be 0-16");
@@ -277,7 +216,18 @@ int make_plan(mg_device* dev, const mg_matrix* m, uint32_t n_keys, uint32_t d,
     pl->parts = (uint32_t)((pl->row_end - pl->row_begin + 63) / 64);
     return MG_OK;
   }
-  split_levels(d, o->lane_depth ? o->lane_depth : (kernel == MG_KERNEL_LANE_DFS ? 8 : 6), pl);
+  // Default lane depth (measured on MI355X): lane-DFS wants deep lane subtrees (kl up to 10)
+  // as long as the keys still have at least ~4096 tiles of 2^(kl+6) rows between them to spread
+  // over the resident waves; the stream kernel's LDS stacks favour kl 5-7.
+  uint32_t kl_default;
+  if (kernel == MG_KERNEL_LANE_DFS) {
+    const uint64_t work = (pl->row_end - pl->row_begin) * (uint64_t)n_keys;
+    kl_default = 6;
+    while (kl_default < 10 && (work >> (kl_default + 7)) >= 4096) kl_default++;
+  } else {
+    kl_default = n_keys >= 3 ? 5 : 7;
+  }
+  split_levels(d, o->lane_depth ? o->lane_depth : kl_default, pl);
   const uint64_t tile_rows = 1ull << (pl->kl + pl->lb);
   if (pl->row_begin % tile_rows != 0)
     return fail(MG_ERR_ROWS, "row_begin %llu is not a multiple of the %llu-row tile",
@@ -286,25 +236,31 @@ int make_plan(mg_device* dev, const mg_matrix* m, uint32_t n_keys, uint32_t d,
   pl->tile_end = (pl->row_end + tile_rows - 1) / tile_rows;
   const uint64_t n_tiles = pl->tile_end - pl->tile_begin;
 
-  if (kernel == MG_KERNEL_LANE_DFS && !o->waves_per_block)
-    while (pl->waves_per_block > 1 && (size_t)pl->waves_per_block * pl->kl * 80 * 16 > 64 * 1024)
-      pl->waves_per_block--;
   uint32_t groups = n_keys;
   int resident = 0;
   int rc;
   if (kernel == MG_KERNEL_LANE_DFS) {
+    if (!o->waves_per_block)
+      while (pl->waves_per_block > 1 &&
+             (size_t)pl->waves_per_block * pl->kl * 80 * 16 > dev->max_lds)
+        pl->waves_per_block--;
     const size_t lds = (size_t)pl->waves_per_block * pl->kl * 80 * 16;
+    if (lds > dev->max_lds)
+      return fail(MG_ERR_INVALID_ARG, "lane-DFS plan needs %zu B of LDS per block (max %zu)", lds,
+                  dev->max_lds);
     rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl->chunks, 0, 0, (int)pl->chunks,
                         (int)pl->waves_per_block, lds, &resident);
   } else {
-    const int S = pl->chunks <= 64 ? 1 : (pl->chunks <= 128 ? 2 : 4);
+    const int S = stream_s(pl->chunks);
     int G = o->keys_per_wave ? (int)o->keys_per_wave : std::min<int>(3, (int)n_keys);
     G = std::max(1, std::min(G, max_keys_per_wave(S)));
     pl->keys_per_wave = (uint32_t)G;
     groups = (n_keys + G - 1) / G;
     if (!o->waves_per_block) pl->waves_per_block = 1;
-    const size_t lds = mg::stream_lds_bytes(G, pl->kl, pl->waves_per_block);
-    if (lds > 160 * 1024) return fail(MG_ERR_INVALID_ARG, "stream plan needs %zu B of LDS", lds);
+    const size_t lds = stream_lds_bytes(G, pl->kl, (int)pl->waves_per_block);
+    if (lds > dev->max_lds)
+      return fail(MG_ERR_INVALID_ARG, "stream plan needs %zu B of LDS per block (max %zu)", lds,
+                  dev->max_lds);
     rc = resident_waves(dev, MG_KERNEL_STREAM * 100 + (uint32_t)(S * 8 + G), S, G,
                         (int)pl->chunks, (int)pl->waves_per_block, lds, &resident);
   }
@@ -323,95 +279,115 @@ int make_plan(mg_device* dev, const mg_matrix* m, uint32_t n_keys, uint32_t d,
   return MG_OK;
 }
 
-// Enqueues the kernels of a plan on the device stream: scan, delta,
-// reductions into dev->out.
-int enqueue_plan(mg_device* dev, const Plan& pl, hipEvent_t ev_scan0, hipEvent_t ev_scan1) {
-  const DevKey* keys = (const DevKey*)dev->keys.p;
-  uint4* out = (uint4*)dev->out.p;
-  uint4* partial = (uint4*)dev->partial.p;
+int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_t ev_scan0,
+                 hipEvent_t ev_scan1) {
   hipStream_t s = dev->stream;
   if (ev_scan0) MG_HIP(hipEventRecord(ev_scan0, s));
-  if (pl.row_end > pl.row_begin) {
+  const bool any_rows = pl.row_end > pl.row_begin;
+  if (any_rows) {
     if (pl.kernel == MG_KERNEL_LANE_DFS) {
-      mg::LaneDfsParams p{};
+      LaneDfsParams p{};
       p.rows = (const uint4*)pl.rows;
       p.n_rows = pl.row_end;
-      p.keys = keys;
+      p.keys = b.keys;
+      p.key_idx = b.key_idx;
       p.n_keys = pl.n_keys;
       p.d = pl.d; p.kl = pl.kl; p.lb = pl.lb; p.tb = pl.tb;
       p.tile_begin = pl.tile_begin;
       p.tile_end = pl.tile_end;
       p.tiles_per_wave = pl.tiles_per_wave;
       p.parts = pl.parts;
-      p.partial = partial;
-      MG_HIP(mg::launch_lane_dfs((int)pl.chunks, p, (int)pl.waves_per_block, s));
+      p.partial = b.partial;
+      MG_HIP(launch_lane_dfs((int)pl.chunks, p, (int)pl.waves_per_block, s));
     } else if (pl.kernel == MG_KERNEL_STREAM) {
-      mg::StreamParams p{};
+      StreamParams p{};
       p.rows = (const uint4*)pl.rows;
       p.n_rows = pl.row_end;
       p.chunks = pl.chunks;
-      p.keys = keys;
+      p.keys = b.keys;
+      p.key_idx = b.key_idx;
       p.n_keys = pl.n_keys;
       p.d = pl.d; p.kl = pl.kl; p.lb = pl.lb; p.tb = pl.tb;
       p.tile_begin = pl.tile_begin;
       p.tile_end = pl.tile_end;
       p.tiles_per_wave = pl.tiles_per_wave;
       p.parts = pl.parts;
-      p.partial = partial;
-      const int S = pl.chunks <= 64 ? 1 : (pl.chunks <= 128 ? 2 : 4);
-      MG_HIP(mg::launch_stream(S, (int)pl.keys_per_wave, p, (int)pl.waves_per_block, s));
+      p.partial = b.partial;
+      MG_HIP(launch_stream(stream_s(pl.chunks), (int)pl.keys_per_wave, p,
+                           (int)pl.waves_per_block, s));
     } else {
-      mg::PointParams p{};
+      PointParams p{};
       p.rows = (const uint4*)pl.rows;
       p.row_idx = nullptr;
       p.first_row = pl.row_begin;
       p.n_entries = pl.row_end - pl.row_begin;
       p.chunks = pl.chunks;
-      p.keys = keys;
+      p.keys = b.keys;
+      p.key_idx = b.key_idx;
       p.n_keys = pl.n_keys;
-      p.partial = partial;
-      MG_HIP(mg::launch_point(p, s));
+      p.partial = b.partial;
+      MG_HIP(launch_point(p, s));
     }
   }
   if (ev_scan1) MG_HIP(hipEventRecord(ev_scan1, s));
-  if (pl.row_end > pl.row_begin) {
-    MG_HIP(mg::launch_reduce(partial, pl.parts, pl.n_keys, pl.chunks, out, false, s));
-  } else {
-    MG_HIP(mg::launch_zero(out, (uint64_t)pl.n_keys * pl.chunks, s));
-  }
+  // With no rows the reduction over zero parts writes zeros (or leaves `out`
+  // unchanged when accumulating).
+  MG_HIP(launch_reduce(b.partial, any_rows ? pl.parts : 0, pl.n_keys, pl.chunks, b.out, b.out_idx,
+                       b.accumulate, s));
   if (pl.n_delta) {
-    mg::PointParams p{};
+    PointParams p{};
     p.rows = (const uint4*)pl.delta_rows;
     p.row_idx = (const uint64_t*)pl.delta_idx;
     p.n_entries = pl.n_delta;
     p.chunks = pl.chunks;
-    p.keys = keys;
+    p.keys = b.keys;
+    p.key_idx = b.key_idx;
     p.n_keys = pl.n_keys;
-    p.partial = (uint4*)dev->partial_delta.p;
-    MG_HIP(mg::launch_point(p, s));
-    MG_HIP(mg::launch_reduce((const uint4*)dev->partial_delta.p, pl.delta_parts, pl.n_keys,
-                             pl.chunks, out, true, s));
+    p.partial = b.partial_delta;
+    MG_HIP(launch_point(p, s));
+    MG_HIP(launch_reduce(b.partial_delta, pl.delta_parts, pl.n_keys, pl.chunks, b.out, b.out_idx,
+                         true, s));
   }
   return MG_OK;
 }
 
-int run_plan_graph(mg_device* dev, const Plan& pl) {
+}  // namespace mg
+
+using mg::DevKey;
+using mg::fail;
+
+namespace {
+
+mg::ScanBuffers device_buffers(mg_device* dev) {
+  mg::ScanBuffers b;
+  b.keys = (const DevKey*)dev->keys.p;
+  b.partial = (uint4*)dev->partial.p;
+  b.partial_delta = (uint4*)dev->partial_delta.p;
+  b.out = (uint4*)dev->out.p;
+  return b;
+}
+
+void invalidate_graph(mg_device* dev) {
+  if (dev->graph_exec) (void)hipGraphExecDestroy(dev->graph_exec);
+  dev->graph_exec = nullptr;
+  dev->graph_valid = false;
+}
+
+int run_plan_graph(mg_device* dev, const mg::Plan& pl) {
   if (!(dev->graph_valid && dev->graph_plan == pl)) {
-    if (dev->graph_exec) (void)hipGraphExecDestroy(dev->graph_exec);
-    dev->graph_exec = nullptr;
-    dev->graph_valid = false;
+    invalidate_graph(dev);
     MG_HIP(hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_plan(dev, pl, nullptr, nullptr);
+    int rc = mg::enqueue_scan(dev, pl, device_buffers(dev), nullptr, nullptr);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(dev->stream, &g);
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
     }
-    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    if (e != hipSuccess) return mg::hip_fail(e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&dev->graph_exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    if (e != hipSuccess) return mg::hip_fail(e, "hipGraphInstantiate");
     dev->graph_plan = pl;
     dev->graph_valid = true;
   }
@@ -422,9 +398,9 @@ int run_plan_graph(mg_device* dev, const Plan& pl) {
 // One scan of n_keys parsed keys (already in dev->h_keys) into `out`.
 int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8_t* out,
                 const mg_scan_opts* opts, mg_timing* t) {
-  const double t0 = now_ms();
-  Plan pl;
-  int rc = make_plan(dev, m, n_keys, d, opts, &pl);
+  const double t0 = mg::now_ms();
+  mg::Plan pl;
+  int rc = mg::make_plan(dev, m->rows, m->n_rows, m->row_bytes, n_keys, d, opts, &pl);
   if (rc) return rc;
   const mg_delta* dl = opts ? opts->delta : nullptr;
   if (dl) {
@@ -462,7 +438,7 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
     if ((rc = run_plan_graph(dev, pl))) return rc;
     MG_HIP(hipEventRecord(dev->ev[2], s));
   } else {
-    if ((rc = enqueue_plan(dev, pl, dev->ev[1], dev->ev[2]))) return rc;
+    if ((rc = mg::enqueue_scan(dev, pl, device_buffers(dev), dev->ev[1], dev->ev[2]))) return rc;
   }
   MG_HIP(hipEventRecord(dev->ev[3], s));
   MG_HIP(hipMemcpyAsync(dev->h_out.p, dev->out.p, out_bytes, hipMemcpyDeviceToHost, s));
@@ -479,7 +455,7 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
     t->scan_kernel_ms = scan;
     t->kernel_ms = kern;
     t->d2h_ms = d2h;
-    t->total_ms = now_ms() - t0;
+    t->total_ms = mg::now_ms() - t0;
     t->kernel = pl.kernel;
     t->lane_depth = pl.kl;
     t->keys_per_wave = pl.kernel == MG_KERNEL_STREAM ? pl.keys_per_wave : 1;
@@ -496,7 +472,7 @@ int stage_keys(mg_device* dev, const uint8_t* keys, const size_t* key_lens, uint
                uint32_t* d) {
   int rc = dev->h_keys.reserve(sizeof(DevKey) * std::max<uint32_t>(n_keys, 1));
   if (rc) return rc;
-  return parse_keys(keys, key_lens, n_keys, (DevKey*)dev->h_keys.p, d);
+  return mg::parse_keys(keys, key_lens, n_keys, (DevKey*)dev->h_keys.p, d);
 }
 
 }  // namespace
@@ -507,7 +483,7 @@ extern "C" {
 
 int mg_abi_version(void) { return MG_ABI_VERSION; }
 
-const char* mg_last_error(void) { return g_err.c_str(); }
+const char* mg_last_error(void) { return mg::last_error(); }
 
 int mg_device_open(int ordinal, mg_device** out) {
   if (!out) return fail(MG_ERR_INVALID_ARG, "out is null");
@@ -526,16 +502,17 @@ int mg_device_open(int ordinal, mg_device** out) {
   mg_device* dev = new mg_device();
   dev->ordinal = ordinal;
   dev->cus = prop.multiProcessorCount;
+  dev->max_lds = prop.sharedMemPerBlock;
   hipError_t se = hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking);
   if (se != hipSuccess) {
     delete dev;
-    return hip_fail(se, "hipStreamCreateWithFlags");
+    return mg::hip_fail(se, "hipStreamCreateWithFlags");
   }
   for (auto& ev : dev->ev) {
     hipError_t ee = hipEventCreate(&ev);
     if (ee != hipSuccess) {
       mg_device_close(dev);
-      return hip_fail(ee, "hipEventCreate");
+      return mg::hip_fail(ee, "hipEventCreate");
     }
   }
   *out = dev;
@@ -546,7 +523,7 @@ int mg_device_close(mg_device* dev) {
   if (!dev) return MG_OK;
   (void)hipSetDevice(dev->ordinal);
   if (dev->stream) (void)hipStreamSynchronize(dev->stream);
-  if (dev->graph_exec) (void)hipGraphExecDestroy(dev->graph_exec);
+  invalidate_graph(dev);
   dev->keys.release();
   dev->partial.release();
   dev->partial_delta.release();
@@ -563,7 +540,7 @@ int mg_device_close(mg_device* dev) {
 int mg_device_info(mg_device* dev, uint64_t* free_bytes, uint64_t* total_bytes,
                    uint32_t* compute_units, char* name, size_t name_len) {
   if (!dev) return fail(MG_ERR_INVALID_ARG, "device is null");
-  int rc = set_device(dev);
+  int rc = mg::set_device(dev);
   if (rc) return rc;
   size_t fr = 0, tot = 0;
   MG_HIP(hipMemGetInfo(&fr, &tot));
@@ -585,7 +562,7 @@ int mg_matrix_create(mg_device* dev, uint64_t n_rows, uint32_t row_bytes, mg_mat
     return fail(MG_ERR_ROWS, "row width %u is not a positive multiple of 16", row_bytes);
   if (n_rows > (1ull << 32)) return fail(MG_ERR_ROWS, "more than 2^32 rows");
   std::lock_guard<std::mutex> lk(dev->mu);
-  int rc = set_device(dev);
+  int rc = mg::set_device(dev);
   if (rc) return rc;
   mg_matrix* m = new mg_matrix();
   m->dev = dev;
@@ -595,9 +572,8 @@ int mg_matrix_create(mg_device* dev, uint64_t n_rows, uint32_t row_bytes, mg_mat
   hipError_t e = hipMalloc((void**)&m->rows, bytes);
   if (e != hipSuccess) {
     delete m;
-    return hip_fail(e, "hipMalloc(matrix)");
+    return mg::hip_fail(e, "hipMalloc(matrix)");
   }
-  rc = MG_OK;
   if (mg::launch_zero(m->rows, bytes / 16, dev->stream) != hipSuccess ||
       hipStreamSynchronize(dev->stream) != hipSuccess) {
     (void)hipFree(m->rows);
@@ -615,7 +591,7 @@ int mg_matrix_upload(mg_matrix* m, uint64_t first_row, const void* rows, uint64_
                 (unsigned long long)first_row, (unsigned long long)n_rows,
                 (unsigned long long)m->n_rows);
   std::lock_guard<std::mutex> lk(m->dev->mu);
-  int rc = set_device(m->dev);
+  int rc = mg::set_device(m->dev);
   if (rc) return rc;
   MG_HIP(hipMemcpyAsync((uint8_t*)m->rows + first_row * m->row_bytes, rows,
                         (size_t)n_rows * m->row_bytes, hipMemcpyHostToDevice, m->dev->stream));
@@ -630,7 +606,7 @@ int mg_matrix_download(mg_matrix* m, uint64_t first_row, void* rows, uint64_t n_
                 (unsigned long long)first_row, (unsigned long long)n_rows,
                 (unsigned long long)m->n_rows);
   std::lock_guard<std::mutex> lk(m->dev->mu);
-  int rc = set_device(m->dev);
+  int rc = mg::set_device(m->dev);
   if (rc) return rc;
   MG_HIP(hipMemcpyAsync(rows, (const uint8_t*)m->rows + first_row * m->row_bytes,
                         (size_t)n_rows * m->row_bytes, hipMemcpyDeviceToHost, m->dev->stream));
@@ -641,7 +617,7 @@ int mg_matrix_download(mg_matrix* m, uint64_t first_row, void* rows, uint64_t n_
 int mg_matrix_fill_seeded(mg_matrix* m, uint64_t seed) {
   if (!m) return fail(MG_ERR_INVALID_ARG, "matrix is null");
   std::lock_guard<std::mutex> lk(m->dev->mu);
-  int rc = set_device(m->dev);
+  int rc = mg::set_device(m->dev);
   if (rc) return rc;
   const uint64_t pairs = (uint64_t)m->n_rows * m->row_bytes / 16;
   if (pairs) MG_HIP(mg::launch_fill_splitmix(m->rows, 0, pairs, seed, m->dev->stream));
@@ -652,7 +628,7 @@ int mg_matrix_fill_seeded(mg_matrix* m, uint64_t seed) {
 int mg_matrix_fill_pattern(mg_matrix* m, uint64_t seed) {
   if (!m) return fail(MG_ERR_INVALID_ARG, "matrix is null");
   std::lock_guard<std::mutex> lk(m->dev->mu);
-  int rc = set_device(m->dev);
+  int rc = mg::set_device(m->dev);
   if (rc) return rc;
   const uint64_t blocks = (uint64_t)m->n_rows * m->row_bytes / 16;
   if (blocks) MG_HIP(mg::launch_fill_lcg(m->rows, blocks, seed, m->dev->stream));
@@ -670,13 +646,9 @@ int mg_matrix_info(const mg_matrix* m, uint64_t* n_rows, uint32_t* row_bytes) {
 int mg_matrix_destroy(mg_matrix* m) {
   if (!m) return MG_OK;
   std::lock_guard<std::mutex> lk(m->dev->mu);
-  set_device(m->dev);
+  (void)mg::set_device(m->dev);
   (void)hipStreamSynchronize(m->dev->stream);
-  if (m->dev->graph_exec) {
-    (void)hipGraphExecDestroy(m->dev->graph_exec);
-    m->dev->graph_exec = nullptr;
-    m->dev->graph_valid = false;
-  }
+  invalidate_graph(m->dev);
   (void)hipFree(m->rows);
   delete m;
   return MG_OK;
@@ -708,7 +680,7 @@ int mg_scan(mg_device* dev, mg_matrix* m, const uint8_t* keys, const size_t* key
   if (n_keys == 0) return MG_OK;
   if (!keys || !key_lens || !out) return fail(MG_ERR_INVALID_ARG, "null keys or output");
   std::lock_guard<std::mutex> lk(dev->mu);
-  int rc = set_device(dev);
+  int rc = mg::set_device(dev);
   if (rc) return rc;
   uint32_t d = 0;
   if ((rc = stage_keys(dev, keys, key_lens, n_keys, &d))) return rc;
@@ -719,6 +691,7 @@ int mg_scan_per_query(mg_device* dev, mg_matrix* m, const uint8_t* keys, const s
                       uint32_t n_keys, uint32_t keys_per_query, uint8_t* out,
                       const mg_scan_opts* opts, mg_timing* timing) {
   if (keys_per_query == 0) return fail(MG_ERR_INVALID_ARG, "keys_per_query is 0");
+  if (!m) return fail(MG_ERR_INVALID_ARG, "null matrix");
   if (!key_lens && n_keys) return fail(MG_ERR_INVALID_ARG, "null key lengths");
   mg_timing acc{};
   size_t off = 0;
@@ -749,7 +722,7 @@ int mg_delta_create(mg_device* dev, uint32_t row_bytes, uint64_t capacity, mg_de
   if (row_bytes == 0 || row_bytes % 16 != 0)
     return fail(MG_ERR_ROWS, "row width %u is not a positive multiple of 16", row_bytes);
   std::lock_guard<std::mutex> lk(dev->mu);
-  int rc = set_device(dev);
+  int rc = mg::set_device(dev);
   if (rc) return rc;
   mg_delta* dl = new mg_delta();
   dl->dev = dev;
@@ -766,7 +739,7 @@ int mg_delta_create(mg_device* dev, uint32_t row_bytes, uint64_t capacity, mg_de
     (void)hipFree(dl->d_uidx);
     (void)hipFree(dl->d_udiffs);
     delete dl;
-    return hip_fail(e, "hipMalloc(delta)");
+    return mg::hip_fail(e, "hipMalloc(delta)");
   }
   *out = dl;
   return MG_OK;
@@ -778,26 +751,13 @@ int mg_delta_set(mg_delta* dl, const uint64_t* row_idx, const uint8_t* diffs, ui
   if (n_entries > dl->capacity)
     return fail(MG_ERR_INVALID_ARG, "%llu entries exceed the delta capacity %llu",
                 (unsigned long long)n_entries, (unsigned long long)dl->capacity);
-  // Duplicate-free copy for merging: row -> XOR of its diffs.
-  std::map<uint64_t, std::vector<uint8_t>> uniq;
   uint64_t mx = 0;
-  for (uint64_t i = 0; i < n_entries; i++) {
-    mx = std::max(mx, row_idx[i]);
-    auto& v = uniq[row_idx[i]];
-    if (v.empty()) v.assign(dl->row_bytes, 0);
-    const uint8_t* src = diffs + i * dl->row_bytes;
-    for (uint32_t b = 0; b < dl->row_bytes; b++) v[b] ^= src[b];
-  }
+  for (uint64_t i = 0; i < n_entries; i++) mx = std::max(mx, row_idx[i]);
   std::vector<uint64_t> uidx;
   std::vector<uint8_t> udiff;
-  uidx.reserve(uniq.size());
-  udiff.reserve(uniq.size() * dl->row_bytes);
-  for (auto& kv : uniq) {
-    uidx.push_back(kv.first);
-    udiff.insert(udiff.end(), kv.second.begin(), kv.second.end());
-  }
+  mg::dedupe_delta(row_idx, diffs, n_entries, dl->row_bytes, &uidx, &udiff);
   std::lock_guard<std::mutex> lk(dl->dev->mu);
-  int rc = set_device(dl->dev);
+  int rc = mg::set_device(dl->dev);
   if (rc) return rc;
   hipStream_t s = dl->dev->stream;
   if (n_entries) {
@@ -824,13 +784,9 @@ int mg_delta_info(const mg_delta* dl, uint64_t* n_entries, uint64_t* epoch) {
 int mg_delta_destroy(mg_delta* dl) {
   if (!dl) return MG_OK;
   std::lock_guard<std::mutex> lk(dl->dev->mu);
-  set_device(dl->dev);
+  (void)mg::set_device(dl->dev);
   (void)hipStreamSynchronize(dl->dev->stream);
-  if (dl->dev->graph_exec) {
-    (void)hipGraphExecDestroy(dl->dev->graph_exec);
-    dl->dev->graph_exec = nullptr;
-    dl->dev->graph_valid = false;
-  }
+  invalidate_graph(dl->dev);
   (void)hipFree(dl->d_idx);
   (void)hipFree(dl->d_diffs);
   (void)hipFree(dl->d_uidx);
@@ -846,7 +802,7 @@ int mg_scan_delta(mg_device* dev, const mg_delta* dl, const uint8_t* keys, const
   if (n_keys == 0) return MG_OK;
   if (!keys || !key_lens || !out) return fail(MG_ERR_INVALID_ARG, "null keys or output");
   std::lock_guard<std::mutex> lk(dev->mu);
-  int rc = set_device(dev);
+  int rc = mg::set_device(dev);
   if (rc) return rc;
   uint32_t d = 0;
   if ((rc = stage_keys(dev, keys, key_lens, n_keys, &d))) return rc;
@@ -863,7 +819,6 @@ int mg_scan_delta(mg_device* dev, const mg_delta* dl, const uint8_t* keys, const
   hipStream_t s = dev->stream;
   MG_HIP(hipMemcpyAsync(dev->keys.p, dev->h_keys.p, sizeof(DevKey) * n_keys,
                         hipMemcpyHostToDevice, s));
-  MG_HIP(mg::launch_zero((uint4*)dev->out.p, (uint64_t)n_keys * chunks, s));
   if (dl->n) {
     mg::PointParams p{};
     p.rows = dl->d_diffs;
@@ -874,9 +829,9 @@ int mg_scan_delta(mg_device* dev, const mg_delta* dl, const uint8_t* keys, const
     p.n_keys = n_keys;
     p.partial = (uint4*)dev->partial_delta.p;
     MG_HIP(mg::launch_point(p, s));
-    MG_HIP(mg::launch_reduce((const uint4*)dev->partial_delta.p, parts, n_keys, chunks,
-                             (uint4*)dev->out.p, false, s));
   }
+  MG_HIP(mg::launch_reduce((const uint4*)dev->partial_delta.p, parts, n_keys, chunks,
+                           (uint4*)dev->out.p, nullptr, false, s));
   MG_HIP(hipMemcpyAsync(dev->h_out.p, dev->out.p, out_bytes, hipMemcpyDeviceToHost, s));
   MG_HIP(hipStreamSynchronize(s));
   memcpy(out, dev->h_out.p, out_bytes);
@@ -893,7 +848,7 @@ int mg_matrix_apply_delta(mg_matrix* m, const mg_delta* dl) {
     return fail(MG_ERR_ROWS, "delta row %llu outside the %llu-row matrix",
                 (unsigned long long)dl->max_idx, (unsigned long long)m->n_rows);
   std::lock_guard<std::mutex> lk(m->dev->mu);
-  int rc = set_device(m->dev);
+  int rc = mg::set_device(m->dev);
   if (rc) return rc;
   if (dl->n_unique)
     MG_HIP(mg::launch_xor_rows(m->rows, dl->d_uidx, dl->d_udiffs, dl->n_unique,

@@ -1,0 +1,624 @@
+// Continuous-batching request loop over one device-resident matrix.
+//
+// Replaces the reference server's GPU batch policy and dispatch
+// (crates/morphogen-server/src/network/gpu_batch.rs:129-149, gpu_scan.rs:36-76)
+// and its epoch-consistent scan wrapper (scan.rs:260-299) with a rolling sweep:
+//  - The matrix is cut into S segments of whole tiles. One sweep step runs the
+//    fused scan over one segment for every active key, XOR-accumulating into
+//    the key's device-resident accumulator.
+//  - A request is admitted, FIFO, as soon as enough key slots are free; its
+//    keys join at the next step and finish after S steps. The XOR fold is
+//    order-free, so each answer equals a whole-matrix scan.
+//  - Updates (epochs) append to a cumulative delta over the current base
+//    rows. A key's accumulator starts at the delta contribution of every
+//    update admitted before it (scan_delta_for_gpu, scan.rs:434-485), so its
+//    answer is the scan of base XOR updates at admission, however the sweep
+//    and later updates interleave with it.
+//  - When the cumulative delta grows past half its capacity the loop merges it
+//    into a base copy: into the spare copy when double-buffered (in-flight
+//    keys keep the old base), otherwise in place once in-flight keys drain.
+#include <algorithm>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <thread>
+
+#include "mg_internal.hpp"
+
+using mg::DevKey;
+using mg::fail;
+
+namespace {
+
+struct Request {
+  uint64_t ticket = 0;
+  uint32_t n_keys = 0;
+  std::vector<DevKey> keys;
+  std::vector<uint8_t> answers;
+  uint32_t keys_done = 0;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  uint64_t epoch = 0;
+  int status = MG_OK;
+  std::string error;
+  bool done = false;
+};
+
+struct Update {
+  std::vector<uint64_t> idx;
+  std::vector<uint8_t> diffs;
+  uint64_t epoch = 0;
+};
+
+// One entry of the submission-ordered queue: a request or an update.
+struct Event {
+  Request* req = nullptr;
+  std::unique_ptr<Update> upd;
+};
+
+struct Slot {
+  Request* req = nullptr;
+  uint32_t key = 0;        // index of the key within its request
+  uint32_t steps_done = 0;
+  int base = 0;
+};
+
+}  // namespace
+
+struct mg_server {
+  mg_device* dev = nullptr;
+  mg_matrix* m = nullptr;
+  uint32_t row_bytes = 0, chunks = 0, d = 0, kl = 0, segments = 0;
+  uint64_t n_rows = 0, seg_rows = 0;
+  uint32_t cap_keys = 0;
+  bool double_buffer = false;
+
+  // Base row copies: 0 is the caller's matrix, 1 the spare (double buffer).
+  uint4* base_rows[2] = {nullptr, nullptr};
+  int cur_base = 0;
+  bool rebase_wanted = false;
+
+  // Cumulative updates over the current base (host mirror + device copy).
+  std::vector<uint64_t> c_idx;
+  std::vector<uint8_t> c_diffs;
+  uint64_t c_cap = 0;
+  uint64_t* d_cidx = nullptr;
+  uint4* d_cdiffs = nullptr;
+  uint64_t rebases = 0;
+
+  // Key slots, device-resident keys and accumulators.
+  std::vector<Slot> slots;
+  std::vector<uint32_t> free_slots;
+  uint32_t active = 0;
+  DevKey* d_keys = nullptr;
+  uint4* d_acc = nullptr;
+  uint32_t* d_idx = nullptr;  // slot lists: [0, cap) scan list, [cap, 2cap) admission list
+  DevKey* d_stage_keys = nullptr;
+  mg::DevBuf partial, partial_delta;
+  mg::HostBuf h_idx, h_keys, h_acc;
+  uint32_t step = 0;
+
+  // Queue, shared with the API threads under `mu`.
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Event> queue;
+  std::map<uint64_t, std::unique_ptr<Request>> requests;
+  uint64_t next_ticket = 1;
+  uint64_t epoch = 0;
+  bool stop = false;
+  int fatal = MG_OK;
+  std::string fatal_msg;
+  mg_server_stats stats{};
+  std::thread thread;
+};
+
+namespace {
+
+void free_server_memory(mg_server* s) {
+  (void)hipFree(s->d_cidx);
+  (void)hipFree(s->d_cdiffs);
+  (void)hipFree(s->d_keys);
+  (void)hipFree(s->d_acc);
+  (void)hipFree(s->d_idx);
+  (void)hipFree(s->d_stage_keys);
+  if (s->base_rows[1]) (void)hipFree(s->base_rows[1]);
+  s->partial.release();
+  s->partial_delta.release();
+  s->h_idx.release();
+  s->h_keys.release();
+  s->h_acc.release();
+}
+
+// Keys pinned to base b that are still in flight.
+uint32_t keys_on_base(const mg_server* s, int b) {
+  uint32_t n = 0;
+  for (const Slot& sl : s->slots)
+    if (sl.req && sl.base == b) n++;
+  return n;
+}
+
+// Merges the cumulative updates into a base copy if that is possible now.
+// Caller holds dev->mu.
+int try_rebase(mg_server* s) {
+  if (!s->rebase_wanted) return MG_OK;
+  hipStream_t st = s->dev->stream;
+  int target;
+  if (s->double_buffer) {
+    target = 1 - s->cur_base;
+    if (keys_on_base(s, target)) return MG_OK;  // old keys still read it
+    MG_HIP(hipMemcpyAsync(s->base_rows[target], s->base_rows[s->cur_base],
+                          (size_t)s->n_rows * s->row_bytes, hipMemcpyDeviceToDevice, st));
+  } else {
+    target = s->cur_base;
+    if (s->active) return MG_OK;  // merge in place once nothing reads the base
+  }
+  std::vector<uint64_t> uidx;
+  std::vector<uint8_t> udiff;
+  mg::dedupe_delta(s->c_idx.data(), s->c_diffs.data(), s->c_idx.size(), s->row_bytes, &uidx,
+                   &udiff);
+  if (!uidx.empty()) {
+    // The cumulative-delta buffers are free to reuse as staging: they are cleared below.
+    MG_HIP(hipMemcpyAsync(s->d_cidx, uidx.data(), uidx.size() * 8, hipMemcpyHostToDevice, st));
+    MG_HIP(hipMemcpyAsync(s->d_cdiffs, udiff.data(), udiff.size(), hipMemcpyHostToDevice, st));
+    MG_HIP(mg::launch_xor_rows(s->base_rows[target], s->d_cidx, s->d_cdiffs, uidx.size(),
+                               s->chunks, st));
+  }
+  MG_HIP(hipStreamSynchronize(st));
+  s->cur_base = target;
+  s->c_idx.clear();
+  s->c_diffs.clear();
+  s->rebase_wanted = false;
+  s->rebases++;
+  return MG_OK;
+}
+
+// Appends one update to the cumulative delta. Caller holds dev->mu.
+int apply_update(mg_server* s, const Update& u) {
+  const uint64_t off = s->c_idx.size();
+  const uint64_t n = u.idx.size();
+  if (n) {
+    hipStream_t st = s->dev->stream;
+    MG_HIP(hipMemcpyAsync(s->d_cidx + off, u.idx.data(), n * 8, hipMemcpyHostToDevice, st));
+    MG_HIP(hipMemcpyAsync((uint8_t*)s->d_cdiffs + off * s->row_bytes, u.diffs.data(),
+                          u.diffs.size(), hipMemcpyHostToDevice, st));
+    MG_HIP(hipStreamSynchronize(st));
+  }
+  s->c_idx.insert(s->c_idx.end(), u.idx.begin(), u.idx.end());
+  s->c_diffs.insert(s->c_diffs.end(), u.diffs.begin(), u.diffs.end());
+  if (s->c_idx.size() * 2 > s->c_cap) s->rebase_wanted = true;
+  return MG_OK;
+}
+
+// Places the keys of newly admitted requests in free slots and sets their
+// accumulators to the contribution of the cumulative delta. Caller holds
+// dev->mu.
+int admit_keys(mg_server* s, const std::vector<Request*>& reqs) {
+  std::vector<uint32_t> new_slots;
+  std::vector<DevKey> keys;
+  for (Request* r : reqs)
+    for (uint32_t k = 0; k < r->n_keys; k++) {
+      const uint32_t sl = s->free_slots.back();
+      s->free_slots.pop_back();
+      s->slots[sl] = Slot{r, k, 0, s->cur_base};
+      new_slots.push_back(sl);
+      keys.push_back(r->keys[k]);
+    }
+  const uint32_t n = (uint32_t)new_slots.size();
+  if (!n) return MG_OK;
+  hipStream_t st = s->dev->stream;
+  uint32_t* h_idx = (uint32_t*)s->h_idx.p + s->cap_keys;
+  memcpy(h_idx, new_slots.data(), n * 4);
+  memcpy(s->h_keys.p, keys.data(), n * sizeof(DevKey));
+  uint32_t* d_list = s->d_idx + s->cap_keys;
+  MG_HIP(hipMemcpyAsync(d_list, h_idx, n * 4, hipMemcpyHostToDevice, st));
+  MG_HIP(hipMemcpyAsync(s->d_stage_keys, s->h_keys.p, n * sizeof(DevKey), hipMemcpyHostToDevice,
+                        st));
+  MG_HIP(mg::launch_scatter_u4((uint4*)s->d_keys, (const uint4*)s->d_stage_keys, d_list, n,
+                               sizeof(DevKey) / 16, st));
+  const uint64_t c_n = s->c_idx.size();
+  const uint32_t parts = (uint32_t)((c_n + 63) / 64);
+  int rc;
+  if (c_n) {
+    if ((rc = s->partial_delta.reserve((size_t)parts * n * s->row_bytes))) return rc;
+    mg::PointParams p{};
+    p.rows = s->d_cdiffs;
+    p.row_idx = s->d_cidx;
+    p.n_entries = c_n;
+    p.chunks = s->chunks;
+    p.keys = s->d_keys;
+    p.key_idx = d_list;
+    p.n_keys = n;
+    p.partial = (uint4*)s->partial_delta.p;
+    MG_HIP(mg::launch_point(p, st));
+  }
+  MG_HIP(mg::launch_reduce((const uint4*)s->partial_delta.p, parts, n, s->chunks, s->d_acc, d_list,
+                           false, st));
+  MG_HIP(hipStreamSynchronize(st));
+  return MG_OK;
+}
+
+// One sweep step over segment s->step for every active key; returns the
+// slots that have now seen every segment. Caller holds dev->mu.
+int sweep_step(mg_server* s, std::vector<uint32_t>* finished) {
+  hipStream_t st = s->dev->stream;
+  const uint64_t row_begin = (uint64_t)s->step * s->seg_rows;
+  const uint64_t row_end = std::min(s->n_rows, row_begin + s->seg_rows);
+  uint32_t* h_list = (uint32_t*)s->h_idx.p;
+  uint32_t off = 0;
+  for (int b = 0; b < 2; b++) {
+    const uint32_t start = off;
+    for (uint32_t i = 0; i < s->cap_keys; i++)
+      if (s->slots[i].req && s->slots[i].base == b) h_list[off++] = i;
+    const uint32_t n = off - start;
+    if (!n || row_end <= row_begin) continue;
+    mg_scan_opts o{};
+    o.kernel = MG_KERNEL_LANE_DFS;
+    if (s->chunks > 8) o.kernel = MG_KERNEL_STREAM;
+    o.lane_depth = s->kl;
+    o.row_begin = row_begin;
+    o.row_end = row_end;
+    mg::Plan pl;
+    int rc = mg::make_plan(s->dev, s->base_rows[b], s->n_rows, s->row_bytes, n, s->d, &o, &pl);
+    if (rc) return rc;
+    if ((rc = s->partial.reserve((size_t)pl.parts * n * s->row_bytes))) return rc;
+    MG_HIP(hipMemcpyAsync(s->d_idx + start, h_list + start, n * 4, hipMemcpyHostToDevice, st));
+    mg::ScanBuffers bufs;
+    bufs.keys = s->d_keys;
+    bufs.key_idx = s->d_idx + start;
+    bufs.partial = (uint4*)s->partial.p;
+    bufs.out = s->d_acc;
+    bufs.out_idx = s->d_idx + start;
+    bufs.accumulate = true;
+    if ((rc = mg::enqueue_scan(s->dev, pl, bufs, nullptr, nullptr))) return rc;
+    // The slot list region is reused by the next base's launch only after
+    // this one's kernels have read it.
+    MG_HIP(hipStreamSynchronize(st));
+  }
+  for (uint32_t i = 0; i < s->cap_keys; i++) {
+    Slot& sl = s->slots[i];
+    if (!sl.req) continue;
+    if (++sl.steps_done == s->segments) finished->push_back(i);
+  }
+  s->step = (s->step + 1) % s->segments;
+  if (!finished->empty()) {
+    uint8_t* h = (uint8_t*)s->h_acc.p;
+    for (size_t j = 0; j < finished->size(); j++)
+      MG_HIP(hipMemcpyAsync(h + j * s->row_bytes, (const uint8_t*)s->d_acc +
+                                                      (size_t)(*finished)[j] * s->row_bytes,
+                            s->row_bytes, hipMemcpyDeviceToHost, st));
+    MG_HIP(hipStreamSynchronize(st));
+  }
+  return MG_OK;
+}
+
+void fail_all(mg_server* s, int code, const std::string& msg) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->fatal = code;
+  s->fatal_msg = msg;
+  for (auto& kv : s->requests) {
+    Request* r = kv.second.get();
+    if (!r->done) {
+      r->done = true;
+      r->status = code;
+      r->error = msg;
+    }
+  }
+  s->cv_done.notify_all();
+}
+
+void server_loop(mg_server* s) {
+  (void)hipSetDevice(s->dev->ordinal);
+  for (;;) {
+    std::vector<std::unique_ptr<Update>> ups_before;  // applied before `admit`
+    std::vector<Request*> admit;
+    std::unique_ptr<Update> upd_after;  // an update queued behind admitted requests
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] {
+        return s->stop || !s->queue.empty() || s->active > 0 || s->rebase_wanted;
+      });
+      if (s->stop) break;
+      // Take events in submission order: updates, then the requests that fit.
+      // An update behind admitted requests waits for the next turn so those
+      // requests start from the state before it.
+      uint32_t free_n = (uint32_t)s->free_slots.size();
+      uint64_t c_n = s->c_idx.size();
+      while (!s->queue.empty()) {
+        Event& ev = s->queue.front();
+        if (ev.upd) {
+          if (!admit.empty()) break;
+          if (c_n + ev.upd->idx.size() > s->c_cap) {
+            s->rebase_wanted = true;  // room is made by a rebase first
+            break;
+          }
+          c_n += ev.upd->idx.size();
+          ups_before.push_back(std::move(ev.upd));
+          s->queue.pop_front();
+        } else {
+          // Single-buffered: a pending in-place merge stops admissions.
+          if (s->rebase_wanted && !s->double_buffer) break;
+          if (ev.req->n_keys > free_n) break;
+          free_n -= ev.req->n_keys;
+          admit.push_back(ev.req);
+          s->queue.pop_front();
+        }
+      }
+    }
+    const double t0 = mg::now_ms();
+    std::vector<uint32_t> finished;
+    int rc = MG_OK;
+    {
+      std::lock_guard<std::mutex> dlk(s->dev->mu);
+      rc = mg::set_device(s->dev);
+      if (!rc) rc = try_rebase(s);
+      for (auto& u : ups_before) {
+        if (rc) break;
+        rc = apply_update(s, *u);
+        if (!rc) {
+          std::lock_guard<std::mutex> lk(s->mu);
+          s->epoch = u->epoch;
+        }
+      }
+      if (!rc && !admit.empty()) {
+        const double now = mg::now_ms();
+        uint32_t n_new = 0;
+        {
+          std::lock_guard<std::mutex> lk(s->mu);
+          for (Request* r : admit) {
+            r->t_admit = now;
+            r->epoch = s->epoch;
+            n_new += r->n_keys;
+          }
+        }
+        rc = admit_keys(s, admit);
+        if (!rc) s->active += n_new;
+      }
+      if (!rc && s->active) rc = sweep_step(s, &finished);
+    }
+    if (rc) {
+      fail_all(s, rc, mg::last_error());
+      break;
+    }
+    const double t1 = mg::now_ms();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stats.busy_ms += t1 - t0;
+      if (s->active) s->stats.sweep_steps++;
+      const uint8_t* h = (const uint8_t*)s->h_acc.p;
+      for (size_t j = 0; j < finished.size(); j++) {
+        Slot& sl = s->slots[finished[j]];
+        Request* r = sl.req;
+        memcpy(r->answers.data() + (size_t)sl.key * s->row_bytes, h + j * s->row_bytes,
+               s->row_bytes);
+        if (++r->keys_done == r->n_keys) {
+          r->done = true;
+          r->t_done = t1;
+          s->stats.completed++;
+          s->stats.keys_completed += r->n_keys;
+        }
+        sl = Slot{};
+        s->free_slots.push_back(finished[j]);
+        s->active--;
+      }
+      s->stats.active_keys = s->active;
+      s->stats.delta_entries = s->c_idx.size();
+      s->stats.rebases = s->rebases;
+      if (!finished.empty()) s->cv_done.notify_all();
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mg_server_create(mg_device* dev, mg_matrix* m, const mg_server_opts* opts, mg_server** out) {
+  if (!dev || !m || !out) return fail(MG_ERR_INVALID_ARG, "null argument");
+  if (m->dev != dev) return fail(MG_ERR_INVALID_ARG, "matrix belongs to another device");
+  *out = nullptr;
+  mg_server_opts def{};
+  const mg_server_opts* o = opts ? opts : &def;
+  if (m->n_rows == 0) return fail(MG_ERR_ROWS, "empty matrix");
+  uint32_t d = o->domain_bits;
+  if (!d) d = std::max<uint32_t>(1, 64 - __builtin_clzll(m->n_rows - 1 ? m->n_rows - 1 : 1));
+  if (d > (uint32_t)mg::kMaxDomainBits) return fail(MG_ERR_DOMAIN, "domain bits %u > 32", d);
+  if (d < 64 && m->n_rows > (1ull << d))
+    return fail(MG_ERR_ROWS, "%llu rows exceed the 2^%u-leaf domain",
+                (unsigned long long)m->n_rows, d);
+  if (m->row_bytes > 4096) return fail(MG_ERR_ROWS, "server rows are at most 4096 bytes");
+
+  std::unique_ptr<mg_server> s(new mg_server());
+  s->dev = dev;
+  s->m = m;
+  s->row_bytes = m->row_bytes;
+  s->chunks = m->row_bytes / 16;
+  s->d = d;
+  s->n_rows = m->n_rows;
+  s->double_buffer = o->double_buffer != 0;
+  // Segments are whole tiles of the fused kernel (2^(kl + lb) rows).
+  s->kl = std::min<uint32_t>(o->lane_depth ? o->lane_depth : (s->chunks > 8 ? 6 : 10), d);
+  const uint32_t lb = std::min<uint32_t>(6, d - s->kl);
+  const uint64_t tile = 1ull << (s->kl + lb);
+  const uint32_t want = o->segments ? o->segments : 64;
+  uint64_t seg = (s->n_rows + want - 1) / want;
+  seg = (seg + tile - 1) / tile * tile;
+  s->seg_rows = seg;
+  s->segments = (uint32_t)((s->n_rows + seg - 1) / seg);
+  s->c_cap = o->max_delta_entries ? o->max_delta_entries : (1ull << 20);
+
+  std::lock_guard<std::mutex> lk(dev->mu);
+  int rc = mg::set_device(dev);
+  if (rc) return rc;
+  size_t free_b = 0, total_b = 0;
+  MG_HIP(hipMemGetInfo(&free_b, &total_b));
+  const size_t matrix_bytes = (size_t)m->n_rows * m->row_bytes;
+  const size_t delta_bytes = s->c_cap * (8 + (size_t)m->row_bytes);
+  size_t need = delta_bytes + (s->double_buffer ? matrix_bytes : 0) + o->hbm_reserve_bytes;
+  if (need > free_b)
+    return fail(MG_ERR_OUT_OF_MEMORY, "server needs %zu B besides the matrix, %zu B free", need,
+                free_b);
+  // Per key: its record, accumulator, two slot-list entries and a share of
+  // the partial buffers; 64 MiB is kept for workspace growth.
+  const size_t per_key = sizeof(DevKey) * 2 + 3 * (size_t)m->row_bytes + 8;
+  size_t budget = free_b - need;
+  budget = budget > (64u << 20) ? budget - (64u << 20) : 0;
+  uint64_t cap = std::min<uint64_t>(budget / per_key, 32768);
+  if (o->max_active_keys) cap = std::min<uint64_t>(cap, o->max_active_keys);
+  if (cap == 0) return fail(MG_ERR_OUT_OF_MEMORY, "no HBM left for key slots");
+  s->cap_keys = (uint32_t)cap;
+
+  hipError_t e = hipMalloc((void**)&s->d_cidx, s->c_cap * 8);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->d_cdiffs, s->c_cap * m->row_bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->d_keys, cap * sizeof(DevKey));
+  if (e == hipSuccess) e = hipMalloc((void**)&s->d_acc, cap * m->row_bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->d_idx, cap * 2 * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->d_stage_keys, cap * sizeof(DevKey));
+  if (e == hipSuccess && s->double_buffer) e = hipMalloc((void**)&s->base_rows[1], matrix_bytes);
+  if (e != hipSuccess) {
+    free_server_memory(s.get());
+    return mg::hip_fail(e, "hipMalloc(server)");
+  }
+  if ((rc = s->h_idx.reserve(cap * 2 * 4)) || (rc = s->h_keys.reserve(cap * sizeof(DevKey))) ||
+      (rc = s->h_acc.reserve(cap * m->row_bytes))) {
+    free_server_memory(s.get());
+    return rc;
+  }
+  s->base_rows[0] = m->rows;
+  s->slots.assign(cap, Slot{});
+  s->free_slots.reserve(cap);
+  for (uint32_t i = (uint32_t)cap; i-- > 0;) s->free_slots.push_back(i);
+  s->stats.max_active_keys = (uint32_t)cap;
+  s->stats.segments = s->segments;
+  s->thread = std::thread(server_loop, s.get());
+  *out = s.release();
+  return MG_OK;
+}
+
+int mg_server_submit(mg_server* s, const uint8_t* keys, const size_t* key_lens, uint32_t n_keys,
+                     uint64_t* ticket) {
+  if (!s || !ticket || (n_keys && (!keys || !key_lens)))
+    return fail(MG_ERR_INVALID_ARG, "null argument");
+  if (n_keys == 0) return fail(MG_ERR_INVALID_ARG, "a request needs at least one key");
+  if (n_keys > s->cap_keys)
+    return fail(MG_ERR_BUSY, "request of %u keys exceeds the %u key slots", n_keys, s->cap_keys);
+  std::unique_ptr<Request> r(new Request());
+  r->keys.resize(n_keys);
+  uint32_t d = 0;
+  int rc = mg::parse_keys(keys, key_lens, n_keys, r->keys.data(), &d);
+  if (rc) return rc;
+  if (d != s->d) return fail(MG_ERR_DOMAIN, "keys have domain %u, the server serves %u", d, s->d);
+  r->n_keys = n_keys;
+  r->answers.assign((size_t)n_keys * s->row_bytes, 0);
+  r->t_submit = mg::now_ms();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->fatal) return fail(s->fatal, "server stopped: %s", s->fatal_msg.c_str());
+  r->ticket = s->next_ticket++;
+  *ticket = r->ticket;
+  Event ev;
+  ev.req = r.get();
+  s->queue.push_back(std::move(ev));
+  s->requests[r->ticket] = std::move(r);
+  s->stats.submitted++;
+  s->cv_work.notify_one();
+  return MG_OK;
+}
+
+int mg_server_wait(mg_server* s, uint64_t ticket, uint8_t* out, uint32_t timeout_ms,
+                   mg_request_info* info) {
+  if (!s) return fail(MG_ERR_INVALID_ARG, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->requests.find(ticket);
+  if (it == s->requests.end()) return fail(MG_ERR_INVALID_ARG, "unknown ticket %llu",
+                                           (unsigned long long)ticket);
+  Request* r = it->second.get();
+  auto ready = [&] { return r->done; };
+  if (timeout_ms) {
+    if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+      return fail(MG_ERR_BUSY, "request %llu not done after %u ms", (unsigned long long)ticket,
+                  timeout_ms);
+  } else {
+    s->cv_done.wait(lk, ready);
+  }
+  if (r->status != MG_OK) {
+    const int code = r->status;
+    const std::string msg = r->error;
+    s->requests.erase(it);
+    return fail(code, "request failed: %s", msg.c_str());
+  }
+  if (out) memcpy(out, r->answers.data(), r->answers.size());
+  if (info) {
+    info->queue_ms = r->t_admit - r->t_submit;
+    info->service_ms = r->t_done - r->t_admit;
+    info->latency_ms = r->t_done - r->t_submit;
+    info->epoch = r->epoch;
+    info->n_keys = r->n_keys;
+  }
+  s->requests.erase(it);
+  return MG_OK;
+}
+
+int mg_server_update(mg_server* s, const uint64_t* row_idx, const uint8_t* diffs,
+                     uint64_t n_entries, uint64_t epoch) {
+  if (!s || (n_entries && (!row_idx || !diffs))) return fail(MG_ERR_INVALID_ARG, "null argument");
+  if (n_entries > s->c_cap)
+    return fail(MG_ERR_INVALID_ARG, "%llu update rows exceed the delta capacity %llu",
+                (unsigned long long)n_entries, (unsigned long long)s->c_cap);
+  std::unique_ptr<Update> u(new Update());
+  u->idx.assign(row_idx, row_idx + n_entries);
+  for (uint64_t i : u->idx)
+    if (i >= s->n_rows)
+      return fail(MG_ERR_ROWS, "update row %llu outside the %llu-row matrix",
+                  (unsigned long long)i, (unsigned long long)s->n_rows);
+  u->diffs.assign(diffs, diffs + n_entries * s->row_bytes);
+  u->epoch = epoch;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->fatal) return fail(s->fatal, "server stopped: %s", s->fatal_msg.c_str());
+  Event ev;
+  ev.upd = std::move(u);
+  s->queue.push_back(std::move(ev));
+  s->cv_work.notify_one();
+  return MG_OK;
+}
+
+int mg_server_stats_get(mg_server* s, mg_server_stats* out) {
+  if (!s || !out) return fail(MG_ERR_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->stats;
+  out->epoch = s->epoch;
+  uint32_t q = 0;
+  for (const Event& ev : s->queue)
+    if (ev.req) q++;
+  out->queued_requests = q;
+  return MG_OK;
+}
+
+int mg_server_destroy(mg_server* s) {
+  if (!s) return MG_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    for (auto& kv : s->requests)
+      if (!kv.second->done) {
+        kv.second->done = true;
+        kv.second->status = MG_ERR_STATE;
+        kv.second->error = "server destroyed";
+      }
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
+  }
+  if (s->thread.joinable()) s->thread.join();
+  {
+    std::lock_guard<std::mutex> dlk(s->dev->mu);
+    (void)mg::set_device(s->dev);
+    (void)hipStreamSynchronize(s->dev->stream);
+    // A double-buffered server may have left the newest rows in its spare copy.
+    if (s->cur_base == 1)
+      (void)hipMemcpy(s->m->rows, s->base_rows[1], (size_t)s->n_rows * s->row_bytes,
+                      hipMemcpyDeviceToDevice);
+    free_server_memory(s);
+  }
+  delete s;
+  return MG_OK;
+}
+
+}  // extern "C"

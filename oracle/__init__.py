@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: used by tests/, by the smoke check and by the CPU
-baseline leg of bench.py, always as the checker or the timed CPU baseline,
+baseline leg of bench.py, always as the reference answers to compare against or the timed CPU baseline,
 never as part of the product path. See oracle.h for the reference functions
 each entry point restates and how each one is pinned.
 """

@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the ChaCha8 DPF + masked-XOR PIR scan.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker that the HIP path is
+ * TEST INFRASTRUCTURE ONLY. This library computes the expected answers the HIP path is
  * compared against, and the CPU baseline that is timed beside it. Nothing in
  * the product library links, loads or calls it.
  *

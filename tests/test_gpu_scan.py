@@ -1,7 +1,7 @@
 """GPU parity: the HIP scan against the CPU oracle, byte for byte.
 
 Runs through the C ABI (morphogenesis_amd -> libmorphogen_hip.so). The oracle
-is the checker only. Covers the reference's own test cases
+only supplies expected answers. Covers the reference's own test cases
 (kernel.rs:1443-1677, scan.rs:1274-1572, network_api.rs batch == singles) at
 W-byte rows, plus every kernel variant, tile/range edges, fuzzed control
 bytes, delta scans and the device fills.
