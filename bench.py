@@ -85,7 +85,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rows-log2", type=int, default=31)
     ap.add_argument("--rows", type=int, default=0, help="explicit row count (overrides --rows-log2)")
-    ap.add_argument("--kernel", choices=["auto", "lane_dfs", "stream", "point"], default="auto")
+    ap.add_argument("--kernel", choices=["auto", "lane_dfs", "stream", "point", "suffix"],
+                    default="auto")
+    ap.add_argument("--ref-plain-steps", type=int, default=1,
+                    help="passes of the reference-shaped per-row suffix-walk kernel, 3 keys each")
     ap.add_argument("--keys-per-wave", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=0)
     ap.add_argument("--row-bytes", type=int, default=32)
@@ -98,6 +101,8 @@ def main():
     ap.add_argument("--cpu-sample-log2", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=31)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy a 4 GiB matrix once (known bytes for counter scaling)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -119,6 +124,15 @@ def main():
     info = scanner.info()
     log(f"device {info['name']} CUs={info['compute_units']} free={info['free_bytes'] / 2**30:.1f} GiB")
 
+    if args.calibrate_copy:
+        ca = mg.GpuPageMatrix(scanner, 1 << 27, 32)
+        cb = mg.GpuPageMatrix(scanner, 1 << 27, 32)
+        ca.fill_seeded(1)
+        cb.copy_from(ca)
+        log("calibration copy: 4 GiB read + 4 GiB written by copy_u4")
+        ca.close()
+        cb.close()
+
     t0 = time.perf_counter()
     m = mg.GpuPageMatrix(scanner, n_rows, w)
     m.fill_seeded(args.seed)
@@ -131,7 +145,8 @@ def main():
     keys = [k for p in pairs for k in p]
     target_rows = [m.download(t, 1)[0] for t in targets]
     kernel = {"auto": mg.KERNEL_AUTO, "lane_dfs": mg.KERNEL_LANE_DFS,
-              "stream": mg.KERNEL_STREAM, "point": mg.KERNEL_POINT}[args.kernel]
+              "stream": mg.KERNEL_STREAM, "point": mg.KERNEL_POINT,
+              "suffix": mg.KERNEL_SUFFIX}[args.kernel]
     opts = mg.make_opts(kernel=kernel, lane_depth=args.lane_depth,
                         keys_per_wave=args.keys_per_wave, waves_per_key=args.waves_per_key,
                         waves_per_block=args.waves_per_block, use_graph=args.use_graph)
@@ -167,6 +182,19 @@ def main():
         scanner.scan_keys(m, plain_keys, opts)
         plain_ms.append((time.perf_counter() - s0) * 1e3)
         log(f"plain pass {i}: {plain_ms[-1]:.1f} ms")
+
+    # Reference-shaped plain kernel: one query per pass, an 11-level path walk
+    # per row from a per-tile prefix (fused_kernel.cu:309-360), same matrix.
+    ref_ms = []
+    if w <= 128:
+        ref_opts = mg.make_opts(kernel=mg.KERNEL_SUFFIX)
+        for i in range(args.ref_plain_steps):
+            s0 = time.perf_counter()
+            ans_ref = scanner.scan_keys(m, keys[:2], ref_opts)
+            ref_ms.append((time.perf_counter() - s0) * 1e3 * KEYS_PER_QUERY / 2)
+            if not np.array_equal(ans_ref[0] ^ ans_ref[1], target_rows[0]):
+                raise SystemExit("bit-exactness failure in the suffix-walk baseline")
+            log(f"reference-shaped plain pass {i}: {ref_ms[-1]:.1f} ms per 3 keys")
 
     queries_per_step = q / KEYS_PER_QUERY
     qps = queries_per_step * args.steps / wall
@@ -218,6 +246,8 @@ def main():
         "keys_per_s": q * args.steps / wall,
         "plain_queries_per_s": plain_qps,
         "batched_vs_plain": (qps / plain_qps) if plain_qps else None,
+        "plain_reference_shaped_queries_per_s": (1e3 / statistics.mean(ref_ms)) if ref_ms else None,
+        "batched_vs_reference_shaped_plain": (qps * statistics.mean(ref_ms) / 1e3) if ref_ms else None,
         "scan_gbs": scan_gbs,
         "bit_exact": True,
         "roofline": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS,

@@ -76,7 +76,11 @@ enum {
   MG_KERNEL_AUTO = 0,
   MG_KERNEL_LANE_DFS = 1, /* per-lane subtree DFS, rows of <= 128 B       */
   MG_KERNEL_STREAM = 2,   /* LDS-staged masks, coalesced wide rows         */
-  MG_KERNEL_POINT = 3     /* one full path walk per row (reference-style)  */
+  MG_KERNEL_POINT = 3,    /* one full path walk per row                    */
+  MG_KERNEL_SUFFIX = 4    /* comparison baseline shaped like the reference
+                             kernel (fused_kernel.cu:309-360): 2048-row
+                             tiles, one prefix walk per tile, an 11-level
+                             walk per row; rows of <= 128 B               */
 };
 
 /* Scan options; zero-initialise for defaults. The tuning fields replace the
@@ -122,6 +126,9 @@ int mg_matrix_fill_seeded(mg_matrix *m, uint64_t seed);
  * the device with LCG jump-ahead. */
 int mg_matrix_fill_pattern(mg_matrix *m, uint64_t seed);
 int mg_matrix_info(const mg_matrix *m, uint64_t *n_rows, uint32_t *row_bytes);
+/* Device-to-device copy of all rows between two matrices of one shape (a
+ * snapshot for rotation, epoch.rs:407-463 submit_snapshot). */
+int mg_matrix_copy(mg_matrix *dst, const mg_matrix *src);
 int mg_matrix_destroy(mg_matrix *m);
 
 /* ChaChaKey::from_bytes (dpf.rs:277-316): exact length 19 + 18d + 16, plus

@@ -30,8 +30,9 @@ MG_ERR_NO_DEVICE = 7
 MG_ERR_BUSY = 8
 MG_ERR_STATE = 9
 
-KERNEL_AUTO, KERNEL_LANE_DFS, KERNEL_STREAM, KERNEL_POINT = range(4)
-KERNEL_NAMES = {KERNEL_LANE_DFS: "lane_dfs", KERNEL_STREAM: "stream", KERNEL_POINT: "point"}
+KERNEL_AUTO, KERNEL_LANE_DFS, KERNEL_STREAM, KERNEL_POINT, KERNEL_SUFFIX = range(5)
+KERNEL_NAMES = {KERNEL_LANE_DFS: "lane_dfs", KERNEL_STREAM: "stream", KERNEL_POINT: "point",
+                KERNEL_SUFFIX: "suffix"}
 
 
 class MgError(RuntimeError):
@@ -140,6 +141,7 @@ def _load():
         "mg_matrix_fill_seeded": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "mg_matrix_fill_pattern": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "mg_matrix_info": (ctypes.c_int, [vp, u64p, ctypes.POINTER(ctypes.c_uint32)]),
+        "mg_matrix_copy": (ctypes.c_int, [vp, vp]),
         "mg_matrix_destroy": (ctypes.c_int, [vp]),
         "mg_keygen": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, u8p,
                                      u8p]),
@@ -390,6 +392,9 @@ class GpuPageMatrix:
 
     def fill_with_pattern(self, seed: int):
         _check(lib.mg_matrix_fill_pattern(self._h, seed))
+
+    def copy_from(self, other: "GpuPageMatrix"):
+        _check(lib.mg_matrix_copy(self._h, other._h))
 
     def apply_delta(self, delta: "DeltaBuffer"):
         _check(lib.mg_matrix_apply_delta(self._h, delta._h))

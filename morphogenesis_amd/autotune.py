@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--kernel", choices=["all", "lane_dfs", "stream"], default="all")
     ap.add_argument("--save", action="store_true", help="store the winner in tuned.json")
+    ap.add_argument("--config", default="",
+                    help="run one setting only: kernel,lane_depth,keys_per_wave,waves_per_block")
     args = ap.parse_args()
     n_rows = args.rows or (1 << args.rows_log2)
     d = args.domain_bits or max(1, (n_rows - 1).bit_length())
@@ -119,6 +121,9 @@ def main():
     if args.kernel != "all":
         kid = KERNEL_LANE_DFS if args.kernel == "lane_dfs" else KERNEL_STREAM
         grid = [c for c in grid if c["kernel"] == kid]
+    if args.config:
+        k, kl, g, wpb = (int(x) for x in args.config.split(","))
+        grid = [dict(kernel=k, lane_depth=kl, keys_per_wave=g, waves_per_block=wpb)]
     t0 = time.time()
     best, rows = tune(scanner, m, keys, args.reps, grid,
                       log=lambda s: print(s, file=sys.stderr, flush=True))

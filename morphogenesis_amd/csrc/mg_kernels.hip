@@ -32,6 +32,25 @@ __device__ __forceinline__ uint4 wave_xor4(uint4 v) {
   return make_uint4(wave_xor(v.x), wave_xor(v.y), wave_xor(v.z), wave_xor(v.w));
 }
 
+// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// The launch has a multiple of 8 blocks; logical block L = (b % 8) * per + b / 8
+// gives each XCD a contiguous range of logical blocks, so the waves that share
+// a tile range (same part, different keys) run on one XCD and read their rows
+// through one L2. Logical waves past `total` are idle.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_logical_wave(uint32_t total, uint32_t wpb, uint32_t wib) {
+  const uint32_t blocks = (total + wpb - 1) / wpb;
+  const uint32_t per = (blocks + kXcds - 1) / kXcds;
+  const uint32_t b = blockIdx.x;
+  return ((b % kXcds) * per + b / kXcds) * wpb + wib;
+}
+
+static uint32_t xcd_grid(uint32_t total, uint32_t wpb) {
+  const uint32_t blocks = (total + wpb - 1) / wpb;
+  return (blocks + kXcds - 1) / kXcds * kXcds;
+}
+
 // A path of wave-uniform nodes, one level per lane: lane L holds level L.
 // Reads use readlane with a uniform index; writes are a per-lane select.
 struct LanePath {
@@ -84,7 +103,7 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   extern __shared__ uint4 lds[];
   const int lane = threadIdx.x & 63;
   const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + wib;
+  const uint32_t w = xcd_logical_wave(p.parts * p.n_keys, blockDim.x >> 6, wib);
   if (w >= p.parts * p.n_keys) return;
   const uint32_t key_i = w % p.n_keys;
   const uint32_t part = w / p.n_keys;
@@ -185,6 +204,46 @@ __global__ __launch_bounds__(256) void scan_point(PointParams p) {
   }
 }
 
+// ---------------------------------------------------- suffix-walk baseline --
+template <int C>
+__global__ __launch_bounds__(256) void scan_suffix(SuffixParams p) {
+  const int lane = threadIdx.x & 63;
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wib;
+  if (t >= p.n_tiles) return;
+  const uint32_t key_i = blockIdx.y;
+  const DevKey* __restrict__ k = p.keys + (p.key_idx ? p.key_idx[key_i] : key_i);
+  const int d = (int)p.d, s = (int)p.s;
+  const uint64_t tile = p.tile_begin + t;
+  const uint64_t start = tile << s;
+  Node root = key_root(k);
+  for (int lvl = 0; lvl < d - s; lvl++)
+    root = ggm_step(k, lvl, root, (uint32_t)(start >> (d - 1 - lvl)) & 1u);
+  uint4 acc[C];
+#pragma unroll
+  for (int c = 0; c < C; c++) acc[c] = make_uint4(0, 0, 0, 0);
+  const uint32_t tile_rows = 1u << s;
+  for (uint32_t it = 0; it < tile_rows; it += 64) {
+    const uint32_t off = it + lane;
+    const uint64_t x = start + off;
+    const bool valid = off < tile_rows && x < p.n_rows;
+    Node n = root;
+    for (int j = 0; j < s; j++)
+      n = ggm_step(k, d - s + j, n, (uint32_t)(x >> (s - 1 - j)) & 1u);
+    const uint4 m = leaf_mask(k, n);
+    if (valid) {
+      const uint4* src = p.rows + x * C;
+#pragma unroll
+      for (int c = 0; c < C; c++) xor_masked(acc[c], src[c], m);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    const uint4 v = wave_xor4(acc[c]);
+    if (lane == 0) p.partial[((size_t)t * p.n_keys + key_i) * C + c] = v;
+  }
+}
+
 // ------------------------------------------------------- stream (wide) ----
 // G keys per wave, rows of C = 64*S chunks or fewer. Every leaf-pair step of
 // the lane DFS leaves 2 masks per lane per key in LDS; the wave then reads
@@ -195,8 +254,8 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
   extern __shared__ uint4 lds[];
   const int lane = threadIdx.x & 63;
   const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + wib;
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
+  const uint32_t w = xcd_logical_wave(p.parts * n_groups, blockDim.x >> 6, wib);
   if (w >= p.parts * n_groups) return;
   const uint32_t grp = w % n_groups;
   const uint32_t part = w / n_groups;
@@ -239,12 +298,52 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
     path[g].s3 = r.s3; path[g].t = r.t;
   }
 
+  // Rows of one leaf-pair step: 2 per live source lane s, tile_base + s*2^kl +
+  // 2*pair + {0,1}; row r of the step uses mask slot r. They are read in
+  // batches of U rows (8 KiB of 1 KiB rows), two batches in flight: the first
+  // two batches of a step are issued before that step's PRG work, so loads
+  // overlap the ChaCha rounds.
+  constexpr int U = 8 / S;
+  const uint32_t step_rows = 2u * (uint32_t)lanes_live;
+  uint4 va[U][S], vb[U][S];
+  auto load_batch = [&](uint64_t tile_base, uint32_t pair, uint32_t r0, uint4 (&v)[U][S]) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t r = r0 + u;
+      const uint64_t x = tile_base + ((uint64_t)(r >> 1) << kl) + 2 * (uint64_t)pair + (r & 1);
+      const bool ok = r < step_rows && x < p.n_rows;
+      const uint4* src = p.rows + x * C;
+#pragma unroll
+      for (int j = 0; j < S; j++) {
+        const uint32_t c = lane + 64 * j;
+        v[u][j] = (ok && c < C) ? load_nt(src + c) : make_uint4(0, 0, 0, 0);
+      }
+    }
+  };
+  auto fold_batch = [&](uint32_t r0, const uint4 (&v)[U][S]) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t r = (r0 + u) & 127u;
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const uint4 m = masks[g * 128 + r];
+#pragma unroll
+        for (int j = 0; j < S; j++) xor_masked(acc[g][j], v[u][j], m);
+      }
+    }
+  };
+  auto prefetch_step = [&](uint64_t tile_base, uint32_t pair) {
+    load_batch(tile_base, pair, 0, va);
+    if (U < step_rows) load_batch(tile_base, pair, U, vb);
+  };
+
   for (uint64_t tile = t_begin; tile < t_end; ++tile) {
+    const uint64_t tile_base = tile << (kl + lb);
+    prefetch_step(tile_base, 0);
     Node node[G];
 #pragma unroll
     for (int g = 0; g < G; g++)
       node[g] = tile_lane_root(kp[g], path[g], lane, tile, tile - 1, tile == t_begin, tb, lb);
-    const uint64_t tile_base = tile << (kl + lb);
     int lvl = 0;
     uint32_t pair = 0;
     for (;;) {
@@ -267,36 +366,16 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
         ++lvl;
         continue;
       }
-      // 2 rows per live source lane s: tile_base + s*2^kl + 2*pair + {0,1};
-      // row r of the step uses mask slot r. Loads go out U rows at a time.
-      constexpr int U = 8;
-      const uint32_t step_rows = 2u * (uint32_t)lanes_live;
-      for (uint32_t r0 = 0; r0 < step_rows; r0 += U) {
-        uint4 v[U][S];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          const uint32_t r = r0 + u;
-          const uint64_t x = tile_base + ((uint64_t)(r >> 1) << kl) + 2 * (uint64_t)pair + (r & 1);
-          const bool ok = r < step_rows && x < p.n_rows;
-          const uint4* src = p.rows + x * C;
-#pragma unroll
-          for (int j = 0; j < S; j++) {
-            const uint32_t c = lane + 64 * j;
-            v[u][j] = (ok && c < C) ? load_nt(src + c) : make_uint4(0, 0, 0, 0);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          const uint32_t r = (r0 + u) & 127u;
-#pragma unroll
-          for (int g = 0; g < G; g++) {
-            const uint4 m = masks[g * 128 + r];
-#pragma unroll
-            for (int j = 0; j < S; j++) xor_masked(acc[g][j], v[u][j], m);
-          }
-        }
+      // va / vb already hold batches 0 and 1 of this step.
+      for (uint32_t r0 = 0; r0 < step_rows; r0 += 2 * U) {
+        fold_batch(r0, va);
+        if (r0 + 2 * U < step_rows) load_batch(tile_base, pair, r0 + 2 * U, va);
+        if (r0 + U >= step_rows) break;
+        fold_batch(r0 + U, vb);
+        if (r0 + 3 * U < step_rows) load_batch(tile_base, pair, r0 + 3 * U, vb);
       }
       if (++pair == n_pairs) break;
+      prefetch_step(tile_base, pair);
       lvl = kl - 1 - __builtin_ctz(pair);
 #pragma unroll
       for (int g = 0; g < G; g++) {
@@ -407,6 +486,13 @@ __global__ __launch_bounds__(256) void zero_u4(uint4* __restrict__ out, uint64_t
     out[i] = make_uint4(0, 0, 0, 0);
 }
 
+__global__ __launch_bounds__(256) void copy_u4(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                               uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // rows[idx[e]] ^= diffs[e]; one lane per 16-byte chunk. Duplicate indices
 // are applied in one pass each by the host (see apply_delta), so no atomics.
 __global__ __launch_bounds__(256) void xor_rows(uint4* __restrict__ rows, const uint64_t* __restrict__ idx,
@@ -448,12 +534,27 @@ hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_blo
                            hipStream_t s) {
   const uint32_t total = p.parts * p.n_keys;
   const dim3 block(64 * waves_per_block);
-  const dim3 grid((total + waves_per_block - 1) / waves_per_block);
+  const dim3 grid(xcd_grid(total, waves_per_block));
   const size_t lds = (size_t)waves_per_block * p.kl * 80 * sizeof(uint4);
   switch (chunks) {
 #define MG_CASE(C)                                                            \
   case C:                                                                     \
     hipLaunchKernelGGL(scan_lane_dfs<C>, grid, block, lds, s, p);             \
+    break;
+    MG_CASE(1) MG_CASE(2) MG_CASE(3) MG_CASE(4) MG_CASE(5) MG_CASE(6) MG_CASE(7) MG_CASE(8)
+#undef MG_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s) {
+  const dim3 grid((uint32_t)((p.n_tiles + 3) / 4), p.n_keys);
+  switch (chunks) {
+#define MG_CASE(C)                                                            \
+  case C:                                                                     \
+    hipLaunchKernelGGL(scan_suffix<C>, grid, dim3(256), 0, s, p);             \
     break;
     MG_CASE(1) MG_CASE(2) MG_CASE(3) MG_CASE(4) MG_CASE(5) MG_CASE(6) MG_CASE(7) MG_CASE(8)
 #undef MG_CASE
@@ -472,7 +573,7 @@ hipError_t launch_stream(int S, int G, const StreamParams& p, int waves_per_bloc
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
   const uint32_t total = p.parts * n_groups;
   const dim3 block(64 * waves_per_block);
-  const dim3 grid((total + waves_per_block - 1) / waves_per_block);
+  const dim3 grid(xcd_grid(total, waves_per_block));
   const size_t lds = stream_lds_bytes(G, p.kl, waves_per_block);
 #define MG_SG(SS, GG)                                                         \
   if (S == SS && G == GG) {                                                   \
@@ -536,6 +637,11 @@ hipError_t launch_fill_splitmix(uint4* out, uint64_t first_pair, uint64_t n_pair
 hipError_t launch_fill_lcg(uint4* out, uint64_t n_blocks16, uint64_t seed, hipStream_t s) {
   hipLaunchKernelGGL(fill_lcg, dim3(grid_for(n_blocks16, 256)), dim3(256), 0, s, out, n_blocks16,
                      seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(uint4* dst, const uint4* src, uint64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(copy_u4, dim3(grid_for(n, 256)), dim3(256), 0, s, dst, src, n);
   return hipGetLastError();
 }
 

@@ -56,6 +56,21 @@ struct PointParams {
   uint4* partial;  // [ceil(n_entries/64)][n_keys][chunks]
 };
 
+// Reference-shaped baseline (comparison only): 2^s-row tiles (s = min(11, d)),
+// one prefix walk per tile, then an s-level path walk per row, as the
+// reference's fused kernel does (fused_kernel.cu:309-360).
+struct SuffixParams {
+  const uint4* rows;
+  uint64_t n_rows;
+  const DevKey* keys;
+  const uint32_t* key_idx;
+  uint32_t n_keys;
+  uint32_t d, s;
+  uint64_t tile_begin, n_tiles;
+  uint4* partial;  // [n_tiles][n_keys][chunks]
+};
+
+hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s);
 hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_block,
                            hipStream_t s);
 hipError_t launch_stream(int S, int G, const StreamParams& p, int waves_per_block,
@@ -72,6 +87,7 @@ hipError_t launch_fill_splitmix(uint4* out, uint64_t first_pair, uint64_t n_pair
                                 hipStream_t s);
 hipError_t launch_fill_lcg(uint4* out, uint64_t n_blocks16, uint64_t seed, hipStream_t s);
 hipError_t launch_zero(uint4* out, uint64_t n, hipStream_t s);
+hipError_t launch_copy(uint4* dst, const uint4* src, uint64_t n, hipStream_t s);
 // dst[idx[e] * per + j] = src[e * per + j] for e < n, j < per (16-byte units).
 hipError_t launch_scatter_u4(uint4* dst, const uint4* src, const uint32_t* idx, uint32_t n,
                              uint32_t per, hipStream_t s);

@@ -196,7 +196,7 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   if (o->lane_depth > 16) return fail(MG_ERR_INVALID_ARG, "lane_depth must be 0-16");
 
   uint32_t kernel = o->kernel;
-  if (kernel > MG_KERNEL_POINT) return fail(MG_ERR_INVALID_ARG, "unknown kernel %u", kernel);
+  if (kernel > MG_KERNEL_SUFFIX) return fail(MG_ERR_INVALID_ARG, "unknown kernel %u", kernel);
   if (kernel == MG_KERNEL_AUTO) {
     if (d == 0) kernel = MG_KERNEL_POINT;
     else if (pl->chunks <= 8) kernel = MG_KERNEL_LANE_DFS;
@@ -209,11 +209,26 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     return fail(MG_ERR_INVALID_ARG, "lane-DFS kernel takes rows of at most 128 bytes");
   if (kernel == MG_KERNEL_STREAM && pl->chunks > 256)
     return fail(MG_ERR_INVALID_ARG, "stream kernel takes rows of at most 4096 bytes");
+  if (kernel == MG_KERNEL_SUFFIX && (pl->chunks > 8 || d == 0))
+    return fail(MG_ERR_INVALID_ARG, "suffix-walk kernel takes rows of at most 128 bytes, d >= 1");
   pl->kernel = kernel;
   pl->waves_per_block = o->waves_per_block ? o->waves_per_block : 4;
 
   if (kernel == MG_KERNEL_POINT) {
     pl->parts = (uint32_t)((pl->row_end - pl->row_begin + 63) / 64);
+    return MG_OK;
+  }
+  if (kernel == MG_KERNEL_SUFFIX) {
+    pl->lb = std::min<uint32_t>(11, d);  // suffix levels
+    const uint64_t tile_rows = 1ull << pl->lb;
+    if (pl->row_begin % tile_rows != 0)
+      return fail(MG_ERR_ROWS, "row_begin %llu is not a multiple of the %llu-row tile",
+                  (unsigned long long)pl->row_begin, (unsigned long long)tile_rows);
+    pl->tile_begin = pl->row_begin / tile_rows;
+    pl->tile_end = (pl->row_end + tile_rows - 1) / tile_rows;
+    const uint64_t n_tiles = pl->tile_end - pl->tile_begin;
+    if (n_tiles >= (1ull << 32)) return fail(MG_ERR_ROWS, "too many tiles");
+    pl->parts = (uint32_t)std::max<uint64_t>(n_tiles, 1);
     return MG_OK;
   }
   // Default lane depth (measured on MI355X): lane-DFS wants deep lane subtrees (kl up to 10)
@@ -315,6 +330,19 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       p.partial = b.partial;
       MG_HIP(launch_stream(stream_s(pl.chunks), (int)pl.keys_per_wave, p,
                            (int)pl.waves_per_block, s));
+    } else if (pl.kernel == MG_KERNEL_SUFFIX) {
+      SuffixParams p{};
+      p.rows = (const uint4*)pl.rows;
+      p.n_rows = pl.row_end;
+      p.keys = b.keys;
+      p.key_idx = b.key_idx;
+      p.n_keys = pl.n_keys;
+      p.d = pl.d;
+      p.s = pl.lb;
+      p.tile_begin = pl.tile_begin;
+      p.n_tiles = pl.tile_end - pl.tile_begin;
+      p.partial = b.partial;
+      MG_HIP(launch_suffix((int)pl.chunks, p, s));
     } else {
       PointParams p{};
       p.rows = (const uint4*)pl.rows;
@@ -459,7 +487,7 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
     t->kernel = pl.kernel;
     t->lane_depth = pl.kl;
     t->keys_per_wave = pl.kernel == MG_KERNEL_STREAM ? pl.keys_per_wave : 1;
-    t->waves = pl.kernel == MG_KERNEL_POINT
+    t->waves = (pl.kernel == MG_KERNEL_POINT || pl.kernel == MG_KERNEL_SUFFIX)
                    ? pl.parts * n_keys
                    : pl.parts * (pl.kernel == MG_KERNEL_STREAM
                                      ? (n_keys + pl.keys_per_wave - 1) / pl.keys_per_wave
@@ -640,6 +668,20 @@ int mg_matrix_info(const mg_matrix* m, uint64_t* n_rows, uint32_t* row_bytes) {
   if (!m) return fail(MG_ERR_INVALID_ARG, "matrix is null");
   if (n_rows) *n_rows = m->n_rows;
   if (row_bytes) *row_bytes = m->row_bytes;
+  return MG_OK;
+}
+
+int mg_matrix_copy(mg_matrix* dst, const mg_matrix* src) {
+  if (!dst || !src) return fail(MG_ERR_INVALID_ARG, "null argument");
+  if (dst->dev != src->dev) return fail(MG_ERR_INVALID_ARG, "matrices on different devices");
+  if (dst->n_rows != src->n_rows || dst->row_bytes != src->row_bytes)
+    return fail(MG_ERR_ROWS, "matrix shapes differ");
+  std::lock_guard<std::mutex> lk(dst->dev->mu);
+  int rc = mg::set_device(dst->dev);
+  if (rc) return rc;
+  const uint64_t n = dst->n_rows * dst->row_bytes / 16;
+  if (n && dst != src) MG_HIP(mg::launch_copy(dst->rows, src->rows, n, dst->dev->stream));
+  MG_HIP(hipStreamSynchronize(dst->dev->stream));
   return MG_OK;
 }
 

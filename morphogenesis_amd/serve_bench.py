@@ -1,0 +1,182 @@
+"""Serving benchmark: Poisson request arrivals against the request loop.
+
+Each request carries 3 cuckoo keys x k lookups, k uniform in [k_min, k_max];
+requests arrive as a Poisson process at --rate per second for --duration
+seconds. An epoch update of --delta-rows uniform rows (random new values)
+is applied every --epoch-seconds. Keys come in party-0/party-1 pairs, so every
+completed request is checked bit for bit: each pair's answers must XOR to
+the target row as of the request's admission epoch (seeded base row XOR the
+updates up to that epoch, recomputed on the host).
+
+Reported (one JSON line): requests and keys completed, queries/s (3 keys =
+1 query), p50/p99 latency (submit -> answers on the host) of completed
+requests, requests still in flight at the end, and the plain rate (one
+3-key query per full pass) measured on the same matrix.
+
+CLI: python -m morphogenesis_amd.serve_bench --rows-log2 27 --rate 5 --duration 20
+"""
+import argparse
+import json
+import sys
+import threading
+import time
+
+import numpy as np
+
+from . import GpuPageMatrix, GpuScanner, GpuServer, generate_keys
+from .data import seeded_rows
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pct(xs, p):
+    return float(np.percentile(np.asarray(xs), p)) if xs else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rows-log2", type=int, default=27)
+    ap.add_argument("--row-bytes", type=int, default=32)
+    ap.add_argument("--rate", type=float, default=5.0, help="requests per second")
+    ap.add_argument("--duration", type=float, default=20.0, help="seconds of arrivals")
+    ap.add_argument("--drain", type=float, default=30.0, help="seconds to wait for stragglers")
+    ap.add_argument("--k-min", type=int, default=1)
+    ap.add_argument("--k-max", type=int, default=32)
+    ap.add_argument("--epoch-seconds", type=float, default=12.0)
+    ap.add_argument("--delta-rows", type=int, default=1 << 16)
+    ap.add_argument("--segments", type=int, default=0)
+    # Admission is bounded by free HBM; this cap additionally bounds the keys
+    # sharing a sweep, trading throughput headroom for latency (FIFO beyond it).
+    ap.add_argument("--max-active-keys", type=int, default=96)
+    ap.add_argument("--double-buffer", action="store_true")
+    ap.add_argument("--seed", type=int, default=5)
+    args = ap.parse_args()
+
+    n = 1 << args.rows_log2
+    w = args.row_bytes
+    d = args.rows_log2
+    scanner = GpuScanner(0)
+    m = GpuPageMatrix(scanner, n, w)
+    m.fill_seeded(args.seed)
+    rng = np.random.default_rng(args.seed)
+
+    # Plain path: one query (3 keys) per full pass over the same matrix.
+    plain_keys = [generate_keys(d, int(t), seed=90 + i)[0]
+                  for i, t in enumerate(rng.integers(0, n, size=3))]
+    scanner.scan_keys(m, plain_keys)
+    t0 = time.perf_counter()
+    scanner.scan_keys(m, plain_keys)
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    log(f"plain pass (3 keys): {plain_ms:.1f} ms")
+
+    srv = GpuServer(scanner, m, segments=args.segments, max_active_keys=args.max_active_keys,
+                    max_delta_entries=max(1 << 20, 8 * args.delta_rows),
+                    double_buffer=args.double_buffer)
+    log(f"server: {srv.stats()}")
+
+    lock = threading.Lock()
+    updates = {}   # row -> [(epoch, diff)] in epoch order
+    records = []   # (ticket, targets, n_keys)
+    stop = threading.Event()
+
+    rng_s = np.random.default_rng(args.seed + 1)  # one generator per thread
+    rng_e = np.random.default_rng(args.seed + 2)
+
+    def submitter():
+        rng = rng_s
+        t_next = time.perf_counter()
+        i = 0
+        while not stop.is_set():
+            t_next += rng.exponential(1.0 / args.rate)
+            delay = t_next - time.perf_counter()
+            if delay > 0 and stop.wait(delay):
+                break
+            n_keys = 3 * int(rng.integers(args.k_min, args.k_max + 1))
+            targets = [int(x) for x in rng.integers(0, n, size=(n_keys + 1) // 2)]
+            keys = []
+            for j, t in enumerate(targets):
+                keys += list(generate_keys(d, t, seed=(i << 16) + j))
+            keys = keys[:n_keys]
+            with lock:
+                records.append((srv.submit(keys), targets, n_keys))
+            i += 1
+
+    def epocher():
+        rng = rng_e
+        e = 0
+        while not stop.wait(args.epoch_seconds):
+            e += 1
+            idx = rng.integers(0, n, size=args.delta_rows).astype(np.uint64)
+            diffs = rng.integers(0, 256, size=(args.delta_rows, w), dtype=np.uint8)
+            with lock:
+                srv.update(idx, diffs, e)
+                for r, dlt in zip(idx.tolist(), diffs):
+                    updates.setdefault(r, []).append((e, dlt.copy()))
+            log(f"epoch {e}: {args.delta_rows} rows updated")
+
+    th_s = threading.Thread(target=submitter)
+    th_e = threading.Thread(target=epocher)
+    t_start = time.perf_counter()
+    th_s.start()
+    th_e.start()
+    time.sleep(args.duration)
+    stop.set()
+    th_s.join()
+    th_e.join()
+    log(f"arrivals done: {len(records)} requests submitted")
+
+    lat, queue_ms = [], []
+    done = keys_done = unfinished = 0
+    exact = True
+    deadline = time.perf_counter() + args.drain
+    t_last = time.perf_counter()
+    for ticket, targets, n_keys in records:
+        remaining = int((deadline - time.perf_counter()) * 1e3)
+        if remaining <= 0:
+            unfinished += 1
+            continue
+        try:
+            ans, info = srv.wait(ticket, n_keys, timeout_ms=remaining)
+        except Exception:
+            unfinished += 1
+            continue
+        t_last = time.perf_counter()
+        done += 1
+        keys_done += n_keys
+        lat.append(info["latency_ms"])
+        queue_ms.append(info["queue_ms"])
+        for j, t in enumerate(targets):
+            if 2 * j + 1 >= n_keys:
+                break
+            want = seeded_rows(args.seed, t, 1, w)[0].copy()
+            for e, dlt in updates.get(t, []):
+                if e <= info["epoch"]:
+                    want ^= dlt
+            if not np.array_equal(ans[2 * j] ^ ans[2 * j + 1], want):
+                exact = False
+    wall = t_last - t_start
+    out = {
+        "workload": f"2^{args.rows_log2} rows x {w} B, Poisson {args.rate} req/s for "
+                    f"{args.duration} s, 3 x U{{{args.k_min}..{args.k_max}}} keys per request, "
+                    f"{args.delta_rows}-row epoch update every {args.epoch_seconds} s",
+        "submitted": len(records), "completed": done, "unfinished": unfinished,
+        "keys_completed": keys_done,
+        "queries_per_s": keys_done / 3.0 / wall if wall > 0 else None,
+        "keys_per_s": keys_done / wall if wall > 0 else None,
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+        "p50_queue_ms": pct(queue_ms, 50),
+        "offered_keys_per_s": args.rate * 3 * (args.k_min + args.k_max) / 2.0,
+        "plain_queries_per_s": 1e3 / plain_ms,
+        "bit_exact": exact,
+        "server": srv.stats(),
+    }
+    print(json.dumps(out))
+    srv.close()
+    m.close()
+    scanner.close()
+
+
+if __name__ == "__main__":
+    main()

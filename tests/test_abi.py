@@ -97,3 +97,9 @@ def test_keygen_rejects_bad_arguments():
         mg.generate_keys(33, 0, 0)
     with pytest.raises(mg.RowsError):
         mg.generate_keys(8, 256, 0)
+
+
+def test_host_seeded_rows_match_oracle():
+    from morphogenesis_amd.data import seeded_rows
+    assert np.array_equal(seeded_rows(77, 0, 100, 48), oracle.fill_rows(77, 0, 100, 48))
+    assert np.array_equal(seeded_rows(5, 123_456, 3, 32), oracle.fill_rows(5, 123_456, 3, 32))

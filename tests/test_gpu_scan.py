@@ -124,6 +124,34 @@ def test_point_kernel_and_zero_domain(mg, scanner):
     m1.close()
 
 
+@pytest.mark.parametrize("n,w,d", [(1, 32, 1), (100, 32, 7), (2048, 16, 11), (5000, 64, 13),
+                                   (70_000, 32, 20)])
+def test_suffix_walk_baseline_matches_oracle(mg, scanner, n, w, d):
+    rows = oracle.fill_rows(seed=n + 7, first_row=0, n_rows=n, row_bytes=w)
+    m = upload(mg, scanner, rows)
+    keys = [p[i % 2] for i, p in enumerate(pairs_for(d, sorted({0, n - 1, n // 2}), seed=3))]
+    got, t = scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_SUFFIX), with_timing=True)
+    assert t["kernel"] == "suffix"
+    assert np.array_equal(got, oracle.scan_tiled(keys, rows, THREADS))
+    m.close()
+
+
+def test_large_domain_small_matrix(mg, scanner):
+    # Keys over 31- and 32-bit domains on a matrix far smaller than 2^d.
+    for d in (31, 32):
+        n = 100_000
+        rows = oracle.fill_rows(seed=d, first_row=0, n_rows=n, row_bytes=32)
+        m = upload(mg, scanner, rows)
+        targets = [0, 99_999, 65_536]
+        pairs = pairs_for(d, targets, seed=d)
+        g0 = scanner.scan_keys(m, [p[0] for p in pairs])
+        g1 = scanner.scan_keys(m, [p[1] for p in pairs])
+        assert np.array_equal(g0, oracle.scan_tiled([p[0] for p in pairs], rows, THREADS))
+        for q, t in enumerate(targets):
+            assert np.array_equal(g0[q] ^ g1[q], rows[t])
+        m.close()
+
+
 def test_fuzzed_control_bytes(mg, scanner):
     rng = np.random.default_rng(11)
     n, w, d = 3000, 32, 12
@@ -139,7 +167,7 @@ def test_fuzzed_control_bytes(mg, scanner):
             b[off + 17] = int(rng.choice([0, 1, 2, 0xFE]))
         keys.append(bytes(b))
     ref = oracle.scan_tiled(keys, rows, THREADS)
-    for kern in (mg.KERNEL_LANE_DFS, mg.KERNEL_STREAM, mg.KERNEL_POINT):
+    for kern in (mg.KERNEL_LANE_DFS, mg.KERNEL_STREAM, mg.KERNEL_POINT, mg.KERNEL_SUFFIX):
         assert np.array_equal(scanner.scan_keys(m, keys, mg.make_opts(kernel=kern)), ref), kern
     m.close()
 
