@@ -72,6 +72,16 @@ __device__ __forceinline__ void prg_expand(uint32_t s0, uint32_t s1,
   rt = (y1 + kC1) & 1u;
 }
 
+// All-ones if t == 1, else zero, kept opaque to the optimiser so that
+// `x ^ (uniform & m)` stays one v_bitop3_b32 with the uniform operand in an
+// SGPR, instead of being rewritten as a select of the uniform value (which
+// costs a v_mov plus a v_cndmask per word).
+__device__ __forceinline__ uint32_t t_mask(uint32_t t) {
+  uint32_t m = (t == 1u) ? 0xFFFFFFFFu : 0u;
+  asm("" : "+v"(m));
+  return m;
+}
+
 // Both children of `n` at tree level `level`, correction word applied when
 // n.t == 1 (dpf.rs:221-239). `k` is wave-uniform, so the correction word is
 // read with scalar loads.
@@ -80,7 +90,7 @@ __device__ __forceinline__ void expand_node(const DevKey* __restrict__ k,
                                             Node& left, Node& right) {
   uint32_t l[4], r[4], lt, rt;
   prg_expand(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
-  const uint32_t m = (n.t == 1u) ? 0xFFFFFFFFu : 0u;
+  const uint32_t m = t_mask(n.t);
   const uint32_t c0 = k->cw_seed[level][0], c1 = k->cw_seed[level][1];
   const uint32_t c2 = k->cw_seed[level][2], c3 = k->cw_seed[level][3];
   const uint32_t ct = k->cw_t[level];
@@ -104,7 +114,7 @@ __device__ __forceinline__ Node ggm_step(const DevKey* __restrict__ k,
 // Leaf output (dpf.rs:136-141): seed ^ final_cw iff t == 1.
 __device__ __forceinline__ uint4 leaf_mask(const DevKey* __restrict__ k,
                                            const Node& n) {
-  const uint32_t m = (n.t == 1u) ? 0xFFFFFFFFu : 0u;
+  const uint32_t m = t_mask(n.t);
   return make_uint4(n.s0 ^ (k->final_cw[0] & m), n.s1 ^ (k->final_cw[1] & m),
                     n.s2 ^ (k->final_cw[2] & m), n.s3 ^ (k->final_cw[3] & m));
 }

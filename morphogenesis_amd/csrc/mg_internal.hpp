@@ -94,6 +94,7 @@ struct mg_device {
   hipEvent_t ev[5] = {};
   std::mutex mu;  // serialises every use of `stream` and the workspace
   mg::DevBuf keys, partial, partial_delta, out;
+  mg::DevBuf zero_rows;  // 4 KiB of zeros, read in place of rows past the end
   mg::HostBuf h_keys, h_out;
   // HIP graph of the kernel sequence for the last plan.
   bool graph_valid = false;

@@ -22,6 +22,13 @@ namespace mg {
 
 // ------------------------------------------------------------- helpers ----
 
+// Materialises a leaf mask once, so each row chunk folds in with a single
+// v_bitop3_b32 instead of re-deriving the mask per chunk.
+__device__ __forceinline__ uint4 opaque4(uint4 v) {
+  asm("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+  return v;
+}
+
 __device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v ^= __shfl_xor(v, off, 64);
@@ -134,42 +141,43 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
     Node node = tile_lane_root(k, path, lane, tile, tile - 1, tile == t_begin, tb, lb);
     const uint64_t base = (tile << (kl + lb)) | ((uint64_t)lane << kl);
 
-    uint4 r0[C], r1[C];
-    auto load_pair = [&](uint32_t pair) {
-      const uint64_t x = base + 2 * (uint64_t)pair;
-      const bool v0 = lane_active && x < p.n_rows;
-      const bool v1 = lane_active && x + 1 < p.n_rows;
-      const uint4* src = p.rows + x * C;
-#pragma unroll
-      for (int c = 0; c < C; c++) {
-        r0[c] = v0 ? src[c] : make_uint4(0, 0, 0, 0);
-        r1[c] = v1 ? src[C + c] : make_uint4(0, 0, 0, 0);
-      }
-    };
-    load_pair(0);
+    // One iteration per leaf pair: pop the pending right child (if any), issue
+    // the pair's row loads, walk down to the bottom node pushing right
+    // children, expand it into the two leaves and fold the rows. The loads
+    // are in flight for at least the bottom expansion, and their registers
+    // live within one iteration. Loads are unconditional: a row past n_rows
+    // (or of an idle lane) is read from a zero buffer instead.
     int lvl = 0;
-    uint32_t pair = 0;
-    for (;;) {
-      Node L, R;
-      expand_node(k, leaf_level + lvl, node, L, R);
-      if (lvl == kl - 1) {
-        const uint4 m0 = leaf_mask(k, L), m1 = leaf_mask(k, R);
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-          xor_masked(acc[c], r0[c], m0);
-          xor_masked(acc[c], r1[c], m1);
-        }
-        if (++pair == n_pairs) break;
+    for (uint32_t pair = 0; pair < n_pairs; ++pair) {
+      if (pair) {
         lvl = kl - 1 - __builtin_ctz(pair);
         const uint4 sv = st_seed[lvl * 64 + lane];
         node.s0 = sv.x; node.s1 = sv.y; node.s2 = sv.z; node.s3 = sv.w;
         node.t = st_t[lvl * 64 + lane];
-        load_pair(pair);
-      } else {
-        ++lvl;
-        st_seed[lvl * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);
-        st_t[lvl * 64 + lane] = R.t;
+      }
+      const uint64_t x = base + 2 * (uint64_t)pair;
+      const uint4* src0 = (lane_active && x < p.n_rows) ? p.rows + x * C : p.zero_rows;
+      const uint4* src1 = (lane_active && x + 1 < p.n_rows) ? p.rows + (x + 1) * C : p.zero_rows;
+      uint4 r0[C], r1[C];
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        r0[c] = src0[c];
+        r1[c] = src1[c];
+      }
+      for (; lvl < kl - 1; ++lvl) {
+        Node L, R;
+        expand_node(k, leaf_level + lvl, node, L, R);
+        st_seed[(lvl + 1) * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);
+        st_t[(lvl + 1) * 64 + lane] = R.t;
         node = L;
+      }
+      Node L, R;
+      expand_node(k, leaf_level + kl - 1, node, L, R);
+      const uint4 m0 = opaque4(leaf_mask(k, L)), m1 = opaque4(leaf_mask(k, R));
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        xor_masked(acc[c], r0[c], m0);
+        xor_masked(acc[c], r1[c], m1);
       }
     }
   }

@@ -18,6 +18,7 @@ struct DevKey;
 struct LaneDfsParams {
   const uint4* rows;
   uint64_t n_rows;
+  const uint4* zero_rows;   // >= 16 zero chunks: read instead of rows past n_rows
   const DevKey* keys;
   const uint32_t* key_idx;  // key i of the launch is keys[key_idx[i]] (identity if null)
   uint32_t n_keys;

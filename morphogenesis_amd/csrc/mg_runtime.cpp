@@ -304,6 +304,7 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       LaneDfsParams p{};
       p.rows = (const uint4*)pl.rows;
       p.n_rows = pl.row_end;
+      p.zero_rows = (const uint4*)dev->zero_rows.p;
       p.keys = b.keys;
       p.key_idx = b.key_idx;
       p.n_keys = pl.n_keys;
@@ -543,6 +544,14 @@ int mg_device_open(int ordinal, mg_device** out) {
       return mg::hip_fail(ee, "hipEventCreate");
     }
   }
+  int rc = dev->zero_rows.reserve(4096);
+  if (!rc && (mg::launch_zero((uint4*)dev->zero_rows.p, 4096 / 16, dev->stream) != hipSuccess ||
+              hipStreamSynchronize(dev->stream) != hipSuccess))
+    rc = fail(MG_ERR_DEVICE, "zeroing the zero-row buffer failed");
+  if (rc) {
+    mg_device_close(dev);
+    return rc;
+  }
   *out = dev;
   return MG_OK;
 }
@@ -556,6 +565,7 @@ int mg_device_close(mg_device* dev) {
   dev->partial.release();
   dev->partial_delta.release();
   dev->out.release();
+  dev->zero_rows.release();
   dev->h_keys.release();
   dev->h_out.release();
   for (auto& ev : dev->ev)
