@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--cpu-sample-log2", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=31)
+    ap.add_argument("--q-sweep", default="",
+                    help="comma-separated key counts: one extra timed pass each, reported in by_keys")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a 4 GiB matrix once (known bytes for counter scaling)")
     args = ap.parse_args()
@@ -196,6 +198,25 @@ def main():
                 raise SystemExit("bit-exactness failure in the suffix-walk baseline")
             log(f"reference-shaped plain pass {i}: {ref_ms[-1]:.1f} ms per 3 keys")
 
+    # Optional single passes at other batch sizes (2^31 x 32 B: Q in {3, 24, 96, 384}).
+    by_keys = {}
+    for qs in [int(x) for x in args.q_sweep.split(",") if x.strip()]:
+        qs = max(2, qs - qs % 2)
+        sw_targets = [int(x) for x in rng.integers(0, n_rows, size=qs // 2)]
+        sw_keys = [k for i, t in enumerate(sw_targets)
+                   for k in mg.generate_keys(d, t, seed=50_000 + i)]
+        s0 = time.perf_counter()
+        ans, t = scanner.scan_keys(m, sw_keys, opts, with_timing=True)
+        wall_q = time.perf_counter() - s0
+        for i, tt in enumerate(sw_targets[:4]):
+            if not np.array_equal(ans[2 * i] ^ ans[2 * i + 1], m.download(tt, 1)[0]):
+                raise SystemExit(f"bit-exactness failure in the {qs}-key pass")
+        by_keys[qs] = {"pass_ms": wall_q * 1e3, "kernel_ms": t["scan_kernel_ms"],
+                       "queries_per_s": qs / KEYS_PER_QUERY / wall_q,
+                       "valu_frac": qs * n_rows * (OPS_PER_NODE + w // 4)
+                       / (t["scan_kernel_ms"] * 1e-3) / 1e12 / VALU_PEAK_TOPS}
+        log(f"{qs}-key pass: {wall_q * 1e3:.1f} ms ({by_keys[qs]['queries_per_s']:.2f} queries/s)")
+
     queries_per_step = q / KEYS_PER_QUERY
     qps = queries_per_step * args.steps / wall
     plain_qps = 1e3 / statistics.mean(plain_ms) if plain_ms else None
@@ -249,6 +270,7 @@ def main():
         "plain_reference_shaped_queries_per_s": (1e3 / statistics.mean(ref_ms)) if ref_ms else None,
         "batched_vs_reference_shaped_plain": (qps * statistics.mean(ref_ms) / 1e3) if ref_ms else None,
         "scan_gbs": scan_gbs,
+        "by_keys": by_keys or None,
         "bit_exact": True,
         "roofline": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS,
                      "unit": "TOP/s", "frac": valu_tops / VALU_PEAK_TOPS, "traffic": traffic,

@@ -189,9 +189,11 @@ int mg_matrix_apply_delta(mg_matrix *m, const mg_delta *dl);
  * updates never tear an in-flight answer.
  * ------------------------------------------------------------------------- */
 typedef struct {
-  uint32_t segments;          /* sweep steps per full pass; 0 = 64          */
-  uint32_t lane_depth;        /* kl of the fused kernel; 0 = 10             */
-  uint32_t max_active_keys;   /* 0 = bounded by free HBM (at most 65536)    */
+  uint32_t segments;          /* sweep steps per full pass; 0 = one per 2^25
+                                 rows, between 1 and 64                     */
+  uint32_t lane_depth;        /* kl of the fused kernel; 0 = 10 for rows of
+                                 at most 128 B, else 6                      */
+  uint32_t max_active_keys;   /* 0 = bounded by free HBM (at most 32768)    */
   uint32_t domain_bits;       /* key domain served; 0 = ceil(log2 n_rows)   */
   uint64_t hbm_reserve_bytes; /* HBM left free for other allocations       */
   uint64_t max_delta_entries; /* update rows kept before a rebase; 0 = 2^20 */

@@ -233,14 +233,17 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   }
   // Default lane depth (measured on MI355X): lane-DFS wants deep lane subtrees (kl up to 10)
   // as long as the keys still have at least ~4096 tiles of 2^(kl+6) rows between them to spread
-  // over the resident waves; the stream kernel's LDS stacks favour kl 5-7.
+  // over the resident waves. The stream kernel at 78,643,200 x 1 KiB: one key per wave runs best
+  // at kl = 4 (12.26 ms, 3072 waves), three keys per wave at kl = 7 (16.94 ms against 17.45 ms
+  // at kl = 5).
   uint32_t kl_default;
   if (kernel == MG_KERNEL_LANE_DFS) {
     const uint64_t work = (pl->row_end - pl->row_begin) * (uint64_t)n_keys;
     kl_default = 6;
     while (kl_default < 10 && (work >> (kl_default + 7)) >= 4096) kl_default++;
   } else {
-    kl_default = n_keys >= 3 ? 5 : 7;
+    const uint32_t g = o->keys_per_wave ? o->keys_per_wave : std::min<uint32_t>(3, n_keys);
+    kl_default = g <= 1 ? 4 : 7;
   }
   split_levels(d, o->lane_depth ? o->lane_depth : kl_default, pl);
   const uint64_t tile_rows = 1ull << (pl->kl + pl->lb);

@@ -439,7 +439,13 @@ int mg_server_create(mg_device* dev, mg_matrix* m, const mg_server_opts* opts, m
   s->kl = std::min<uint32_t>(o->lane_depth ? o->lane_depth : (s->chunks > 8 ? 6 : 10), d);
   const uint32_t lb = std::min<uint32_t>(6, d - s->kl);
   const uint64_t tile = 1ull << (s->kl + lb);
-  const uint32_t want = o->segments ? o->segments : 64;
+  // Default: one step per 2^25 rows (at most 64 steps). Measured at 2^27 x 32 B
+  // under Poisson load, 4 steps per pass gave p50/p99 of 416/1340 ms against
+  // 504/2003 ms at 32 steps: larger steps spend less on launches and host
+  // round trips, and admission waits at most one step either way.
+  const uint32_t want =
+      o->segments ? o->segments
+                  : (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, s->n_rows >> 25));
   uint64_t seg = (s->n_rows + want - 1) / want;
   seg = (seg + tile - 1) / tile * tile;
   s->seg_rows = seg;

@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--seed", type=int, default=5)
     args = ap.parse_args()
 
+    if args.max_active_keys < 3 * args.k_max:
+        raise SystemExit(f"--max-active-keys {args.max_active_keys} is below the largest request "
+                         f"({3 * args.k_max} keys); such requests would be rejected")
     n = 1 << args.rows_log2
     w = args.row_bytes
     d = args.rows_log2
