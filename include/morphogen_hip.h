@@ -234,6 +234,61 @@ int mg_server_update(mg_server *s, const uint64_t *row_idx,
 int mg_server_stats_get(mg_server *s, mg_server_stats *out);
 int mg_server_destroy(mg_server *s);
 
+/* ---------------------------------------------------------------------------
+ * Wire API of the GPU page-query endpoints, transport-free: the caller's
+ * HTTP layer passes method, path, Content-Type and body and sends back the
+ * status and JSON body. Replaces page_query_gpu_handler and
+ * page_query_gpu_batch_handler (crates/morphogen-server/src/network/api/
+ * mod.rs:544-842), their routes and body limit (mod.rs:849, 886-896), the
+ * DTOs (dto.rs:83, 126-156) and the hex codec (serde_hex.rs:58-85):
+ *   POST /query/page/gpu        {"keys":["0x..","0x..","0x.."]}
+ *        -> {"epoch_id":E,"pages":["0x..","0x..","0x.."]}
+ *   POST /query/page/gpu/batch  {"queries":[{"keys":[...]},...]}   (1..32)
+ *        -> {"epoch_id":E,"results":[{"pages":[...]},...]}
+ * Pages are row_bytes long (the reference's 4096-byte pages are rows of a
+ * 4096-byte matrix). Status: 404/405 routing, 415 not JSON, 413 body over the
+ * limit, 400 malformed JSON, 422 wrong shape or bad hex, 400 batch size, key
+ * count or a key that does not parse (keys up to d = 32; the reference's
+ * kernel stops at 25, kernel.rs:45), 503 request larger than the loop's key
+ * slots or wait timed out, 500 device error.
+ * ------------------------------------------------------------------------- */
+#define MG_WIRE_KEYS_PER_QUERY 3   /* kernel.rs KEYS_PER_QUERY              */
+#define MG_WIRE_MAX_BATCH 32       /* dto.rs:83 MAX_BATCH_SIZE              */
+#define MG_WIRE_MAX_BODY 65536     /* api/mod.rs:849 MAX_REQUEST_BODY_SIZE  */
+
+typedef struct {
+  uint64_t max_body_bytes;  /* 0 = MG_WIRE_MAX_BODY                         */
+  uint32_t max_batch;       /* 0 = MG_WIRE_MAX_BATCH                        */
+  uint32_t timeout_ms;      /* wait for the loop; 0 = forever. A request that
+                               times out (503) still runs to completion and
+                               its answers are discarded with the loop.     */
+} mg_wire_opts;
+
+/* Output bytes a response of n_queries queries of row_bytes pages needs. */
+size_t mg_wire_response_capacity(uint32_t n_queries, uint32_t row_bytes);
+/* Routing, extractor and handler checks only (no device): on MG_OK the
+ * request is valid (*http_status = 200) and keys holds its 3 x n_queries
+ * wire keys back to back (key_lens[i] each). Otherwise an MG_ERR_* code with
+ * *http_status set to the status the endpoint answers. */
+int mg_wire_parse(const mg_wire_opts *opts, const char *method, const char *path,
+                  const char *content_type, const char *body, size_t body_len,
+                  uint8_t *keys, size_t keys_cap, size_t *key_lens,
+                  uint32_t key_lens_cap, uint32_t *n_queries, int *http_status);
+/* JSON response body for answers pages[n_queries][3][row_bytes]; batch = 0
+ * for the single-query shape. NUL-terminated; *out_len excludes the NUL. */
+int mg_wire_format(uint64_t epoch, const uint8_t *pages, uint32_t n_queries,
+                   uint32_t row_bytes, int batch, char *out, size_t out_cap,
+                   size_t *out_len);
+/* One request end to end through the request loop `s` (all keys of a batch
+ * as one loop request, so they share one admission epoch). Returns MG_OK
+ * whenever a response was produced, whatever its status; the body is empty
+ * for errors (mg_last_error() says why). s = NULL answers valid requests
+ * with 503. Safe to call from many threads on one loop. */
+int mg_wire_handle(mg_server *s, const mg_wire_opts *opts, const char *method,
+                   const char *path, const char *content_type, const char *body,
+                   size_t body_len, char *out, size_t out_cap, size_t *out_len,
+                   int *http_status);
+
 #ifdef __cplusplus
 }
 #endif

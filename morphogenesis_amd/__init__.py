@@ -91,6 +91,11 @@ class ScanOpts(ctypes.Structure):
                 ("delta", ctypes.c_void_p)]
 
 
+class WireOpts(ctypes.Structure):
+    _fields_ = [("max_body_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_uint32),
+                ("timeout_ms", ctypes.c_uint32)]
+
+
 class ServerOpts(ctypes.Structure):
     _fields_ = [("segments", ctypes.c_uint32), ("lane_depth", ctypes.c_uint32),
                 ("max_active_keys", ctypes.c_uint32), ("domain_bits", ctypes.c_uint32),
@@ -166,6 +171,18 @@ def _load():
         "mg_server_update": (ctypes.c_int, [vp, u64p, u8p, ctypes.c_uint64, ctypes.c_uint64]),
         "mg_server_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
         "mg_server_destroy": (ctypes.c_int, [vp]),
+        "mg_wire_response_capacity": (ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32]),
+        "mg_wire_parse": (ctypes.c_int, [ctypes.POINTER(WireOpts), ctypes.c_char_p,
+                                         ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                         ctypes.c_size_t, u8p, ctypes.c_size_t, szp,
+                                         ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                                         ctypes.POINTER(ctypes.c_int)]),
+        "mg_wire_format": (ctypes.c_int, [ctypes.c_uint64, u8p, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, szp]),
+        "mg_wire_handle": (ctypes.c_int, [vp, ctypes.POINTER(WireOpts), ctypes.c_char_p,
+                                          ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                          ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t, szp,
+                                          ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -503,3 +520,82 @@ class GpuServer:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------- wire API ----
+WIRE_MAX_BATCH = 32        # dto.rs:83 MAX_BATCH_SIZE
+WIRE_MAX_BODY = 64 * 1024  # api/mod.rs:849 MAX_REQUEST_BODY_SIZE
+WIRE_PATH = "/query/page/gpu"
+WIRE_BATCH_PATH = "/query/page/gpu/batch"
+
+
+def _bytes_arg(x):
+    if x is None:
+        return None
+    return x.encode() if isinstance(x, str) else bytes(x)
+
+
+def wire_parse(body, path=WIRE_PATH, method="POST", content_type="application/json",
+               max_body_bytes=0, max_batch=0):
+    """Routing, extractor and handler validation of one request
+    (page_query_gpu(_batch)_handler, api/mod.rs:544-842) without a device.
+    Returns (http_status, keys, n_queries): keys is the flat list of
+    3 x n_queries wire keys when the status is 200, else empty."""
+    body = _bytes_arg(body) or b""
+    o = WireOpts(max_body_bytes, max_batch, 0)
+    cap_keys = 3 * max(max_batch or WIRE_MAX_BATCH, 1)
+    keys_cap = max(len(body) // 2, 1)
+    buf = np.zeros(keys_cap, dtype=np.uint8)
+    lens = (ctypes.c_size_t * cap_keys)()
+    nq = ctypes.c_uint32()
+    st = ctypes.c_int()
+    rc = lib.mg_wire_parse(ctypes.byref(o), _bytes_arg(method), _bytes_arg(path),
+                           _bytes_arg(content_type), body, len(body), _u8(buf), keys_cap, lens,
+                           cap_keys, ctypes.byref(nq), ctypes.byref(st))
+    if rc != MG_OK:
+        return st.value, [], 0
+    keys, off = [], 0
+    for i in range(3 * nq.value):
+        keys.append(buf[off:off + lens[i]].tobytes())
+        off += lens[i]
+    return st.value, keys, nq.value
+
+
+def wire_format(epoch: int, pages: np.ndarray, batch: bool) -> bytes:
+    """JSON body of a response: pages is (n_queries, 3, W) or (3, W)."""
+    p = np.ascontiguousarray(pages, dtype=np.uint8)
+    if p.ndim == 2:
+        p = p[None]
+    nq, _, w = p.shape
+    cap = lib.mg_wire_response_capacity(nq, w)
+    out = ctypes.create_string_buffer(cap)
+    n = ctypes.c_size_t()
+    _check(lib.mg_wire_format(epoch, _u8(p), nq, w, 1 if batch else 0, out, cap, ctypes.byref(n)))
+    return out.raw[:n.value]
+
+
+class WireApi:
+    """The GPU page-query endpoints (POST /query/page/gpu and
+    /query/page/gpu/batch) over a GpuServer request loop; `handle` is the
+    transport-free handler an HTTP layer calls (morphogenesis_amd.http_api)."""
+
+    def __init__(self, server: Optional[GpuServer], max_body_bytes=0, max_batch=0,
+                 timeout_ms=0):
+        self.server = server
+        self.opts = WireOpts(max_body_bytes, max_batch, timeout_ms)
+        w = server.row_bytes if server is not None else 16
+        self.cap = lib.mg_wire_response_capacity(max_batch or WIRE_MAX_BATCH, w)
+
+    def handle(self, method: str, path: str, content_type: Optional[str], body: bytes):
+        """(http_status, response body bytes, error message)."""
+        body = bytes(body or b"")
+        out = ctypes.create_string_buffer(self.cap)
+        n = ctypes.c_size_t()
+        st = ctypes.c_int()
+        h = self.server._h if self.server is not None else None
+        _check(lib.mg_wire_handle(h, ctypes.byref(self.opts), _bytes_arg(method),
+                                  _bytes_arg(path), _bytes_arg(content_type), body, len(body),
+                                  out, self.cap, ctypes.byref(n), ctypes.byref(st)))
+        err = "" if st.value == 200 else (lib.mg_last_error() or b"").decode(errors="replace")
+        return st.value, out.raw[:n.value], err
+

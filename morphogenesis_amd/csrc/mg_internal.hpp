@@ -146,4 +146,7 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
 void dedupe_delta(const uint64_t* idx, const uint8_t* diffs, uint64_t n, uint32_t row_bytes,
                   std::vector<uint64_t>* uidx, std::vector<uint8_t>* udiffs);
 
+// Row width of the matrix a request loop serves (its answers' length).
+uint32_t server_row_bytes(const mg_server* s);
+
 }  // namespace mg

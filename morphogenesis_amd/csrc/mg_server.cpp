@@ -410,6 +410,10 @@ void server_loop(mg_server* s) {
 
 }  // namespace
 
+namespace mg {
+uint32_t server_row_bytes(const mg_server* s) { return s->row_bytes; }
+}  // namespace mg
+
 extern "C" {
 
 int mg_server_create(mg_device* dev, mg_matrix* m, const mg_server_opts* opts, mg_server** out) {
