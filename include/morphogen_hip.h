@@ -142,6 +142,41 @@ int mg_key_parse(const uint8_t *bytes, size_t len, mg_key_info *out);
 int mg_keygen(uint32_t domain_bits, uint64_t target, uint64_t seed,
               uint8_t *key0, uint8_t *key1);
 
+/* CuckooAddresser::with_seeds(num_rows, seeds).hash_indices(key)
+ * (crates/morphogen-core/src/cuckoo.rs:25-40): out[i] = SipHash-1-3 with the
+ * all-zero key (Rust's DefaultHasher) over LE64(seeds[i]) || key, modulo
+ * num_rows. CuckooAddresser::new's seeds are 0x12345678, 0x9ABCDEF0,
+ * 0xFEDCBA98. num_rows = 0: MG_ERR_ROWS. */
+int mg_cuckoo_hash_indices(const uint8_t *key, size_t key_len, uint64_t num_rows,
+                           const uint64_t seeds[3], uint64_t out[3]);
+
+#define MG_ROWS_PER_PAGE 16    /* crates/morphogen-dpf/src/page.rs:23 */
+#define MG_PAGE_ROW_BYTES 256  /* page.rs:24 */
+#define MG_PAGE_BYTES 4096     /* page.rs:25 */
+
+/* generate_gpu_page_query (crates/morphogen-client/src/gpu.rs:17-49): the
+ * account's 3 cuckoo rows among num_pages x 16, their page addresses
+ * (page = row / 16, offset = row % 16) and a key pair per page; keys_a and
+ * keys_b receive 3 wire keys each, 19 + 18d + 16 bytes apart. Root seeds
+ * derive from rng_seed (independent per pair) instead of the OS RNG.
+ * MG_ERR_DOMAIN for d outside 1..32, MG_ERR_ROWS when a page index does not
+ * fit the domain (PageIndexTooLarge). */
+int mg_page_query(const uint8_t *account_key, size_t key_len, uint64_t num_pages,
+                  const uint64_t seeds[3], uint32_t domain_bits, uint64_t rng_seed,
+                  uint8_t *keys_a, uint8_t *keys_b, uint64_t page_index[3],
+                  uint32_t row_offset[3]);
+/* aggregate_page_responses (crates/morphogen-client/src/page.rs:67-103):
+ * out = pages_a XOR pages_b (n_pages pages of page_bytes); MG_ERR_STATE when
+ * the two servers answered for different epochs. */
+int mg_page_aggregate(uint64_t epoch_a, const uint8_t *pages_a, uint64_t epoch_b,
+                      const uint8_t *pages_b, size_t page_bytes, uint32_t n_pages,
+                      uint8_t *out);
+/* extract_row_from_page (crates/morphogen-dpf/src/page.rs:698-708): the
+ * 256-byte row at row_offset; MG_ERR_ROWS if row_offset >= 16 or the row
+ * runs past the page. */
+int mg_page_extract_row(const uint8_t *page, size_t page_bytes, uint32_t row_offset,
+                        uint8_t *out);
+
 /* GpuScanner::scan / scan_batch / scan_batch_optimized(_tiled)(_with_graph)
  * (kernel.rs:685-1162), flattened: n_keys keys of the same domain in one
  * pass over the matrix. keys = concatenated wire keys, key_lens[i] their

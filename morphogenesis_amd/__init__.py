@@ -171,6 +171,14 @@ def _load():
         "mg_server_update": (ctypes.c_int, [vp, u64p, u8p, ctypes.c_uint64, ctypes.c_uint64]),
         "mg_server_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
         "mg_server_destroy": (ctypes.c_int, [vp]),
+        "mg_cuckoo_hash_indices": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint64, u64p,
+                                                  u64p]),
+        "mg_page_query": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint64, u64p,
+                                         ctypes.c_uint32, ctypes.c_uint64, u8p, u8p, u64p,
+                                         ctypes.POINTER(ctypes.c_uint32)]),
+        "mg_page_aggregate": (ctypes.c_int, [ctypes.c_uint64, u8p, ctypes.c_uint64, u8p,
+                                             ctypes.c_size_t, ctypes.c_uint32, u8p]),
+        "mg_page_extract_row": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint32, u8p]),
         "mg_wire_response_capacity": (ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32]),
         "mg_wire_parse": (ctypes.c_int, [ctypes.POINTER(WireOpts), ctypes.c_char_p,
                                          ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
@@ -598,4 +606,131 @@ class WireApi:
                                   out, self.cap, ctypes.byref(n), ctypes.byref(st)))
         err = "" if st.value == 200 else (lib.mg_last_error() or b"").decode(errors="replace")
         return st.value, out.raw[:n.value], err
+
+
+# ----------------------------------------------------------- client side ---
+CUCKOO_SEEDS = (0x12345678, 0x9ABCDEF0, 0xFEDCBA98)  # CuckooAddresser::new (cuckoo.rs:23-25)
+ROWS_PER_PAGE = 16      # crates/morphogen-dpf/src/page.rs:23
+ROW_SIZE_BYTES = 256    # page.rs:24
+PAGE_SIZE_BYTES = 4096  # page.rs:25
+
+
+def _u64_arr(vals):
+    a = np.ascontiguousarray(np.array(vals, dtype=np.uint64))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def _blob(data: bytes):
+    return np.frombuffer(bytes(data) or b"\0", dtype=np.uint8).copy()
+
+
+def cuckoo_hash_indices(key: bytes, num_rows: int, seeds=CUCKOO_SEEDS) -> List[int]:
+    """CuckooAddresser::with_seeds(num_rows, seeds).hash_indices(key) (cuckoo.rs:25-40)."""
+    buf = _blob(key)
+    sd, sdp = _u64_arr(seeds)
+    out, outp = _u64_arr([0, 0, 0])
+    _check(lib.mg_cuckoo_hash_indices(_u8(buf), len(key), num_rows, sdp, outp))
+    return [int(x) for x in out]
+
+
+@dataclass(frozen=True)
+class PageAddress:
+    """PageAddress (crates/morphogen-dpf/src/page.rs:39-50)."""
+    page_index: int
+    row_offset: int
+
+    @staticmethod
+    def from_row_index(row_index: int) -> "PageAddress":
+        return PageAddress(row_index // ROWS_PER_PAGE, row_index % ROWS_PER_PAGE)
+
+
+@dataclass
+class GpuPageQueryKeys:
+    """GpuPageQueryKeys (crates/morphogen-client/src/gpu.rs:11-15): wire keys."""
+    page_addresses: List[PageAddress]
+    keys_a: List[bytes]
+    keys_b: List[bytes]
+
+
+def generate_gpu_page_query(account_key: bytes, num_pages: int, seeds=CUCKOO_SEEDS,
+                            domain_bits: int = 0, rng_seed: Optional[int] = None
+                            ) -> GpuPageQueryKeys:
+    """generate_gpu_page_query (gpu.rs:17-49). domain_bits 0 = ceil(log2 num_pages);
+    rng_seed None draws the root seeds' seed from the OS RNG."""
+    if not domain_bits:
+        domain_bits = max(1, (max(num_pages, 1) - 1).bit_length())
+    if rng_seed is None:
+        rng_seed = int.from_bytes(os.urandom(8), "little")
+    buf = _blob(account_key)
+    sd, sdp = _u64_arr(seeds)
+    klen = 19 + 18 * domain_bits + 16 if 0 < domain_bits <= 32 else 1
+    ka = np.zeros(3 * klen, dtype=np.uint8)
+    kb = np.zeros(3 * klen, dtype=np.uint8)
+    pidx, pidxp = _u64_arr([0, 0, 0])
+    offs = (ctypes.c_uint32 * 3)()
+    rc = lib.mg_page_query(_u8(buf), len(account_key), num_pages, sdp, domain_bits,
+                           rng_seed & (2**64 - 1), _u8(ka), _u8(kb), pidxp, offs)
+    if rc != MG_OK:
+        raise _ERRORS.get(rc, MgError)(rc, f"page query over {num_pages} pages, d={domain_bits}")
+    return GpuPageQueryKeys(
+        [PageAddress(int(pidx[i]), int(offs[i])) for i in range(3)],
+        [ka[i * klen:(i + 1) * klen].tobytes() for i in range(3)],
+        [kb[i * klen:(i + 1) * klen].tobytes() for i in range(3)])
+
+
+class PageAggregationError(MgError):
+    """EpochMismatch / PageLengthMismatch (crates/morphogen-client/src/page.rs:54-65)."""
+
+
+class ExtractError(MgError):
+    """InvalidRowOffset (page.rs:163-166)."""
+
+
+def aggregate_gpu_page_responses(epoch_a: int, pages_a: Sequence[bytes], epoch_b: int,
+                                 pages_b: Sequence[bytes]):
+    """aggregate_page_responses (page.rs:67-103): (epoch, [a_i XOR b_i])."""
+    if len(pages_a) != len(pages_b):
+        raise PageAggregationError(MG_ERR_INVALID_ARG, "page count mismatch")
+    out = []
+    for i, (a, b) in enumerate(zip(pages_a, pages_b)):
+        if len(a) != len(b):
+            raise PageAggregationError(MG_ERR_INVALID_ARG,
+                                       f"page {i}: lengths {len(a)} and {len(b)}")
+        ba, bb = _blob(a), _blob(b)
+        o = np.zeros(max(len(a), 1), dtype=np.uint8)
+        rc = lib.mg_page_aggregate(epoch_a, _u8(ba), epoch_b, _u8(bb), len(a), 1, _u8(o))
+        if rc == MG_ERR_STATE:
+            raise PageAggregationError(rc, f"epoch mismatch: {epoch_a} vs {epoch_b}")
+        _check(rc)
+        out.append(o[:len(a)].tobytes())
+    return epoch_a, out
+
+
+def extract_row_from_page(page: bytes, row_offset: int) -> bytes:
+    """extract_row_from_page (crates/morphogen-dpf/src/page.rs:698-708)."""
+    buf = _blob(page)
+    out = np.zeros(ROW_SIZE_BYTES, dtype=np.uint8)
+    rc = lib.mg_page_extract_row(_u8(buf), len(page), row_offset, _u8(out))
+    if rc == MG_ERR_ROWS:
+        raise ExtractError(rc, f"row offset {row_offset}")
+    _check(rc)
+    return out.tobytes()
+
+
+def extract_rows_from_gpu_pages(pages: Sequence[bytes], addresses: Sequence[PageAddress]):
+    """extract_rows_from_pages (page.rs:140-162)."""
+    return [extract_row_from_page(p, a.row_offset) for p, a in zip(pages, addresses)]
+
+
+def page_query_body(keys: Sequence[bytes]) -> bytes:
+    """Request body of POST /query/page/gpu (dto.rs:126-130)."""
+    return ('{"keys":[' + ",".join('"0x' + bytes(k).hex() + '"' for k in keys) + "]}").encode()
+
+
+def parse_page_response(body: bytes):
+    """(epoch_id, pages) of a /query/page/gpu response (dto.rs PageQueryResponse)."""
+    import json
+    doc = json.loads(body)
+    return doc["epoch_id"], [bytes.fromhex(p[2:] if p.startswith("0x") else p)
+                             for p in doc["pages"]]
 

@@ -1,6 +1,8 @@
-// Client-side DPF key generation (host code): the caller of the scan needs
-// key pairs, as the reference client does (crates/morphogen-client/src/gpu.rs
-// calls generate_chacha_dpf_keys, crates/morphogen-gpu-dpf/src/dpf.rs:322-432).
+// Client side (host code): DPF key generation, cuckoo addressing, page
+// queries and the reconstruction of answers, as the reference client does
+// (crates/morphogen-client/src/gpu.rs:17-49 and page.rs:67-163 over
+// crates/morphogen-gpu-dpf/src/dpf.rs:322-432 and
+// crates/morphogen-core/src/cuckoo.rs:30-40).
 // Root seeds come from a caller-given 64-bit seed instead of the OS RNG so
 // load generators and tests are reproducible; keys are emitted in the wire
 // format of dpf.rs:259-274.
@@ -127,3 +129,106 @@ extern "C" int mg_keygen(uint32_t domain_bits, uint64_t target, uint64_t seed, u
   key1[18] = (uint8_t)d;
   return MG_OK;
 }
+
+// ------------------------------------------------------ cuckoo addressing --
+namespace {
+
+// SipHash-1-3 with the all-zero key: what Rust's std DefaultHasher::new()
+// computes, over the bytes written to it. Written from the published
+// SipHash construction (Aumasson & Bernstein): four 64-bit lanes, one
+// SipRound per 8-byte little-endian word, the length mod 256 in the top byte
+// of the last word, three finalisation rounds after v2 ^= 0xff.
+struct Sip13 {
+  uint64_t v0 = 0x736f6d6570736575ull, v1 = 0x646f72616e646f6dull;
+  uint64_t v2 = 0x6c7967656e657261ull, v3 = 0x7465646279746573ull;
+  static uint64_t rotl(uint64_t x, int b) { return (x << b) | (x >> (64 - b)); }
+  void round() {
+    v0 += v1; v1 = rotl(v1, 13); v1 ^= v0; v0 = rotl(v0, 32);
+    v2 += v3; v3 = rotl(v3, 16); v3 ^= v2;
+    v0 += v3; v3 = rotl(v3, 21); v3 ^= v0;
+    v2 += v1; v1 = rotl(v1, 17); v1 ^= v2; v2 = rotl(v2, 32);
+  }
+  void absorb(uint64_t m) {
+    v3 ^= m;
+    round();
+    v0 ^= m;
+  }
+  uint64_t finish(uint64_t last) {
+    absorb(last);
+    v2 ^= 0xff;
+    round();
+    round();
+    round();
+    return v0 ^ v1 ^ v2 ^ v3;
+  }
+};
+
+// DefaultHasher: write_u64(seed) then write(key) (cuckoo.rs:33-37). The seed
+// is exactly one 8-byte word, so the key's words start word-aligned.
+uint64_t cuckoo_hash(uint64_t seed, const uint8_t* key, size_t len) {
+  Sip13 h;
+  h.absorb(seed);
+  size_t i = 0;
+  for (; i + 8 <= len; i += 8) {
+    uint64_t m = 0;
+    for (int b = 0; b < 8; b++) m |= (uint64_t)key[i + b] << (8 * b);
+    h.absorb(m);
+  }
+  uint64_t last = (uint64_t)((len + 8) & 0xff) << 56;
+  for (size_t b = 0; i + b < len; b++) last |= (uint64_t)key[i + b] << (8 * b);
+  return h.finish(last);
+}
+
+}  // namespace
+
+extern "C" int mg_cuckoo_hash_indices(const uint8_t* key, size_t key_len, uint64_t num_rows,
+                                      const uint64_t seeds[3], uint64_t out[3]) {
+  if ((!key && key_len) || !seeds || !out) return MG_ERR_INVALID_ARG;
+  if (num_rows == 0) return MG_ERR_ROWS;
+  for (int i = 0; i < 3; i++) out[i] = cuckoo_hash(seeds[i], key, key_len) % num_rows;
+  return MG_OK;
+}
+
+extern "C" int mg_page_query(const uint8_t* account_key, size_t key_len, uint64_t num_pages,
+                             const uint64_t seeds[3], uint32_t domain_bits, uint64_t rng_seed,
+                             uint8_t* keys_a, uint8_t* keys_b, uint64_t page_index[3],
+                             uint32_t row_offset[3]) {
+  if (!keys_a || !keys_b || !page_index || !row_offset) return MG_ERR_INVALID_ARG;
+  if (domain_bits == 0 || domain_bits > 32) return MG_ERR_DOMAIN;
+  // num_pages * ROWS_PER_PAGE must not overflow (gpu.rs:23-29).
+  if (num_pages > UINT64_MAX / MG_ROWS_PER_PAGE) return MG_ERR_DOMAIN;
+  uint64_t rows[3];
+  int rc = mg_cuckoo_hash_indices(account_key, key_len, num_pages * MG_ROWS_PER_PAGE, seeds, rows);
+  if (rc) return rc;
+  const size_t klen = 19 + 18 * (size_t)domain_bits + 16;
+  for (int i = 0; i < 3; i++) {
+    page_index[i] = rows[i] / MG_ROWS_PER_PAGE;  // PageAddress::from_row_index
+    row_offset[i] = (uint32_t)(rows[i] % MG_ROWS_PER_PAGE);
+    // Independent root seeds per key pair.
+    rc = mg_keygen(domain_bits, page_index[i], splitmix64(rng_seed, 16 + (uint64_t)i),
+                   keys_a + i * klen, keys_b + i * klen);
+    if (rc) return rc;  // MG_ERR_ROWS: page index beyond 2^d (PageIndexTooLarge)
+  }
+  return MG_OK;
+}
+
+extern "C" int mg_page_aggregate(uint64_t epoch_a, const uint8_t* pages_a, uint64_t epoch_b,
+                                 const uint8_t* pages_b, size_t page_bytes, uint32_t n_pages,
+                                 uint8_t* out) {
+  if ((!pages_a || !pages_b || !out) && page_bytes && n_pages) return MG_ERR_INVALID_ARG;
+  if (epoch_a != epoch_b) return MG_ERR_STATE;  // PageAggregationError::EpochMismatch
+  const size_t n = page_bytes * n_pages;
+  for (size_t i = 0; i < n; i++) out[i] = (uint8_t)(pages_a[i] ^ pages_b[i]);
+  return MG_OK;
+}
+
+extern "C" int mg_page_extract_row(const uint8_t* page, size_t page_bytes, uint32_t row_offset,
+                                   uint8_t* out) {
+  if (!page || !out) return MG_ERR_INVALID_ARG;
+  if (row_offset >= MG_ROWS_PER_PAGE) return MG_ERR_ROWS;  // ExtractError::InvalidRowOffset
+  const size_t start = (size_t)row_offset * MG_PAGE_ROW_BYTES;
+  if (start + MG_PAGE_ROW_BYTES > page_bytes) return MG_ERR_ROWS;
+  memcpy(out, page + start, MG_PAGE_ROW_BYTES);
+  return MG_OK;
+}
+

@@ -70,6 +70,10 @@ def lib():
                                      ctypes.c_uint32, u8p]),
             "orc_lcg_fill": (None, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, u8p]),
             "orc_lcg_state_at": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64]),
+            "orc_siphash": (ctypes.c_uint64, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                              ctypes.c_uint64, u8p, ctypes.c_size_t]),
+            "orc_cuckoo_indices": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint64, u64p,
+                                                  u64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -217,3 +221,20 @@ def lcg_fill(seed, first_byte, n_bytes) -> np.ndarray:
 
 def lcg_state_at(seed, steps):
     return lib().orc_lcg_state_at(seed, steps)
+
+
+def siphash(msg: bytes, c_rounds=1, d_rounds=3, k0=0, k1=0) -> int:
+    buf = np.frombuffer(bytes(msg) or b"\0", dtype=np.uint8).copy()
+    return int(lib().orc_siphash(c_rounds, d_rounds, k0, k1, _ptr(buf), len(msg)))
+
+
+def cuckoo_indices(key: bytes, num_rows: int, seeds=(0x12345678, 0x9ABCDEF0, 0xFEDCBA98)):
+    """CuckooAddresser::with_seeds(num_rows, seeds).hash_indices(key) (cuckoo.rs:25-40);
+    the default seeds are CuckooAddresser::new's."""
+    buf = np.frombuffer(bytes(key) or b"\0", dtype=np.uint8).copy()
+    sd = np.array(seeds, dtype=np.uint64)
+    out = np.zeros(3, dtype=np.uint64)
+    _check(lib().orc_cuckoo_indices(_ptr(buf), len(key), num_rows, _ptr(sd, ctypes.c_uint64),
+                                    _ptr(out, ctypes.c_uint64)), "cuckoo_indices")
+    return [int(x) for x in out]
+

@@ -491,3 +491,56 @@ void orc_lcg_fill(uint64_t seed, uint64_t first_byte, uint64_t n_bytes,
     out[i] = (uint8_t)(state >> 56);
   }
 }
+
+/* ------------------------------------------------------------- SipHash -- */
+/* The published SipHash-c-d: four 64-bit lanes initialised from the key and
+ * the constants "somepseudorandomlygeneratedbytes"; each 8-byte little-endian
+ * word m is absorbed as v3 ^= m, c SipRounds, v0 ^= m; the last word carries
+ * the length mod 256 in its top byte; finalisation XORs 0xff into v2 and runs
+ * d SipRounds. Rust's std DefaultHasher::new() is SipHash-1-3 with k0 = k1 = 0
+ * (cuckoo.rs:33-37 hashes with it). */
+static inline uint64_t rotl64(uint64_t x, int b) { return (x << b) | (x >> (64 - b)); }
+
+static inline void sip_round(uint64_t v[4]) {
+  v[0] += v[1]; v[1] = rotl64(v[1], 13); v[1] ^= v[0]; v[0] = rotl64(v[0], 32);
+  v[2] += v[3]; v[3] = rotl64(v[3], 16); v[3] ^= v[2];
+  v[0] += v[3]; v[3] = rotl64(v[3], 21); v[3] ^= v[0];
+  v[2] += v[1]; v[1] = rotl64(v[1], 17); v[1] ^= v[2]; v[2] = rotl64(v[2], 32);
+}
+
+uint64_t orc_siphash(int c_rounds, int d_rounds, uint64_t k0, uint64_t k1,
+                     const uint8_t *msg, size_t len) {
+  uint64_t v[4] = {k0 ^ 0x736f6d6570736575ull, k1 ^ 0x646f72616e646f6dull,
+                   k0 ^ 0x6c7967656e657261ull, k1 ^ 0x7465646279746573ull};
+  size_t i = 0;
+  for (; i + 8 <= len; i += 8) {
+    uint64_t m = 0;
+    for (int b = 0; b < 8; b++) m |= (uint64_t)msg[i + b] << (8 * b);
+    v[3] ^= m;
+    for (int r = 0; r < c_rounds; r++) sip_round(v);
+    v[0] ^= m;
+  }
+  uint64_t last = (uint64_t)(len & 0xff) << 56;
+  for (size_t b = 0; i + b < len; b++) last |= (uint64_t)msg[i + b] << (8 * b);
+  v[3] ^= last;
+  for (int r = 0; r < c_rounds; r++) sip_round(v);
+  v[0] ^= last;
+  v[2] ^= 0xff;
+  for (int r = 0; r < d_rounds; r++) sip_round(v);
+  return v[0] ^ v[1] ^ v[2] ^ v[3];
+}
+
+int orc_cuckoo_indices(const uint8_t *key, size_t len, uint64_t num_rows,
+                       const uint64_t seeds[3], uint64_t out[3]) {
+  if (num_rows == 0) return ORC_ERR_ROWS;
+  uint8_t *buf = (uint8_t *)malloc(len + 8);
+  if (!buf) return ORC_ERR_ROWS;
+  for (int i = 0; i < 3; i++) {
+    for (int b = 0; b < 8; b++) buf[b] = (uint8_t)(seeds[i] >> (8 * b));
+    if (len) memcpy(buf + 8, key, len);
+    out[i] = orc_siphash(1, 3, 0, 0, buf, len + 8) % num_rows;
+  }
+  free(buf);
+  return ORC_OK;
+}
+

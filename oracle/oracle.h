@@ -14,6 +14,8 @@
  *   - masked XOR + fused tiled scan      crates/morphogen-gpu-dpf/src/kernel.rs:1243-1383
  *   - delta scan                         crates/morphogen-server/src/scan.rs:434-485
  *   - byte-LCG fill                      crates/morphogen-storage/src/chunked_matrix.rs:83-93
+ *   - cuckoo addressing                  crates/morphogen-core/src/cuckoo.rs:30-40 (Rust std
+ *                                        DefaultHasher = SipHash-1-3, all-zero key)
  * generalised from 4096-byte pages to W-byte rows (W a multiple of 16) with one
  * DPF leaf per row.
  *
@@ -33,6 +35,14 @@
  *                      XOR == target row; delta linearity; closed forms for
  *                      all-zero / all-ones rows.
  *   orc_lcg_fill       jump-ahead equals the sequential byte loop.
+ *   orc_siphash        (c, d) = (1, 3) and (2, 4) equal OpenSSL 3's SIPHASH MAC
+ *                      with c-rounds/d-rounds set (committed fixture
+ *                      tests/golden/siphash_openssl.json); (2, 4) with the
+ *                      zero key also equals CPython 3.10's bytes hash under
+ *                      PYTHONHASHSEED=0 (an independent implementation).
+ *   orc_cuckoo_indices the message layout (LE64(seed) || key, then % rows) is
+ *                      pinned by reading cuckoo.rs:30-40 and Rust's Hasher
+ *                      (write_u64 writes the native little-endian bytes).
  * Wire-format field semantics (which byte is t_cw_left) are pinned by reading
  * dpf.rs:259-316 only: no reference-generated key exists to compare against.
  */
@@ -101,6 +111,14 @@ void orc_fill_rows(uint64_t seed, uint64_t first_row, uint64_t n_rows,
 void orc_lcg_fill(uint64_t seed, uint64_t first_byte, uint64_t n_bytes,
                   uint8_t *out);
 uint64_t orc_lcg_state_at(uint64_t seed, uint64_t steps);
+
+/* SipHash-c-d (Aumasson & Bernstein 2012) of msg under key (k0, k1). */
+uint64_t orc_siphash(int c_rounds, int d_rounds, uint64_t k0, uint64_t k1,
+                     const uint8_t *msg, size_t len);
+/* CuckooAddresser::hash_indices: out[i] = SipHash-1-3_0(LE64(seeds[i]) ||
+ * key) % num_rows. num_rows = 0 is refused (the reference divides by it). */
+int orc_cuckoo_indices(const uint8_t *key, size_t len, uint64_t num_rows,
+                       const uint64_t seeds[3], uint64_t out[3]);
 
 #ifdef __cplusplus
 }

@@ -142,6 +142,26 @@ def test_statuses_that_need_the_loop(mg, setup):
         small.close()
 
 
+def test_client_flow_through_the_endpoint(mg, setup):
+    # generate_gpu_page_query -> one request per server -> aggregate -> rows
+    # (crates/morphogen-client/src/gpu.rs:17-49, page.rs:67-162). Both
+    # parties' requests go to the same loop here; each sees only its own keys.
+    _, _, _, api, rows = setup
+    for acct in (b"0xd8dA6BF26964aF9D7eEd9e03E53415D37aA96045", b"test_account"):
+        q = mg.generate_gpu_page_query(acct, N, rng_seed=17)
+        answers = []
+        for keys in (q.keys_a, q.keys_b):
+            st, out, _ = api.handle("POST", mg.WIRE_PATH, "application/json",
+                                    mg.page_query_body(keys))
+            assert st == 200
+            answers.append(mg.parse_page_response(out))
+        epoch, pages = mg.aggregate_gpu_page_responses(*answers[0], *answers[1])
+        got = mg.extract_rows_from_gpu_pages(pages, q.page_addresses)
+        for i, a in enumerate(q.page_addresses):
+            assert pages[i] == rows[a.page_index].tobytes()
+            assert got[i] == rows[a.page_index][256 * a.row_offset:256 * (a.row_offset + 1)].tobytes()
+
+
 def test_http_round_trip(mg, setup):
     from morphogenesis_amd.http_api import serve
     scanner, m, _, api, _ = setup
