@@ -131,6 +131,20 @@ int mg_matrix_info(const mg_matrix *m, uint64_t *n_rows, uint32_t *row_bytes);
 int mg_matrix_copy(mg_matrix *dst, const mg_matrix *src);
 int mg_matrix_destroy(mg_matrix *m);
 
+/* Raw row file -> new device matrix (load_matrix_from_file,
+ * crates/morphogen-server/src/bin/bench_page_gpu_batch.rs:326-379, then
+ * GpuPageMatrix::from_chunked_matrix, crates/morphogen-gpu-dpf/src/
+ * storage.rs:46-86). The file is rows back to back (e.g. the 32-byte Compact
+ * schema, docs/DATA_STRUCTURES.md:17-26). With MG_LOAD_PAD_TAIL a ragged last
+ * row is zero-padded, as the reference pads to a whole page; without it the
+ * file must hold whole rows (decode_snapshot_matrix, network/
+ * snapshot_config.rs:144-159). An empty file is MG_ERR_ROWS. Reads overlap
+ * the copies (two pinned 64 MiB staging buffers on a stream of its own), so
+ * a loop can keep serving while the next snapshot loads. */
+#define MG_LOAD_PAD_TAIL 1u
+int mg_matrix_load_file(mg_device *dev, const char *path, uint32_t row_bytes, uint32_t flags,
+                        mg_matrix **out);
+
 /* ChaChaKey::from_bytes (dpf.rs:277-316): exact length 19 + 18d + 16, plus
  * the evaluator's limit d <= 32 (ChaChaParams::new, dpf.rs:65-78). */
 int mg_key_parse(const uint8_t *bytes, size_t len, mg_key_info *out);
@@ -250,6 +264,8 @@ typedef struct {
   uint64_t rebases;
   uint32_t active_keys, queued_requests, max_active_keys, segments;
   double busy_ms;             /* time spent in sweep steps                 */
+  uint64_t snapshots;         /* matrices swapped in by mg_server_snapshot */
+  uint64_t n_rows;            /* rows of the matrix served now             */
 } mg_server_stats;
 
 int mg_server_create(mg_device *dev, mg_matrix *m, const mg_server_opts *opts,
@@ -266,6 +282,15 @@ int mg_server_wait(mg_server *s, uint64_t ticket, uint8_t *out,
  * after this call (EpochManager::try_advance, epoch.rs:465-549). */
 int mg_server_update(mg_server *s, const uint64_t *row_idx,
                      const uint8_t *diffs, uint64_t n_entries, uint64_t epoch);
+/* Snapshot hot-swap (EpochManager::submit_snapshot, crates/morphogen-server/
+ * src/epoch.rs:407-463, behind admin_snapshot_handler, network/api/mod.rs:
+ * 190-237): the loop serves `m` (same row width and device, at most 2^d rows)
+ * for every request submitted after this call, at epoch `epoch`. Requests
+ * admitted before it finish on the old rows; updates not yet merged are
+ * dropped, as the reference drains its pending delta. Blocks until the swap
+ * is done; the old matrix is then no longer read and may be destroyed. The
+ * loop owns the matrix it serves: merges write into it. */
+int mg_server_snapshot(mg_server *s, mg_matrix *m, uint64_t epoch);
 int mg_server_stats_get(mg_server *s, mg_server_stats *out);
 int mg_server_destroy(mg_server *s);
 

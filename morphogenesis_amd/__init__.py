@@ -118,7 +118,8 @@ class ServerStats(ctypes.Structure):
                 ("epoch", ctypes.c_uint64), ("delta_entries", ctypes.c_uint64),
                 ("rebases", ctypes.c_uint64), ("active_keys", ctypes.c_uint32),
                 ("queued_requests", ctypes.c_uint32), ("max_active_keys", ctypes.c_uint32),
-                ("segments", ctypes.c_uint32), ("busy_ms", ctypes.c_double)]
+                ("segments", ctypes.c_uint32), ("busy_ms", ctypes.c_double),
+                ("snapshots", ctypes.c_uint64), ("n_rows", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -148,6 +149,8 @@ def _load():
         "mg_matrix_info": (ctypes.c_int, [vp, u64p, ctypes.POINTER(ctypes.c_uint32)]),
         "mg_matrix_copy": (ctypes.c_int, [vp, vp]),
         "mg_matrix_destroy": (ctypes.c_int, [vp]),
+        "mg_matrix_load_file": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_uint32,
+                                               ctypes.c_uint32, ctypes.POINTER(vp)]),
         "mg_keygen": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, u8p,
                                      u8p]),
         "mg_key_parse": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.POINTER(KeyInfo)]),
@@ -170,6 +173,7 @@ def _load():
                                           ctypes.POINTER(RequestInfo)]),
         "mg_server_update": (ctypes.c_int, [vp, u64p, u8p, ctypes.c_uint64, ctypes.c_uint64]),
         "mg_server_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
+        "mg_server_snapshot": (ctypes.c_int, [vp, vp, ctypes.c_uint64]),
         "mg_server_destroy": (ctypes.c_int, [vp]),
         "mg_cuckoo_hash_indices": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint64, u64p,
                                                   u64p]),
@@ -395,6 +399,24 @@ class GpuPageMatrix:
         return cls(scanner, n_rows, row_bytes)
 
     @classmethod
+    def from_file(cls, scanner, path: str, row_bytes: int, pad_tail: bool = True):
+        """A raw row file streamed into a new device matrix (load_matrix_from_file +
+        from_chunked_matrix, bench_page_gpu_batch.rs:326-379, storage.rs:46-86).
+        pad_tail=False refuses a file that is not whole rows (snapshot_config.rs:152)."""
+        h = ctypes.c_void_p()
+        _check(lib.mg_matrix_load_file(scanner._h, os.fsencode(path), row_bytes,
+                                       1 if pad_tail else 0, ctypes.byref(h)))
+        m = cls.__new__(cls)
+        m._h = h
+        m.scanner = scanner
+        m.row_bytes = row_bytes
+        n = ctypes.c_uint64()
+        w = ctypes.c_uint32()
+        _check(lib.mg_matrix_info(h, ctypes.byref(n), ctypes.byref(w)))
+        m.n_rows = n.value
+        return m
+
+    @classmethod
     def new(cls, scanner, rows: np.ndarray):
         rows = np.ascontiguousarray(rows, dtype=np.uint8)
         m = cls(scanner, rows.shape[0], rows.shape[1])
@@ -517,6 +539,13 @@ class GpuServer:
         st = ServerStats()
         _check(lib.mg_server_stats_get(self._h, ctypes.byref(st)))
         return st.as_dict()
+
+    def submit_snapshot(self, matrix: GpuPageMatrix, epoch: int):
+        """EpochManager::submit_snapshot (epoch.rs:407-463): serve `matrix` for every
+        request submitted from now on, at `epoch`; pending updates are dropped.
+        Returns once the swap is done; the previous matrix is then free to close."""
+        _check(lib.mg_server_snapshot(self._h, matrix._h, epoch))
+        self.matrix = matrix
 
     def close(self):
         if self._h:

@@ -17,6 +17,10 @@
 //  - When the cumulative delta grows past half its capacity the loop merges it
 //    into a base copy: into the spare copy when double-buffered (in-flight
 //    keys keep the old base), otherwise in place once in-flight keys drain.
+//  - A snapshot (EpochManager::submit_snapshot, epoch.rs:407-463) replaces
+//    the served matrix: requests admitted before it finish on the old rows,
+//    the pending updates are dropped (the new rows include them), and later
+//    requests start on the new rows at the snapshot's epoch.
 #include <algorithm>
 #include <condition_variable>
 #include <cstring>
@@ -50,10 +54,22 @@ struct Update {
   uint64_t epoch = 0;
 };
 
-// One entry of the submission-ordered queue: a request or an update.
+// A matrix replacing the served one; owned by the caller blocked in
+// mg_server_snapshot until `done`.
+struct Snapshot {
+  mg_matrix* m = nullptr;
+  uint64_t epoch = 0;
+  bool done = false;
+  int status = MG_OK;
+  std::string error;
+};
+
+// One entry of the submission-ordered queue: a request, an update or a
+// snapshot.
 struct Event {
   Request* req = nullptr;
   std::unique_ptr<Update> upd;
+  Snapshot* snap = nullptr;
 };
 
 struct Slot {
@@ -70,6 +86,8 @@ struct mg_server {
   mg_matrix* m = nullptr;
   uint32_t row_bytes = 0, chunks = 0, d = 0, kl = 0, segments = 0;
   uint64_t n_rows = 0, seg_rows = 0;
+  uint64_t queued_n_rows = 0;  // rows after every queued snapshot (under mu)
+  uint32_t seg_want = 0;       // segments option (0 = one per 2^25 rows)
   uint32_t cap_keys = 0;
   bool double_buffer = false;
 
@@ -303,7 +321,56 @@ void fail_all(mg_server* s, int code, const std::string& msg) {
       r->error = msg;
     }
   }
+  for (Event& ev : s->queue)
+    if (ev.snap && !ev.snap->done) {
+      ev.snap->done = true;
+      ev.snap->status = code;
+      ev.snap->error = msg;
+    }
   s->cv_done.notify_all();
+}
+
+// Rows per sweep step: whole tiles of the fused kernel (2^(kl + lb) rows),
+// `want` steps per pass, or one step per 2^25 rows (at most 64) by default.
+// Measured at 2^27 x 32 B under Poisson load, 4 steps per pass gave p50/p99
+// of 416/1340 ms against 504/2003 ms at 32 steps: larger steps spend less on
+// launches and host round trips, and admission waits at most one step.
+void plan_segments(mg_server* s) {
+  const uint32_t lb = std::min<uint32_t>(6, s->d - s->kl);
+  const uint64_t tile = 1ull << (s->kl + lb);
+  const uint32_t want =
+      s->seg_want ? s->seg_want
+                  : (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, s->n_rows >> 25));
+  uint64_t seg = (s->n_rows + want - 1) / want;
+  seg = std::max<uint64_t>(tile, (seg + tile - 1) / tile * tile);
+  s->seg_rows = seg;
+  s->segments = (uint32_t)((s->n_rows + seg - 1) / seg);
+}
+
+// Serves `m` from now on. Caller holds dev->mu; no key is in flight.
+void apply_snapshot(mg_server* s, const Snapshot& sn) {
+  const size_t bytes = (size_t)sn.m->n_rows * s->row_bytes;
+  if (s->double_buffer && sn.m->n_rows != s->n_rows) {
+    // A spare of the new size; without the HBM for it the loop goes on
+    // single-buffered (merges then wait for in-flight keys).
+    uint4* spare = nullptr;
+    if (hipMalloc((void**)&spare, bytes) != hipSuccess) {
+      spare = nullptr;
+      (void)hipGetLastError();
+    }
+    (void)hipFree(s->base_rows[1]);
+    s->base_rows[1] = spare;
+    if (!spare) s->double_buffer = false;
+  }
+  s->m = sn.m;
+  s->base_rows[0] = sn.m->rows;
+  s->cur_base = 0;
+  s->n_rows = sn.m->n_rows;
+  plan_segments(s);
+  s->step = 0;
+  s->c_idx.clear();
+  s->c_diffs.clear();
+  s->rebase_wanted = false;
 }
 
 void server_loop(mg_server* s) {
@@ -311,6 +378,8 @@ void server_loop(mg_server* s) {
   for (;;) {
     std::vector<std::unique_ptr<Update>> ups_before;  // applied before `admit`
     std::vector<Request*> admit;
+    Snapshot* snap = nullptr;  // taken only when no key is in flight
+    bool snap_applied = false;  // once set, `snap` may already be gone
     std::unique_ptr<Update> upd_after;  // an update queued behind admitted requests
     {
       std::unique_lock<std::mutex> lk(s->mu);
@@ -325,6 +394,13 @@ void server_loop(mg_server* s) {
       uint64_t c_n = s->c_idx.size();
       while (!s->queue.empty()) {
         Event& ev = s->queue.front();
+        if (ev.snap) {
+          // Requests admitted before it finish on the old rows first.
+          if (!admit.empty() || s->active > 0) break;
+          snap = ev.snap;
+          s->queue.pop_front();
+          break;
+        }
         if (ev.upd) {
           if (!admit.empty()) break;
           if (c_n + ev.upd->idx.size() > s->c_cap) {
@@ -350,7 +426,7 @@ void server_loop(mg_server* s) {
     {
       std::lock_guard<std::mutex> dlk(s->dev->mu);
       rc = mg::set_device(s->dev);
-      if (!rc) rc = try_rebase(s);
+      if (!rc && !snap) rc = try_rebase(s);
       for (auto& u : ups_before) {
         if (rc) break;
         rc = apply_update(s, *u);
@@ -358,6 +434,17 @@ void server_loop(mg_server* s) {
           std::lock_guard<std::mutex> lk(s->mu);
           s->epoch = u->epoch;
         }
+      }
+      if (!rc && snap) {
+        apply_snapshot(s, *snap);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->epoch = snap->epoch;
+        s->stats.segments = s->segments;
+        s->stats.snapshots++;
+        s->stats.n_rows = s->n_rows;
+        snap->done = true;
+        snap_applied = true;
+        s->cv_done.notify_all();
       }
       if (!rc && !admit.empty()) {
         const double now = mg::now_ms();
@@ -374,6 +461,13 @@ void server_loop(mg_server* s) {
         if (!rc) s->active += n_new;
       }
       if (!rc && s->active) rc = sweep_step(s, &finished);
+    }
+    if (snap && !snap_applied) {  // the loop failed before the swap
+      std::lock_guard<std::mutex> lk(s->mu);
+      snap->done = true;
+      snap->status = rc ? rc : MG_ERR_STATE;
+      snap->error = mg::last_error();
+      s->cv_done.notify_all();
     }
     if (rc) {
       fail_all(s, rc, mg::last_error());
@@ -439,21 +533,10 @@ int mg_server_create(mg_device* dev, mg_matrix* m, const mg_server_opts* opts, m
   s->d = d;
   s->n_rows = m->n_rows;
   s->double_buffer = o->double_buffer != 0;
-  // Segments are whole tiles of the fused kernel (2^(kl + lb) rows).
   s->kl = std::min<uint32_t>(o->lane_depth ? o->lane_depth : (s->chunks > 8 ? 6 : 10), d);
-  const uint32_t lb = std::min<uint32_t>(6, d - s->kl);
-  const uint64_t tile = 1ull << (s->kl + lb);
-  // Default: one step per 2^25 rows (at most 64 steps). Measured at 2^27 x 32 B
-  // under Poisson load, 4 steps per pass gave p50/p99 of 416/1340 ms against
-  // 504/2003 ms at 32 steps: larger steps spend less on launches and host
-  // round trips, and admission waits at most one step either way.
-  const uint32_t want =
-      o->segments ? o->segments
-                  : (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, s->n_rows >> 25));
-  uint64_t seg = (s->n_rows + want - 1) / want;
-  seg = (seg + tile - 1) / tile * tile;
-  s->seg_rows = seg;
-  s->segments = (uint32_t)((s->n_rows + seg - 1) / seg);
+  s->seg_want = o->segments;
+  s->queued_n_rows = s->n_rows;
+  plan_segments(s.get());
   s->c_cap = o->max_delta_entries ? o->max_delta_entries : (1ull << 20);
 
   std::lock_guard<std::mutex> lk(dev->mu);
@@ -499,6 +582,7 @@ int mg_server_create(mg_device* dev, mg_matrix* m, const mg_server_opts* opts, m
   for (uint32_t i = (uint32_t)cap; i-- > 0;) s->free_slots.push_back(i);
   s->stats.max_active_keys = (uint32_t)cap;
   s->stats.segments = s->segments;
+  s->stats.n_rows = s->n_rows;
   s->thread = std::thread(server_loop, s.get());
   *out = s.release();
   return MG_OK;
@@ -575,18 +659,44 @@ int mg_server_update(mg_server* s, const uint64_t* row_idx, const uint8_t* diffs
                 (unsigned long long)n_entries, (unsigned long long)s->c_cap);
   std::unique_ptr<Update> u(new Update());
   u->idx.assign(row_idx, row_idx + n_entries);
-  for (uint64_t i : u->idx)
-    if (i >= s->n_rows)
-      return fail(MG_ERR_ROWS, "update row %llu outside the %llu-row matrix",
-                  (unsigned long long)i, (unsigned long long)s->n_rows);
   u->diffs.assign(diffs, diffs + n_entries * s->row_bytes);
   u->epoch = epoch;
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->fatal) return fail(s->fatal, "server stopped: %s", s->fatal_msg.c_str());
+  // Rows as of the queue's end: an update applies to the matrix of the last
+  // snapshot submitted before it.
+  for (uint64_t i : u->idx)
+    if (i >= s->queued_n_rows)
+      return fail(MG_ERR_ROWS, "update row %llu outside the %llu-row matrix",
+                  (unsigned long long)i, (unsigned long long)s->queued_n_rows);
   Event ev;
   ev.upd = std::move(u);
   s->queue.push_back(std::move(ev));
   s->cv_work.notify_one();
+  return MG_OK;
+}
+
+int mg_server_snapshot(mg_server* s, mg_matrix* m, uint64_t epoch) {
+  if (!s || !m) return fail(MG_ERR_INVALID_ARG, "null argument");
+  if (m->dev != s->dev) return fail(MG_ERR_INVALID_ARG, "snapshot matrix is on another device");
+  if (m->row_bytes != s->row_bytes)
+    return fail(MG_ERR_ROWS, "snapshot rows are %u bytes, the loop serves %u", m->row_bytes,
+                s->row_bytes);
+  if (m->n_rows == 0 || (s->d < 64 && m->n_rows > (1ull << s->d)))
+    return fail(MG_ERR_ROWS, "snapshot of %llu rows does not fit the 2^%u-leaf domain",
+                (unsigned long long)m->n_rows, s->d);
+  Snapshot sn;
+  sn.m = m;
+  sn.epoch = epoch;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (s->fatal) return fail(s->fatal, "server stopped: %s", s->fatal_msg.c_str());
+  Event ev;
+  ev.snap = &sn;
+  s->queue.push_back(std::move(ev));
+  s->queued_n_rows = m->n_rows;
+  s->cv_work.notify_one();
+  s->cv_done.wait(lk, [&] { return sn.done; });
+  if (sn.status != MG_OK) return fail(sn.status, "snapshot failed: %s", sn.error.c_str());
   return MG_OK;
 }
 
@@ -612,6 +722,12 @@ int mg_server_destroy(mg_server* s) {
         kv.second->done = true;
         kv.second->status = MG_ERR_STATE;
         kv.second->error = "server destroyed";
+      }
+    for (Event& ev : s->queue)
+      if (ev.snap && !ev.snap->done) {
+        ev.snap->done = true;
+        ev.snap->status = MG_ERR_STATE;
+        ev.snap->error = "server destroyed";
       }
     s->cv_work.notify_all();
     s->cv_done.notify_all();
