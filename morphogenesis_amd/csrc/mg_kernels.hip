@@ -29,6 +29,15 @@ __device__ __forceinline__ uint4 opaque4(uint4 v) {
   return v;
 }
 
+// acc ^ (v & m) as one v_bitop3_b32 (truth table 0x78 over S0 = acc, S1 = v,
+// S2 = m). Used where the compiler would otherwise emit v_and + v_xor.
+__device__ __forceinline__ void xor_masked3(uint4& acc, const uint4& v, const uint4& m) {
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x78" : "=v"(acc.x) : "v"(acc.x), "v"(v.x), "v"(m.x));
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x78" : "=v"(acc.y) : "v"(acc.y), "v"(v.y), "v"(m.y));
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x78" : "=v"(acc.z) : "v"(acc.z), "v"(v.z), "v"(m.z));
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x78" : "=v"(acc.w) : "v"(acc.w), "v"(v.w), "v"(m.w));
+}
+
 __device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v ^= __shfl_xor(v, off, 64);
@@ -311,32 +320,39 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
   // batches of U rows (8 KiB of 1 KiB rows), two batches in flight: the first
   // two batches of a step are issued before that step's PRG work, so loads
   // overlap the ChaCha rounds.
+  // Loads are branch-free: a row's base address is wave-uniform (a row past
+  // the step or the matrix reads the zero buffer instead), and each lane's
+  // chunk offset is fixed for the whole kernel. A lane past the row's last
+  // chunk re-reads chunk 0; its accumulator is never written out.
   constexpr int U = 8 / S;
   const uint32_t step_rows = 2u * (uint32_t)lanes_live;
+  uint32_t lane_off[S];
+#pragma unroll
+  for (int j = 0; j < S; j++) {
+    const uint32_t c = lane + 64 * j;
+    lane_off[j] = c < C ? c : 0;
+  }
   uint4 va[U][S], vb[U][S];
   auto load_batch = [&](uint64_t tile_base, uint32_t pair, uint32_t r0, uint4 (&v)[U][S]) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const uint32_t r = r0 + u;
       const uint64_t x = tile_base + ((uint64_t)(r >> 1) << kl) + 2 * (uint64_t)pair + (r & 1);
-      const bool ok = r < step_rows && x < p.n_rows;
-      const uint4* src = p.rows + x * C;
+      const uint4* src = (r < step_rows && x < p.n_rows) ? p.rows + x * C : p.zero_rows;
 #pragma unroll
-      for (int j = 0; j < S; j++) {
-        const uint32_t c = lane + 64 * j;
-        v[u][j] = (ok && c < C) ? load_nt(src + c) : make_uint4(0, 0, 0, 0);
-      }
+      for (int j = 0; j < S; j++) v[u][j] = load_nt(src + lane_off[j]);
     }
   };
+  // Slot indices stay below 128 (r0 + u < max(step_rows, U) <= 128), so they
+  // need no wrap and become immediate LDS offsets.
   auto fold_batch = [&](uint32_t r0, const uint4 (&v)[U][S]) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const uint32_t r = (r0 + u) & 127u;
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        const uint4 m = masks[g * 128 + r];
+        const uint4 m = masks[g * 128 + r0 + u];
 #pragma unroll
-        for (int j = 0; j < S; j++) xor_masked(acc[g][j], v[u][j], m);
+        for (int j = 0; j < S; j++) xor_masked3(acc[g][j], v[u][j], m);
       }
     }
   };

@@ -43,6 +43,7 @@ struct StreamParams {
   uint64_t tiles_per_wave;
   uint32_t parts;  // waves per key group
   uint4* partial;  // [parts][n_keys][chunks]
+  const uint4* zero_rows;  // >= 256 zero chunks, read in place of absent rows
 };
 
 struct PointParams {
