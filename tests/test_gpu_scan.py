@@ -93,11 +93,12 @@ def test_stream_kernel_matches_oracle(mg, scanner, n, w, d):
     ref = oracle.scan_tiled(keys, rows, THREADS)
     max_g = 4 if w <= 1024 else (3 if w <= 2048 else 2)
     for g in range(1, max_g + 1):
-        for kl in (1, 4, 6):
+        for kl, wpb in ((1, 1), (4, 2), (6, 1), (7, 1)):
             got, t = scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM, lane_depth=kl,
-                                                            keys_per_wave=g), with_timing=True)
+                                                            keys_per_wave=g, waves_per_block=wpb),
+                                       with_timing=True)
             assert t["kernel"] == "stream"
-            assert np.array_equal(got, ref), (g, kl)
+            assert np.array_equal(got, ref), (g, kl, wpb)
     k1 = [p[1] for p in pairs]
     g1 = scanner.scan_keys(m, k1)
     for q, tgt in enumerate(targets):
