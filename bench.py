@@ -1,10 +1,12 @@
 """Benchmark of the fused ChaCha8-DPF masked-XOR PIR scan on one MI355X.
 
 Workload (default): the full-mainnet shape, 2^31 rows x 32 B (64 GiB) of
-seeded uniform bytes resident in HBM, DPF keys over a 31-bit domain with
-uniform targets. One step = one batched pass of --keys keys over the whole
-matrix. Keys come in party-0/party-1 pairs so every step also checks, bit for
-bit, that each pair's answers XOR to the target row.
+seeded uniform bytes resident in HBM plus an epoch DeltaBuffer of 2^16
+updated rows (uniform indices, random diffs), DPF keys over a 31-bit domain
+with uniform targets. One step = one batched pass of --keys keys over the
+whole matrix and the delta. Keys come in party-0/party-1 pairs so every step
+also checks, bit for bit, that each pair's answers XOR to the target row of
+the updated matrix.
 
 Reported: batched queries/s (a query = 3 cuckoo keys), per-pass p50/p99
 latency, the plain one-query-per-pass rate beside it, scan GB/s and the
@@ -103,6 +105,9 @@ def main():
     ap.add_argument("--seed", type=int, default=31)
     ap.add_argument("--q-sweep", default="",
                     help="comma-separated key counts: one extra timed pass each, reported in by_keys")
+    ap.add_argument("--delta-rows", type=int, default=1 << 16,
+                    help="epoch DeltaBuffer entries applied in every pass (uniform rows, random "
+                         "diffs); 0 = none")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a 4 GiB matrix once (known bytes for counter scaling)")
     args = ap.parse_args()
@@ -145,13 +150,36 @@ def main():
     targets = [int(x) for x in rng.integers(0, n_rows, size=q // 2)]
     pairs = [mg.generate_keys(d, t, seed=1000 + i) for i, t in enumerate(targets)]
     keys = [k for p in pairs for k in p]
-    target_rows = [m.download(t, 1)[0] for t in targets]
+    # The epoch DeltaBuffer every pass applies on top of the matrix: uniform
+    # rows with replacement, random diffs; the first target is forced into it
+    # so the bit-exact check covers the delta path. Answers are for the
+    # matrix XOR the delta (scan_delta_for_gpu semantics).
+    delta = None
+    d_idx = np.zeros(0, dtype=np.uint64)
+    d_diffs = np.zeros((0, w), dtype=np.uint8)
+    if args.delta_rows:
+        d_idx = rng.integers(0, n_rows, size=args.delta_rows).astype(np.uint64)
+        d_idx[0] = targets[0]
+        d_diffs = rng.integers(0, 256, size=(args.delta_rows, w), dtype=np.uint8)
+        delta = mg.DeltaBuffer(scanner, w, args.delta_rows)
+        delta.set(d_idx, d_diffs, epoch=1)
+        log(f"delta: {args.delta_rows} rows applied in every pass")
+
+    def want_row(t):
+        """Row t of the matrix XOR every delta diff aimed at it."""
+        row = m.download(t, 1)[0].copy()
+        for j in np.nonzero(d_idx == np.uint64(t))[0]:
+            row ^= d_diffs[j]
+        return row
+
+    target_rows = [want_row(t) for t in targets]
     kernel = {"auto": mg.KERNEL_AUTO, "lane_dfs": mg.KERNEL_LANE_DFS,
               "stream": mg.KERNEL_STREAM, "point": mg.KERNEL_POINT,
               "suffix": mg.KERNEL_SUFFIX}[args.kernel]
     opts = mg.make_opts(kernel=kernel, lane_depth=args.lane_depth,
                         keys_per_wave=args.keys_per_wave, waves_per_key=args.waves_per_key,
-                        waves_per_block=args.waves_per_block, use_graph=args.use_graph)
+                        waves_per_block=args.waves_per_block, use_graph=args.use_graph,
+                        delta=delta)
 
     def check(ans):
         for i, t in enumerate(targets):
@@ -189,7 +217,7 @@ def main():
     # per row from a per-tile prefix (fused_kernel.cu:309-360), same matrix.
     ref_ms = []
     if w <= 128:
-        ref_opts = mg.make_opts(kernel=mg.KERNEL_SUFFIX)
+        ref_opts = mg.make_opts(kernel=mg.KERNEL_SUFFIX, delta=delta)
         for i in range(args.ref_plain_steps):
             s0 = time.perf_counter()
             ans_ref = scanner.scan_keys(m, keys[:2], ref_opts)
@@ -209,7 +237,7 @@ def main():
         ans, t = scanner.scan_keys(m, sw_keys, opts, with_timing=True)
         wall_q = time.perf_counter() - s0
         for i, tt in enumerate(sw_targets[:4]):
-            if not np.array_equal(ans[2 * i] ^ ans[2 * i + 1], m.download(tt, 1)[0]):
+            if not np.array_equal(ans[2 * i] ^ ans[2 * i + 1], want_row(tt)):
                 raise SystemExit(f"bit-exactness failure in the {qs}-key pass")
         by_keys[qs] = {"pass_ms": wall_q * 1e3, "kernel_ms": t["scan_kernel_ms"],
                        "queries_per_s": qs / KEYS_PER_QUERY / wall_q,
@@ -271,9 +299,11 @@ def main():
         "dtype": "u32",
         "data": "synthetic: splitmix64-seeded uniform rows generated in HBM, seeded DPF keygen, "
                 "uniform targets",
-        "config": {"workload": f"{rows_label} rows x {w} B, {q} DPF keys per pass over a "
-                               f"{d}-bit domain ({q // 2} two-party pairs)",
+        "config": {"workload": f"{rows_label} rows x {w} B + a {args.delta_rows}-row epoch "
+                               f"delta, {q} DPF keys per pass over a {d}-bit domain "
+                               f"({q // 2} two-party pairs)",
                    "rows": n_rows, "row_bytes": w, "domain_bits": d, "keys_per_pass": q,
+                   "delta_rows": args.delta_rows,
                    "kernel": timing["kernel"], "lane_depth": timing["lane_depth"],
                    "waves": timing["waves"], "use_graph": bool(args.use_graph)},
         "p50_ms": percentile(step_ms, 50),
