@@ -149,6 +149,23 @@ int mg_matrix_load_file(mg_device *dev, const char *path, uint32_t row_bytes, ui
  * the evaluator's limit d <= 32 (ChaChaParams::new, dpf.rs:65-78). */
 int mg_key_parse(const uint8_t *bytes, size_t len, mg_key_info *out);
 
+/* ChaChaKey::eval (dpf.rs:103-142) on the GPU: out[16 i .. 16 i + 16) = the
+ * key's 16-byte leaf (Seed128, four little-endian words) at indices[i], for
+ * i < n. Only the low d bits of an index are read, as in the reference. */
+int mg_key_eval(mg_device *dev, const uint8_t *key, size_t key_len,
+                const uint64_t *indices, uint64_t n, uint8_t *out);
+/* ChaChaKey::eval_subtree (dpf.rs:160-196) on the GPU: the leaves of
+ * [start, start + size) in order, 16 bytes each. size must be a power of two
+ * and start a multiple of it (MG_ERR_KEY_FORMAT, the reference's
+ * InvalidKeyFormat); a subtree larger than the 2^d-leaf domain is MG_ERR_ROWS
+ * (the reference writes a partial pattern there). */
+int mg_key_eval_subtree(mg_device *dev, const uint8_t *key, size_t key_len,
+                        uint64_t start, uint64_t size, uint8_t *out);
+/* ChaChaKey::full_eval (dpf.rs:147-158) on the GPU: all 2^d leaves; out_len
+ * must be exactly 2^d x 16 bytes (MG_ERR_INVALID_ARG, InvalidKeyLength). */
+int mg_key_full_eval(mg_device *dev, const uint8_t *key, size_t key_len, uint8_t *out,
+                     uint64_t out_len);
+
 /* generate_chacha_dpf_keys (dpf.rs:322-432), client side, host only: writes
  * the two parties' wire keys (19 + 18d + 16 bytes each) for a point function
  * at `target` over a d-bit domain. Root seeds are derived from `seed`

@@ -6,7 +6,8 @@
  *     (crates/morphogen-gpu-dpf/src/dpf.rs:60-99, 259-432)
  *   GpuScanner::{new,scan,scan_batch,scan_batch_optimized(_tiled)(_with_graph),
  *     scan_batch_single_query_multistream_optimized} (kernel.rs:158-171, 258,
- *     685-1162)
+ *     685-1162), and ChaChaKey::{eval,eval_subtree,full_eval} (dpf.rs:103-246)
+ *     as GpuScanner methods, since they run on the scanner's device
  *   GpuPageMatrix::{new,alloc_empty,update_pages,num_pages,size_bytes}
  *     (storage.rs:18-145)
  *   PirResult, KernelTiming (kernel.rs:1213-1240)
@@ -169,6 +170,31 @@ class GpuScanner {
     if (dev_) mg_device_close(dev_);
   }
   mg_device* raw() const { return dev_; }
+
+  using Seed128 = std::array<uint8_t, 16>;  // a leaf: four little-endian words
+  // ChaChaKey::eval (dpf.rs:103-142), on this device.
+  Seed128 eval(const ChaChaKey& k, uint64_t index) const {
+    Seed128 out{};
+    check(mg_key_eval(dev_, k.to_bytes().data(), k.to_bytes().size(), &index, 1, out.data()),
+          "ChaChaKey::eval");
+    return out;
+  }
+  // ChaChaKey::eval_subtree (dpf.rs:160-196): leaves [start, start + size).
+  std::vector<Seed128> eval_subtree(const ChaChaKey& k, uint64_t start, uint64_t size) const {
+    std::vector<Seed128> out(size);
+    check(mg_key_eval_subtree(dev_, k.to_bytes().data(), k.to_bytes().size(), start, size,
+                              out.empty() ? nullptr : out.front().data()),
+          "ChaChaKey::eval_subtree");
+    return out;
+  }
+  // ChaChaKey::full_eval (dpf.rs:147-158): all 2^d leaves.
+  std::vector<Seed128> full_eval(const ChaChaKey& k) const {
+    std::vector<Seed128> out(1ull << k.domain_bits());
+    check(mg_key_full_eval(dev_, k.to_bytes().data(), k.to_bytes().size(), out.front().data(),
+                           out.size() * 16),
+          "ChaChaKey::full_eval");
+    return out;
+  }
 
   // scan (kernel.rs:685-709): one query of three keys in one pass.
   PirResult scan(const GpuPageMatrix& db, const std::array<const ChaChaKey*, 3>& keys) const {

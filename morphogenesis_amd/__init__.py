@@ -154,6 +154,10 @@ def _load():
         "mg_keygen": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, u8p,
                                      u8p]),
         "mg_key_parse": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.POINTER(KeyInfo)]),
+        "mg_key_eval": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, u64p, ctypes.c_uint64, u8p]),
+        "mg_key_eval_subtree": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, ctypes.c_uint64,
+                                               ctypes.c_uint64, u8p]),
+        "mg_key_full_eval": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, u8p, ctypes.c_uint64]),
         "mg_scan": (ctypes.c_int, [vp, vp, u8p, szp, ctypes.c_uint32, u8p,
                                    ctypes.POINTER(ScanOpts), ctypes.POINTER(Timing)]),
         "mg_scan_per_query": (ctypes.c_int, [vp, vp, u8p, szp, ctypes.c_uint32, ctypes.c_uint32,
@@ -373,6 +377,31 @@ class GpuScanner:
                                           with_timing=True)
         return [PirResult([out[i * per + j].tobytes() for j in range(per)], [bytes(16)] * per, t)
                 for i in range(len(queries))]
+
+    def eval(self, key: bytes, indices) -> np.ndarray:
+        """ChaChaKey::eval (dpf.rs:103-142) at each index on the GPU: (n, 16) uint8."""
+        idx = np.ascontiguousarray(np.atleast_1d(np.asarray(indices, dtype=np.uint64)))
+        out = np.zeros((max(len(idx), 1), 16), dtype=np.uint8)
+        kb = _blob(key)
+        _check(lib.mg_key_eval(self._h, _u8(kb), len(key),
+                               idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(idx),
+                               _u8(out)))
+        return out[:len(idx)]
+
+    def eval_subtree(self, key: bytes, start: int, size: int) -> np.ndarray:
+        """ChaChaKey::eval_subtree (dpf.rs:160-196) on the GPU: (size, 16) uint8."""
+        out = np.zeros((max(size, 1), 16), dtype=np.uint8)
+        kb = _blob(key)
+        _check(lib.mg_key_eval_subtree(self._h, _u8(kb), len(key), start, size, _u8(out)))
+        return out[:size]
+
+    def full_eval(self, key: bytes) -> np.ndarray:
+        """ChaChaKey::full_eval (dpf.rs:147-158) on the GPU: (2^d, 16) uint8."""
+        d = parse_key(key).domain_bits
+        out = np.zeros((1 << d, 16), dtype=np.uint8)
+        kb = _blob(key)
+        _check(lib.mg_key_full_eval(self._h, _u8(kb), len(key), _u8(out), out.nbytes))
+        return out
 
     def scan_delta(self, delta: "DeltaBuffer", keys: Sequence[bytes]) -> np.ndarray:
         """scan_delta_for_gpu (scan.rs:434-485) for one leaf per row."""

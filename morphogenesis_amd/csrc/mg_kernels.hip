@@ -423,6 +423,67 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
   }
 }
 
+// -------------------------------------------------------- key evaluation --
+// ChaChaKey::eval (dpf.rs:103-142) at n indices: one lane per index, a full
+// d-level path walk each (only the index's low d bits are read).
+__global__ __launch_bounds__(256) void eval_points(const DevKey* __restrict__ key,
+                                                   const uint64_t* __restrict__ idx, uint64_t n,
+                                                   uint4* __restrict__ out) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    out[i] = point_eval(key, idx[i]);
+}
+
+// Leaves [start, start + n_lanes * 2^kl) of the key's tree in order
+// (ChaChaKey::eval_subtree / full_eval, dpf.rs:147-246), start aligned to
+// 2^kl. Lane j walks d - kl levels from the root to the root of leaves
+// [start + j*2^kl, +2^kl), then expands them depth-first with a per-lane LDS
+// stack (kl levels of right children, as in the scan) and writes each leaf's
+// 16 bytes in place.
+__global__ __launch_bounds__(256) void eval_range(const DevKey* __restrict__ key, uint64_t start,
+                                                  uint64_t n_lanes, int kl,
+                                                  uint4* __restrict__ out) {
+  extern __shared__ uint4 lds[];
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const int d = (int)key->domain_bits;
+  uint4* st_seed = lds + (size_t)wib * kl * 80;
+  uint32_t* st_t = reinterpret_cast<uint32_t*>(st_seed + kl * 64);
+  const int leaf_level = d - kl;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < n_lanes; j += stride) {
+    const uint64_t x0 = start + (j << kl);
+    Node node = key_root(key);
+    for (int level = 0; level < leaf_level; level++)
+      node = ggm_step(key, level, node, (uint32_t)(x0 >> (d - 1 - level)) & 1u);
+    uint4* o = out + (j << kl);
+    if (kl == 0) {
+      o[0] = leaf_mask(key, node);
+      continue;
+    }
+    const uint32_t n_pairs = 1u << (kl - 1);
+    int lvl = 0;
+    for (uint32_t pair = 0; pair < n_pairs; ++pair) {
+      if (pair) {
+        lvl = kl - 1 - __builtin_ctz(pair);
+        const uint4 sv = st_seed[lvl * 64 + lane];
+        node.s0 = sv.x; node.s1 = sv.y; node.s2 = sv.z; node.s3 = sv.w;
+        node.t = st_t[lvl * 64 + lane];
+      }
+      Node L, R;
+      for (; lvl < kl - 1; ++lvl) {
+        expand_node(key, leaf_level + lvl, node, L, R);
+        st_seed[(lvl + 1) * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);
+        st_t[(lvl + 1) * 64 + lane] = R.t;
+        node = L;
+      }
+      expand_node(key, leaf_level + kl - 1, node, L, R);
+      o[2 * pair] = leaf_mask(key, L);
+      o[2 * pair + 1] = leaf_mask(key, R);
+    }
+  }
+}
+
 // ------------------------------------------------------------ reductions --
 // out[key][c] (^)= XOR over parts of partial[part][key][c]. One block per
 // (key, chunk group of 4); 256 threads stride over parts.
@@ -685,6 +746,21 @@ hipError_t launch_xor_rows(uint4* rows, const uint64_t* idx, const uint4* diffs,
                            uint64_t n_entries, uint32_t chunks, hipStream_t s) {
   hipLaunchKernelGGL(xor_rows, dim3(grid_for(n_entries * chunks, 256)), dim3(256), 0, s, rows,
                      idx, diffs, n_entries, chunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_points(const DevKey* key, const uint64_t* idx, uint64_t n, uint4* out,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(eval_points, dim3(grid_for(n, 256)), dim3(256), 0, s, key, idx, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_range(const DevKey* key, uint64_t start, uint64_t n_lanes, int kl,
+                             uint4* out, hipStream_t s) {
+  if (kl < 0 || kl > 12) return hipErrorInvalidValue;
+  const size_t lds = (size_t)4 * kl * 80 * 16;
+  hipLaunchKernelGGL(eval_range, dim3(grid_for(n_lanes, 256)), dim3(256), lds, s, key, start,
+                     n_lanes, kl, out);
   return hipGetLastError();
 }
 

@@ -95,5 +95,12 @@ hipError_t launch_scatter_u4(uint4* dst, const uint4* src, const uint32_t* idx, 
                              uint32_t per, hipStream_t s);
 hipError_t launch_xor_rows(uint4* rows, const uint64_t* idx, const uint4* diffs,
                            uint64_t n_entries, uint32_t chunks, hipStream_t s);
+// out[i] = leaf of `key` at idx[i] (ChaChaKey::eval), i < n.
+hipError_t launch_eval_points(const DevKey* key, const uint64_t* idx, uint64_t n, uint4* out,
+                              hipStream_t s);
+// out[i] = leaf start + i, i < n_lanes * 2^kl; start aligned to 2^kl, kl <= 12
+// and kl <= d (ChaChaKey::eval_subtree).
+hipError_t launch_eval_range(const DevKey* key, uint64_t start, uint64_t n_lanes, int kl,
+                             uint4* out, hipStream_t s);
 
 }  // namespace mg

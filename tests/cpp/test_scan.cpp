@@ -166,6 +166,41 @@ void gpu_tests() {
   // fused_timing_is_recorded (kernel.rs:1495-1508).
   CHECK(all[0].timing.total_ns > 0 && all[0].timing.kernel_ns > 0, "timing recorded");
 
+  // ChaChaKey::{eval, eval_subtree, full_eval} on the device == the oracle.
+  std::printf("case key evaluation\n");
+  std::fflush(stdout);
+  {
+    const auto kp = generate_chacha_dpf_keys(ChaChaParams::create(12), 1234, 55);
+    orc_key ok;
+    const auto& w = kp.first.to_bytes();
+    CHECK(orc_key_from_bytes(w.data(), w.size(), &ok) == ORC_OK, "oracle key");
+    const auto full = scanner.full_eval(kp.first);
+    const auto full1 = scanner.full_eval(kp.second);
+    std::vector<uint32_t> ref(4u << 12);
+    CHECK(orc_full_eval(&ok, ref.data()) == ORC_OK, "oracle full_eval");
+    CHECK(full.size() == 4096 && std::memcmp(full.data(), ref.data(), 4096 * 16) == 0,
+          "full_eval != oracle");
+    for (size_t i = 0; i < full.size(); i++) {
+      bool all_ff = true, all_zero = true;
+      for (int b = 0; b < 16; b++) {
+        const uint8_t x = full[i][b] ^ full1[i][b];
+        all_ff &= x == 0xFF;
+        all_zero &= x == 0;
+      }
+      CHECK(i == 1234 ? all_ff : all_zero, "two-party XOR at %zu", i);
+    }
+    const auto sub = scanner.eval_subtree(kp.first, 1024, 512);
+    CHECK(std::memcmp(sub.data(), full.data() + 1024, 512 * 16) == 0, "subtree != slice");
+    for (uint64_t i : {0ull, 1ull, 1234ull, 4095ull}) {
+      uint32_t r[4];
+      orc_eval(&ok, i, r);
+      CHECK(std::memcmp(scanner.eval(kp.first, i).data(), r, 16) == 0, "eval(%llu)",
+            (unsigned long long)i);
+    }
+    CHECK(throws<InvalidKeyFormat>([&] { scanner.eval_subtree(kp.first, 8, 16); }),
+          "misaligned subtree");
+  }
+
   // Errors the reference reports (or panics on).
   std::printf("case errors\n");
   std::fflush(stdout);
