@@ -332,8 +332,31 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
     const uint32_t c = lane + 64 * j;
     lane_off[j] = c < C ? c : 0;
   }
+  // In a tile that lies wholly inside the matrix (every tile but a ragged
+  // last one) a batch's rows sit at fixed distances from its first row: row
+  // r0 + u (r0 even) is (u >> 1) lane subtrees of 2^kl rows and (u & 1) rows
+  // further on. Those byte offsets, with this lane's chunk, are computed once,
+  // so a batch costs one uniform base address and U x S plain loads.
+  uint32_t row_off[U][S];
+#pragma unroll
+  for (int u = 0; u < U; u++)
+#pragma unroll
+    for (int j = 0; j < S; j++)
+      row_off[u][j] =
+          (uint32_t)(((((uint64_t)(u >> 1) << kl) + (u & 1)) * C + lane_off[j]) * 16);
   uint4 va[U][S], vb[U][S];
-  auto load_batch = [&](uint64_t tile_base, uint32_t pair, uint32_t r0, uint4 (&v)[U][S]) {
+  auto load_batch = [&](uint64_t tile_base, uint32_t pair, uint32_t r0, uint4 (&v)[U][S],
+                        bool whole_tile) {
+    if (whole_tile) {
+      const char* base = reinterpret_cast<const char*>(
+          p.rows + (tile_base + ((uint64_t)(r0 >> 1) << kl) + 2 * (uint64_t)pair) * C);
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < S; j++)
+          v[u][j] = load_nt(reinterpret_cast<const uint4*>(base + row_off[u][j]));
+      return;
+    }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const uint32_t r = r0 + u;
@@ -356,14 +379,18 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
       }
     }
   };
-  auto prefetch_step = [&](uint64_t tile_base, uint32_t pair) {
-    load_batch(tile_base, pair, 0, va);
-    if (U < step_rows) load_batch(tile_base, pair, U, vb);
+  auto prefetch_step = [&](uint64_t tile_base, uint32_t pair, bool whole_tile) {
+    load_batch(tile_base, pair, 0, va, whole_tile);
+    if (U < step_rows) load_batch(tile_base, pair, U, vb, whole_tile);
   };
 
   for (uint64_t tile = t_begin; tile < t_end; ++tile) {
     const uint64_t tile_base = tile << (kl + lb);
-    prefetch_step(tile_base, 0);
+    // Every row a step of this tile reads exists, and batches never run past
+    // the step (step_rows is a power of two >= U).
+    const bool whole_tile =
+        step_rows >= (uint32_t)U && tile_base + (1ull << (kl + lb)) <= p.n_rows;
+    prefetch_step(tile_base, 0, whole_tile);
     Node node[G];
 #pragma unroll
     for (int g = 0; g < G; g++)
@@ -393,13 +420,13 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
       // va / vb already hold batches 0 and 1 of this step.
       for (uint32_t r0 = 0; r0 < step_rows; r0 += 2 * U) {
         fold_batch(r0, va);
-        if (r0 + 2 * U < step_rows) load_batch(tile_base, pair, r0 + 2 * U, va);
+        if (r0 + 2 * U < step_rows) load_batch(tile_base, pair, r0 + 2 * U, va, whole_tile);
         if (r0 + U >= step_rows) break;
         fold_batch(r0 + U, vb);
-        if (r0 + 3 * U < step_rows) load_batch(tile_base, pair, r0 + 3 * U, vb);
+        if (r0 + 3 * U < step_rows) load_batch(tile_base, pair, r0 + 3 * U, vb, whole_tile);
       }
       if (++pair == n_pairs) break;
-      prefetch_step(tile_base, pair);
+      prefetch_step(tile_base, pair, whole_tile);
       lvl = kl - 1 - __builtin_ctz(pair);
 #pragma unroll
       for (int g = 0; g < G; g++) {

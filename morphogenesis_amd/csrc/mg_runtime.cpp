@@ -234,8 +234,8 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   // Default lane depth (measured on MI355X): lane-DFS wants deep lane subtrees (kl up to 10)
   // as long as the keys still have at least ~4096 tiles of 2^(kl+6) rows between them to spread
   // over the resident waves. The stream kernel at 78,643,200 x 1 KiB: one key per wave runs best
-  // at kl = 6 (12.02 ms, 3840 waves; 12.18 ms at kl = 4), three keys per wave at kl = 7
-  // (15.29 ms against 15.60 ms at kl = 4).
+  // at kl = 4 (11.97 ms; 12.08 ms at kl = 6), two keys per wave at kl = 6 (13.04 ms; 13.14 ms
+  // at kl = 7), three at kl = 4 to 7 alike (15.02 to 15.13 ms).
   uint32_t kl_default;
   if (kernel == MG_KERNEL_LANE_DFS) {
     const uint64_t work = (pl->row_end - pl->row_begin) * (uint64_t)n_keys;
@@ -243,7 +243,7 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     while (kl_default < 10 && (work >> (kl_default + 7)) >= 4096) kl_default++;
   } else {
     const uint32_t g = o->keys_per_wave ? o->keys_per_wave : std::min<uint32_t>(3, n_keys);
-    kl_default = g <= 1 ? 6 : 7;
+    kl_default = g <= 1 ? 4 : (g == 2 ? 6 : 7);
   }
   split_levels(d, o->lane_depth ? o->lane_depth : kl_default, pl);
   const uint64_t tile_rows = 1ull << (pl->kl + pl->lb);
