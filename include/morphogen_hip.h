@@ -105,6 +105,8 @@ const char *mg_last_error(void);
 
 /* GpuScanner::new(device_ord) (kernel.rs:258). */
 int mg_device_open(int ordinal, mg_device **out);
+/* MG_ERR_STATE (nothing freed) while matrices, deltas or request loops
+ * opened on the device are still open. */
 int mg_device_close(mg_device *dev);
 /* Free/total HBM and compute-unit count of the device. */
 int mg_device_info(mg_device *dev, uint64_t *free_bytes, uint64_t *total_bytes,
@@ -129,6 +131,8 @@ int mg_matrix_info(const mg_matrix *m, uint64_t *n_rows, uint32_t *row_bytes);
 /* Device-to-device copy of all rows between two matrices of one shape (a
  * snapshot for rotation, epoch.rs:407-463 submit_snapshot). */
 int mg_matrix_copy(mg_matrix *dst, const mg_matrix *src);
+/* MG_ERR_STATE (nothing freed) while a request loop serves the matrix or
+ * holds it as a queued snapshot. */
 int mg_matrix_destroy(mg_matrix *m);
 
 /* Raw row file -> new device matrix (load_matrix_from_file,

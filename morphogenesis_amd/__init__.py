@@ -294,8 +294,10 @@ class GpuScanner:
         self.device_ord = device_ord
 
     def close(self):
+        """Releases the device; refused (MgError, MG_ERR_STATE) while matrices, deltas or
+        request loops opened on it are still open."""
         if self._h:
-            lib.mg_device_close(self._h)
+            _check(lib.mg_device_close(self._h))
             self._h = None
 
     def __del__(self):
@@ -482,8 +484,9 @@ class GpuPageMatrix:
         return self.n_rows * self.row_bytes
 
     def close(self):
+        """Frees the rows; refused (MgError, MG_ERR_STATE) while a request loop serves them."""
         if self._h:
-            lib.mg_matrix_destroy(self._h)
+            _check(lib.mg_matrix_destroy(self._h))
             self._h = None
 
     def __del__(self):

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -101,6 +102,9 @@ struct mg_device {
   mg::Plan graph_plan;
   hipGraphExec_t graph_exec = nullptr;
   std::map<std::pair<uint32_t, uint32_t>, int> occupancy;  // (variant, lds) -> blocks/CU
+  // Matrices, deltas and request loops still open on this device: closing
+  // the device first is refused.
+  std::atomic<int> children{0};
 };
 
 struct mg_matrix {
@@ -108,6 +112,9 @@ struct mg_matrix {
   uint4* rows = nullptr;
   uint64_t n_rows = 0;
   uint32_t row_bytes = 0;
+  // Request loops serving these rows (or holding them as a queued
+  // snapshot): destroying the matrix first is refused.
+  std::atomic<int> users{0};
 };
 
 struct mg_delta {

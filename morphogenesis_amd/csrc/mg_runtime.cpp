@@ -562,6 +562,8 @@ int mg_device_open(int ordinal, mg_device** out) {
 
 int mg_device_close(mg_device* dev) {
   if (!dev) return MG_OK;
+  if (const int n = dev->children.load())
+    return fail(MG_ERR_STATE, "%d matrices, deltas or request loops still use this device", n);
   (void)hipSetDevice(dev->ordinal);
   if (dev->stream) (void)hipStreamSynchronize(dev->stream);
   invalidate_graph(dev);
@@ -622,6 +624,7 @@ int mg_matrix_create(mg_device* dev, uint64_t n_rows, uint32_t row_bytes, mg_mat
     delete m;
     return fail(MG_ERR_DEVICE, "zeroing the matrix failed");
   }
+  dev->children++;
   *out = m;
   return MG_OK;
 }
@@ -701,6 +704,9 @@ int mg_matrix_copy(mg_matrix* dst, const mg_matrix* src) {
 
 int mg_matrix_destroy(mg_matrix* m) {
   if (!m) return MG_OK;
+  if (const int n = m->users.load())
+    return fail(MG_ERR_STATE, "%d request loops still serve this matrix", n);
+  m->dev->children--;
   std::lock_guard<std::mutex> lk(m->dev->mu);
   (void)mg::set_device(m->dev);
   (void)hipStreamSynchronize(m->dev->stream);
@@ -797,6 +803,7 @@ int mg_delta_create(mg_device* dev, uint32_t row_bytes, uint64_t capacity, mg_de
     delete dl;
     return mg::hip_fail(e, "hipMalloc(delta)");
   }
+  dev->children++;
   *out = dl;
   return MG_OK;
 }
@@ -839,6 +846,7 @@ int mg_delta_info(const mg_delta* dl, uint64_t* n_entries, uint64_t* epoch) {
 
 int mg_delta_destroy(mg_delta* dl) {
   if (!dl) return MG_OK;
+  dl->dev->children--;
   std::lock_guard<std::mutex> lk(dl->dev->mu);
   (void)mg::set_device(dl->dev);
   (void)hipStreamSynchronize(dl->dev->stream);

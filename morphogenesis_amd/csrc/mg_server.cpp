@@ -362,6 +362,7 @@ void apply_snapshot(mg_server* s, const Snapshot& sn) {
     s->base_rows[1] = spare;
     if (!spare) s->double_buffer = false;
   }
+  s->m->users--;
   s->m = sn.m;
   s->base_rows[0] = sn.m->rows;
   s->cur_base = 0;
@@ -583,6 +584,8 @@ int mg_server_create(mg_device* dev, mg_matrix* m, const mg_server_opts* opts, m
   s->stats.max_active_keys = (uint32_t)cap;
   s->stats.segments = s->segments;
   s->stats.n_rows = s->n_rows;
+  m->users++;
+  dev->children++;
   s->thread = std::thread(server_loop, s.get());
   *out = s.release();
   return MG_OK;
@@ -690,13 +693,17 @@ int mg_server_snapshot(mg_server* s, mg_matrix* m, uint64_t epoch) {
   sn.epoch = epoch;
   std::unique_lock<std::mutex> lk(s->mu);
   if (s->fatal) return fail(s->fatal, "server stopped: %s", s->fatal_msg.c_str());
+  m->users++;  // held from the queue on; the swap releases the old matrix
   Event ev;
   ev.snap = &sn;
   s->queue.push_back(std::move(ev));
   s->queued_n_rows = m->n_rows;
   s->cv_work.notify_one();
   s->cv_done.wait(lk, [&] { return sn.done; });
-  if (sn.status != MG_OK) return fail(sn.status, "snapshot failed: %s", sn.error.c_str());
+  if (sn.status != MG_OK) {
+    m->users--;
+    return fail(sn.status, "snapshot failed: %s", sn.error.c_str());
+  }
   return MG_OK;
 }
 
@@ -743,6 +750,8 @@ int mg_server_destroy(mg_server* s) {
                       hipMemcpyDeviceToDevice);
     free_server_memory(s);
   }
+  s->m->users--;
+  s->dev->children--;
   delete s;
   return MG_OK;
 }

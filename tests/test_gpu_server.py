@@ -171,3 +171,27 @@ def test_wide_rows_through_the_server(mg, scanner):
     assert np.array_equal(ans, oracle.scan_tiled(keys, rows, THREADS))
     srv.close()
     m.close()
+
+
+def test_lifetimes_are_enforced(mg):
+    # A loop's matrix cannot be freed under it, nor the device under its objects.
+    scanner = mg.GpuScanner(0)
+    m = mg.GpuPageMatrix(scanner, 1 << 12, 32)
+    spare = mg.GpuPageMatrix(scanner, 1 << 12, 32)
+    srv = mg.GpuServer(scanner, m, segments=2, lane_depth=2, max_active_keys=6)
+    with pytest.raises(mg.MgError) as e:
+        m.close()
+    assert e.value.code == mg.MG_ERR_STATE
+    with pytest.raises(mg.MgError) as e:
+        scanner.close()
+    assert e.value.code == mg.MG_ERR_STATE
+    srv.submit_snapshot(spare, 1)      # the loop now serves `spare`: `m` is free
+    m.close()
+    with pytest.raises(mg.MgError):
+        spare.close()
+    keys = [oracle.keygen_bytes(12, 7, seed=1)[0]]
+    ans, info = srv.wait(srv.submit(keys), 1)
+    assert info["epoch"] == 1 and np.array_equal(ans, scanner.scan_keys(spare, keys))
+    srv.close()
+    spare.close()
+    scanner.close()
