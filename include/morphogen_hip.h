@@ -299,6 +299,9 @@ int mg_server_submit(mg_server *s, const uint8_t *keys, const size_t *key_lens,
  * MG_ERR_BUSY on timeout (the request stays queued). */
 int mg_server_wait(mg_server *s, uint64_t ticket, uint8_t *out,
                    uint32_t timeout_ms, mg_request_info *info);
+/* Gives up a ticket nobody will wait for (e.g. after a timed-out wait): the
+ * request still runs, and its answers are dropped when it completes. */
+int mg_server_abandon(mg_server *s, uint64_t ticket);
 /* Epoch update: rows[row_idx[i]] ^= diffs[i] for every request admitted
  * after this call (EpochManager::try_advance, epoch.rs:465-549). */
 int mg_server_update(mg_server *s, const uint64_t *row_idx,
@@ -341,8 +344,8 @@ typedef struct {
   uint64_t max_body_bytes;  /* 0 = MG_WIRE_MAX_BODY                         */
   uint32_t max_batch;       /* 0 = MG_WIRE_MAX_BATCH                        */
   uint32_t timeout_ms;      /* wait for the loop; 0 = forever. A request that
-                               times out (503) still runs to completion and
-                               its answers are discarded with the loop.     */
+                               times out (503) still runs to completion; its
+                               answers are dropped then (mg_server_abandon). */
 } mg_wire_opts;
 
 /* Output bytes a response of n_queries queries of row_bytes pages needs. */

@@ -178,6 +178,7 @@ def _load():
         "mg_server_update": (ctypes.c_int, [vp, u64p, u8p, ctypes.c_uint64, ctypes.c_uint64]),
         "mg_server_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
         "mg_server_snapshot": (ctypes.c_int, [vp, vp, ctypes.c_uint64]),
+        "mg_server_abandon": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "mg_server_destroy": (ctypes.c_int, [vp]),
         "mg_cuckoo_hash_indices": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint64, u64p,
                                                   u64p]),
@@ -560,6 +561,10 @@ class GpuServer:
         info = RequestInfo()
         _check(lib.mg_server_wait(self._h, ticket, _u8(out), timeout_ms, ctypes.byref(info)))
         return out[:n_keys], info.as_dict()
+
+    def abandon(self, ticket: int):
+        """Gives up a ticket: the request still runs; its answers are dropped when done."""
+        _check(lib.mg_server_abandon(self._h, ticket))
 
     def update(self, row_idx: np.ndarray, diffs: np.ndarray, epoch: int):
         idx = np.ascontiguousarray(row_idx, dtype=np.uint64)

@@ -46,6 +46,7 @@ struct Request {
   int status = MG_OK;
   std::string error;
   bool done = false;
+  bool abandoned = false;  // nobody will collect it: dropped once done
 };
 
 struct Update {
@@ -490,6 +491,7 @@ void server_loop(mg_server* s) {
           r->t_done = t1;
           s->stats.completed++;
           s->stats.keys_completed += r->n_keys;
+          if (r->abandoned) s->requests.erase(r->ticket);  // `r` is gone from here on
         }
         sl = Slot{};
         s->free_slots.push_back(finished[j]);
@@ -651,6 +653,19 @@ int mg_server_wait(mg_server* s, uint64_t ticket, uint8_t* out, uint32_t timeout
     info->n_keys = r->n_keys;
   }
   s->requests.erase(it);
+  return MG_OK;
+}
+
+int mg_server_abandon(mg_server* s, uint64_t ticket) {
+  if (!s) return fail(MG_ERR_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->requests.find(ticket);
+  if (it == s->requests.end())
+    return fail(MG_ERR_INVALID_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+  if (it->second->done)
+    s->requests.erase(it);
+  else
+    it->second->abandoned = true;
   return MG_OK;
 }
 

@@ -583,6 +583,7 @@ extern "C" int mg_wire_handle(mg_server* s, const mg_wire_opts* opts, const char
   mg_request_info info;
   rc = mg_server_wait(s, ticket, pages.data(), opts ? opts->timeout_ms : 0, &info);
   if (rc) {
+    if (rc == MG_ERR_BUSY) (void)mg_server_abandon(s, ticket);  // timed out: nobody collects it
     *http_status = rc == MG_ERR_BUSY ? 503 : 500;
     return MG_OK;
   }

@@ -39,4 +39,5 @@ int mg_server_submit(mg_server*, const uint8_t*, const size_t*, uint32_t, uint64
 int mg_server_wait(mg_server*, uint64_t, uint8_t*, uint32_t, mg_request_info*) {
   return MG_ERR_STATE;
 }
+int mg_server_abandon(mg_server*, uint64_t) { return MG_ERR_STATE; }
 }

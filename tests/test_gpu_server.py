@@ -195,3 +195,29 @@ def test_lifetimes_are_enforced(mg):
     srv.close()
     spare.close()
     scanner.close()
+
+
+def test_abandoned_requests_are_dropped(mg, scanner):
+    n, d = 1 << 12, 12
+    m = mg.GpuPageMatrix(scanner, n, 32)
+    m.fill_seeded(4)
+    srv = mg.GpuServer(scanner, m, segments=4, lane_depth=2)
+    keys = [oracle.keygen_bytes(d, t, seed=t)[0] for t in (1, 2, 3)]
+    t_gone = srv.submit(keys)
+    srv.abandon(t_gone)                  # still running; its answers will be dropped
+    t_kept = srv.submit(keys)
+    ans, _ = srv.wait(t_kept, 3)
+    assert np.array_equal(ans, scanner.scan_keys(m, keys))
+    deadline = time.time() + 30
+    while srv.stats()["completed"] < 2 and time.time() < deadline:
+        time.sleep(0.01)
+    assert srv.stats()["completed"] == 2
+    with pytest.raises(mg.MgError):
+        srv.wait(t_gone, 3, timeout_ms=100)  # no longer known
+    t_done = srv.submit(keys)
+    time.sleep(0.2)
+    srv.abandon(t_done)                  # done or not, the ticket goes away
+    with pytest.raises(mg.MgError):
+        srv.abandon(t_done + 1000)       # unknown ticket
+    srv.close()
+    m.close()
