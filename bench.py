@@ -319,7 +319,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    # Objects go before the device that holds them (closing it first is refused).
+    if delta is not None:
+        delta.close()
     m.close()
     scanner.close()
 
