@@ -87,7 +87,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rows-log2", type=int, default=31)
     ap.add_argument("--rows", type=int, default=0, help="explicit row count (overrides --rows-log2)")
-    ap.add_argument("--kernel", choices=["auto", "lane_dfs", "stream", "point", "suffix"],
+    ap.add_argument("--kernel", choices=["auto", "lane_dfs", "stream", "point", "suffix", "split"],
                     default="auto")
     ap.add_argument("--ref-plain-steps", type=int, default=1,
                     help="passes of the reference-shaped per-row suffix-walk kernel, 3 keys each")
@@ -175,7 +175,7 @@ def main():
     target_rows = [want_row(t) for t in targets]
     kernel = {"auto": mg.KERNEL_AUTO, "lane_dfs": mg.KERNEL_LANE_DFS,
               "stream": mg.KERNEL_STREAM, "point": mg.KERNEL_POINT,
-              "suffix": mg.KERNEL_SUFFIX}[args.kernel]
+              "suffix": mg.KERNEL_SUFFIX, "split": mg.KERNEL_STREAM_SPLIT}[args.kernel]
     opts = mg.make_opts(kernel=kernel, lane_depth=args.lane_depth,
                         keys_per_wave=args.keys_per_wave, waves_per_key=args.waves_per_key,
                         waves_per_block=args.waves_per_block, use_graph=args.use_graph,

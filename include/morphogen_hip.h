@@ -77,10 +77,13 @@ enum {
   MG_KERNEL_LANE_DFS = 1, /* per-lane subtree DFS, rows of <= 128 B       */
   MG_KERNEL_STREAM = 2,   /* LDS-staged masks, coalesced wide rows         */
   MG_KERNEL_POINT = 3,    /* one full path walk per row                    */
-  MG_KERNEL_SUFFIX = 4    /* comparison baseline shaped like the reference
+  MG_KERNEL_SUFFIX = 4,   /* comparison baseline shaped like the reference
                              kernel (fused_kernel.cu:309-360): 2048-row
                              tiles, one prefix walk per tile, an 11-level
                              walk per row; rows of <= 128 B               */
+  MG_KERNEL_STREAM_SPLIT = 5 /* the stream kernel with each block split
+                             into a DFS wave and a row-streaming wave;
+                             rows of >= 16 B and <= 4096 B, d >= kl + 6   */
 };
 
 /* Scan options; zero-initialise for defaults. The tuning fields replace the

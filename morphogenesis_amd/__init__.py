@@ -30,9 +30,10 @@ MG_ERR_NO_DEVICE = 7
 MG_ERR_BUSY = 8
 MG_ERR_STATE = 9
 
-KERNEL_AUTO, KERNEL_LANE_DFS, KERNEL_STREAM, KERNEL_POINT, KERNEL_SUFFIX = range(5)
+(KERNEL_AUTO, KERNEL_LANE_DFS, KERNEL_STREAM, KERNEL_POINT, KERNEL_SUFFIX,
+ KERNEL_STREAM_SPLIT) = range(6)
 KERNEL_NAMES = {KERNEL_LANE_DFS: "lane_dfs", KERNEL_STREAM: "stream", KERNEL_POINT: "point",
-                KERNEL_SUFFIX: "suffix"}
+                KERNEL_SUFFIX: "suffix", KERNEL_STREAM_SPLIT: "stream_split"}
 
 
 class MgError(RuntimeError):

@@ -21,8 +21,8 @@ import time
 
 import numpy as np
 
-from . import (KERNEL_LANE_DFS, KERNEL_STREAM, GpuPageMatrix, GpuScanner, generate_keys,
-               make_opts)
+from . import (KERNEL_LANE_DFS, KERNEL_STREAM, KERNEL_STREAM_SPLIT, GpuPageMatrix, GpuScanner,
+               generate_keys, make_opts)
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 
@@ -63,6 +63,11 @@ def candidates(row_bytes: int, n_keys: int):
                 continue
             out.append(dict(kernel=KERNEL_STREAM, lane_depth=kl, keys_per_wave=g,
                             waves_per_block=wpb))
+        for kl, g in itertools.product((4, 5, 6, 7, 8), range(1, gmax + 1)):
+            if g > n_keys:
+                continue
+            out.append(dict(kernel=KERNEL_STREAM_SPLIT, lane_depth=kl, keys_per_wave=g,
+                            waves_per_block=2))
     return out
 
 
@@ -103,7 +108,7 @@ def main():
     ap.add_argument("--domain-bits", type=int, default=0)
     ap.add_argument("--keys", type=int, default=3)
     ap.add_argument("--reps", type=int, default=2)
-    ap.add_argument("--kernel", choices=["all", "lane_dfs", "stream"], default="all")
+    ap.add_argument("--kernel", choices=["all", "lane_dfs", "stream", "split"], default="all")
     ap.add_argument("--save", action="store_true", help="store the winner in tuned.json")
     ap.add_argument("--config", default="",
                     help="run one setting only: kernel,lane_depth,keys_per_wave,waves_per_block")
@@ -119,7 +124,8 @@ def main():
             for i, t in enumerate(rng.integers(0, n_rows, size=args.keys))]
     grid = candidates(args.row_bytes, args.keys)
     if args.kernel != "all":
-        kid = KERNEL_LANE_DFS if args.kernel == "lane_dfs" else KERNEL_STREAM
+        kid = {"lane_dfs": KERNEL_LANE_DFS, "stream": KERNEL_STREAM,
+               "split": KERNEL_STREAM_SPLIT}[args.kernel]
         grid = [c for c in grid if c["kernel"] == kid]
     if args.config:
         k, kl, g, wpb = (int(x) for x in args.config.split(","))

@@ -450,6 +450,186 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
   }
 }
 
+// ------------------------------------------- split producer/consumer scan --
+// Wide rows, several keys per pass. The two waves of a block split the work:
+// wave 0 runs the keys' lane DFS exactly as scan_stream does and publishes
+// each leaf-pair step's 2 x 64 masks per key into one of two LDS mask
+// buffers; wave 1 only streams the step's 128 rows (NB batches of U rows in
+// flight, including the next step's first batches while it waits) and folds
+// them. A block barrier per step hands a buffer over, so wave 0 runs at most
+// one step ahead and never overwrites a buffer wave 1 still reads. Both waves
+// walk the same step sequence of the block's tile range and meet at every
+// barrier. Needs all 64 lanes live (lb = 6), so a step is always 128 rows.
+constexpr int kSplitNB = 4;
+
+template <int S, int G>
+__global__ __launch_bounds__(128) void scan_split(StreamParams p) {
+  extern __shared__ uint4 lds[];
+  const int lane = threadIdx.x & 63;
+  const bool producer = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0;
+  const uint32_t n_groups = (p.n_keys + G - 1) / G;
+  const uint32_t total = p.parts * n_groups;
+  const uint32_t per = (gridDim.x + kXcds - 1) / kXcds;
+  const uint32_t blk = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;  // XCD-aware
+  if (blk >= total) return;  // the whole block
+  const uint32_t grp = blk % n_groups;
+  const uint32_t part = blk / n_groups;
+  const uint32_t key0 = grp * G;
+  const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
+  const uint32_t C = p.chunks;
+  const uint64_t t_begin = p.tile_begin + (uint64_t)part * p.tiles_per_wave;
+  uint64_t t_end = t_begin + p.tiles_per_wave;
+  if (t_end > p.tile_end) t_end = p.tile_end;
+  const uint32_t n_pairs = 1u << (kl - 1);
+  const uint64_t n_steps = t_end > t_begin ? (t_end - t_begin) * n_pairs : 0;
+  // Right children wait at depths 1 .. kl-1 (depth 0 is the lane root,
+  // never pushed), so a key's stack has kl - 1 slots of 64 lanes.
+  const int slots = kl - 1;
+  uint4* stacks = lds;                              // [G][slots][80]
+  uint4* masks = lds + (size_t)G * slots * 80;      // [2][G][128]
+
+  if (producer) {
+    const DevKey* kp[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const uint32_t ki = key0 + g < p.n_keys ? key0 + g : key0;
+      kp[g] = p.keys + (p.key_idx ? p.key_idx[ki] : ki);
+    }
+    LanePath path[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const Node r = key_root(kp[g]);
+      path[g].s0 = r.s0; path[g].s1 = r.s1; path[g].s2 = r.s2;
+      path[g].s3 = r.s3; path[g].t = r.t;
+    }
+    const int leaf_level = d - kl;
+    uint64_t s = 0;
+    for (uint64_t tile = t_begin; tile < t_end; ++tile) {
+      Node node[G];
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        node[g] = tile_lane_root(kp[g], path[g], lane, tile, tile - 1, tile == t_begin, tb, lb);
+      int lvl = 0;
+      for (uint32_t pair = 0; pair < n_pairs; ++pair, ++s) {
+        uint4* mbuf = masks + (size_t)(s & 1) * G * 128;
+        if (pair) lvl = kl - 1 - __builtin_ctz(pair);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+          uint4* st_seed = stacks + (size_t)g * slots * 80;
+          uint32_t* st_t = reinterpret_cast<uint32_t*>(st_seed + slots * 64);
+          if (pair) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
+            const uint4 sv = st_seed[(lvl - 1) * 64 + lane];
+            node[g].s0 = sv.x; node[g].s1 = sv.y; node[g].s2 = sv.z; node[g].s3 = sv.w;
+            node[g].t = st_t[(lvl - 1) * 64 + lane];
+          }
+          Node L, R;
+          for (int l = lvl; l < kl - 1; ++l) {
+            expand_node(kp[g], leaf_level + l, node[g], L, R);
+            st_seed[l * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth l + 1
+            st_t[l * 64 + lane] = R.t;
+            node[g] = L;
+          }
+          expand_node(kp[g], leaf_level + kl - 1, node[g], L, R);
+          mbuf[g * 128 + 2 * lane] = leaf_mask(kp[g], L);
+          mbuf[g * 128 + 2 * lane + 1] = leaf_mask(kp[g], R);
+        }
+        __syncthreads();  // step s's masks are published
+      }
+    }
+    return;
+  }
+
+  // Consumer: rows of step s are tile_base + j*2^kl + 2*pair + {0,1} for
+  // source lane j < 64; row r of the step uses mask slot r.
+  constexpr int U = 8 / S;
+  constexpr int NB = kSplitNB;
+  const uint32_t n_batches = 128 / U;  // a multiple of NB
+  uint32_t lane_off[S];
+#pragma unroll
+  for (int j = 0; j < S; j++) {
+    const uint32_t c = lane + 64 * j;
+    lane_off[j] = c < C ? c : 0;
+  }
+  uint32_t row_off[U][S];
+#pragma unroll
+  for (int u = 0; u < U; u++)
+#pragma unroll
+    for (int j = 0; j < S; j++)
+      row_off[u][j] =
+          (uint32_t)(((((uint64_t)(u >> 1) << kl) + (u & 1)) * C + lane_off[j]) * 16);
+  auto load = [&](uint64_t s, uint32_t b, uint4 (&v)[U][S]) {
+    const uint64_t tile_base = (t_begin + (s >> (kl - 1))) << (kl + lb);
+    const uint32_t pair = (uint32_t)(s & (n_pairs - 1));
+    const uint32_t r0 = b * U;
+    if (tile_base + (1ull << (kl + lb)) <= p.n_rows) {
+      const char* base = reinterpret_cast<const char*>(
+          p.rows + (tile_base + ((uint64_t)(r0 >> 1) << kl) + 2 * (uint64_t)pair) * C);
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < S; j++)
+          v[u][j] = load_nt(reinterpret_cast<const uint4*>(base + row_off[u][j]));
+      return;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t r = r0 + u;
+      const uint64_t x = tile_base + ((uint64_t)(r >> 1) << kl) + 2 * (uint64_t)pair + (r & 1);
+      const uint4* src = x < p.n_rows ? p.rows + x * C : p.zero_rows;
+#pragma unroll
+      for (int j = 0; j < S; j++) v[u][j] = load_nt(src + lane_off[j]);
+    }
+  };
+
+  uint4 acc[G][S];
+#pragma unroll
+  for (int g = 0; g < G; g++)
+#pragma unroll
+    for (int j = 0; j < S; j++) acc[g][j] = make_uint4(0, 0, 0, 0);
+  uint4 ring[NB][U][S];
+  if (n_steps) {
+#pragma unroll
+    for (int k = 0; k < NB; k++) load(0, (uint32_t)k, ring[k]);
+  }
+  for (uint64_t s = 0; s < n_steps; ++s) {
+    __syncthreads();  // step s's masks are in buffer s & 1
+    const uint4* mbuf = masks + (size_t)(s & 1) * G * 128;
+    for (uint32_t b0 = 0; b0 < n_batches; b0 += NB) {
+#pragma unroll
+      for (int k = 0; k < NB; k++) {
+        const uint32_t b = b0 + (uint32_t)k;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int g = 0; g < G; g++) {
+            const uint4 m = mbuf[g * 128 + b * U + u];
+#pragma unroll
+            for (int j = 0; j < S; j++) xor_masked3(acc[g][j], ring[k][u][j], m);
+          }
+        // Batch b + NB goes into the freed slot: later in this step, or the
+        // start of the next one (batch indices stay congruent mod NB).
+        uint32_t nb = b + NB;
+        uint64_t ns = s;
+        if (nb >= n_batches) {
+          nb -= n_batches;
+          ++ns;
+        }
+        if (ns < n_steps) load(ns, nb, ring[k]);
+      }
+    }
+  }
+
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    if (key0 + g >= p.n_keys) continue;
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+      const uint32_t c = lane + 64 * j;
+      if (c < C) p.partial[((size_t)part * p.n_keys + key0 + g) * C + c] = acc[g][j];
+    }
+  }
+}
+
 // -------------------------------------------------------- key evaluation --
 // ChaChaKey::eval (dpf.rs:103-142) at n indices: one lane per index, a full
 // d-level path walk each (only the index's low d bits are read).
@@ -674,6 +854,39 @@ hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s) {
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_split(int S, int G, const StreamParams& p, hipStream_t s) {
+  const uint32_t n_groups = (p.n_keys + G - 1) / G;
+  const uint32_t total = p.parts * n_groups;
+  const dim3 block(128);
+  const dim3 grid((total + kXcds - 1) / kXcds * kXcds);
+  const size_t lds = split_lds_bytes(G, p.kl);
+#define MG_SG(SS, GG)                                                         \
+  if (S == SS && G == GG) {                                                   \
+    hipLaunchKernelGGL((scan_split<SS, GG>), grid, block, lds, s, p);         \
+    return hipGetLastError();                                                 \
+  }
+  MG_SG(1, 1) MG_SG(1, 2) MG_SG(1, 3) MG_SG(1, 4)
+  MG_SG(2, 1) MG_SG(2, 2) MG_SG(2, 3)
+  MG_SG(4, 1) MG_SG(4, 2)
+#undef MG_SG
+  return hipErrorInvalidValue;
+}
+
+size_t split_lds_bytes(int G, uint32_t kl) {
+  return ((size_t)G * (kl - 1) * 80 + (size_t)2 * G * 128) * sizeof(uint4);
+}
+
+hipError_t split_blocks_per_cu(int S, int G, size_t lds, int* blocks) {
+#define MG_SG(SS, GG)                                                                       \
+  if (S == SS && G == GG)                                                                   \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_split<SS, GG>, 128, lds);
+  MG_SG(1, 1) MG_SG(1, 2) MG_SG(1, 3) MG_SG(1, 4)
+  MG_SG(2, 1) MG_SG(2, 2) MG_SG(2, 3)
+  MG_SG(4, 1) MG_SG(4, 2)
+#undef MG_SG
+  return hipErrorInvalidValue;
 }
 
 size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block) {

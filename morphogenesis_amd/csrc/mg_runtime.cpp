@@ -151,6 +151,8 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int chunks, int 
   } else {
     if (kind == MG_KERNEL_LANE_DFS * 100 + (uint32_t)chunks)
       MG_HIP(lane_dfs_blocks_per_cu(chunks, 64 * waves_per_block, lds, &blocks));
+    else if (kind / 100 == MG_KERNEL_STREAM_SPLIT)
+      MG_HIP(split_blocks_per_cu(S, G, lds, &blocks));
     else
       MG_HIP(stream_blocks_per_cu(S, G, 64 * waves_per_block, lds, &blocks));
     dev->occupancy[key] = blocks;
@@ -196,7 +198,8 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   if (o->lane_depth > 16) return fail(MG_ERR_INVALID_ARG, "lane_depth must be 0-16");
 
   uint32_t kernel = o->kernel;
-  if (kernel > MG_KERNEL_SUFFIX) return fail(MG_ERR_INVALID_ARG, "unknown kernel %u", kernel);
+  if (kernel > MG_KERNEL_STREAM_SPLIT)
+    return fail(MG_ERR_INVALID_ARG, "unknown kernel %u", kernel);
   if (kernel == MG_KERNEL_AUTO) {
     if (d == 0) kernel = MG_KERNEL_POINT;
     else if (pl->chunks <= 8) kernel = MG_KERNEL_LANE_DFS;
@@ -207,8 +210,8 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     return fail(MG_ERR_INVALID_ARG, "a 0-bit domain needs the point kernel");
   if (kernel == MG_KERNEL_LANE_DFS && pl->chunks > 8)
     return fail(MG_ERR_INVALID_ARG, "lane-DFS kernel takes rows of at most 128 bytes");
-  if (kernel == MG_KERNEL_STREAM && pl->chunks > 256)
-    return fail(MG_ERR_INVALID_ARG, "stream kernel takes rows of at most 4096 bytes");
+  if ((kernel == MG_KERNEL_STREAM || kernel == MG_KERNEL_STREAM_SPLIT) && pl->chunks > 256)
+    return fail(MG_ERR_INVALID_ARG, "stream kernels take rows of at most 4096 bytes");
   if (kernel == MG_KERNEL_SUFFIX && (pl->chunks > 8 || d == 0))
     return fail(MG_ERR_INVALID_ARG, "suffix-walk kernel takes rows of at most 128 bytes, d >= 1");
   pl->kernel = kernel;
@@ -241,11 +244,18 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     const uint64_t work = (pl->row_end - pl->row_begin) * (uint64_t)n_keys;
     kl_default = 6;
     while (kl_default < 10 && (work >> (kl_default + 7)) >= 4096) kl_default++;
+  } else if (kernel == MG_KERNEL_STREAM_SPLIT) {
+    // Split kernel, same matrix, three blocks per CU: kl = 6 gives 11.70 / 11.93 / 13.02 /
+    // 14.92 ms for one to four keys per group (kl = 7: 11.42 / 11.94 / 13.21 / 15.08 ms).
+    kl_default = 6;
   } else {
     const uint32_t g = o->keys_per_wave ? o->keys_per_wave : std::min<uint32_t>(3, n_keys);
     kl_default = g <= 1 ? 4 : (g == 2 ? 6 : 7);
   }
   split_levels(d, o->lane_depth ? o->lane_depth : kl_default, pl);
+  if (kernel == MG_KERNEL_STREAM_SPLIT && pl->lb != 6)
+    return fail(MG_ERR_INVALID_ARG,
+                "the split stream kernel needs d >= kl + 6 (d = %u, kl = %u)", d, pl->kl);
   const uint64_t tile_rows = 1ull << (pl->kl + pl->lb);
   if (pl->row_begin % tile_rows != 0)
     return fail(MG_ERR_ROWS, "row_begin %llu is not a multiple of the %llu-row tile",
@@ -271,6 +281,14 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   } else {
     const int S = stream_s(pl->chunks);
     int G = o->keys_per_wave ? (int)o->keys_per_wave : std::min<int>(3, (int)n_keys);
+    if (!o->keys_per_wave && kernel == MG_KERNEL_STREAM_SPLIT) {
+      // The split kernel's cost per key falls up to the widest group (1 KiB rows: 11.7 ms for
+      // one key, 14.9 ms for four); spread the keys evenly over the fewest groups.
+      const int gmax = max_keys_per_wave(S);
+      const int nk = std::max(1, (int)n_keys);
+      const int ng = (nk + gmax - 1) / gmax;
+      G = (nk + ng - 1) / ng;
+    }
     G = std::max(1, std::min(G, max_keys_per_wave(S)));
     pl->keys_per_wave = (uint32_t)G;
     groups = (n_keys + G - 1) / G;
@@ -281,6 +299,22 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
                   dev->max_lds);
     rc = resident_waves(dev, MG_KERNEL_STREAM * 100 + (uint32_t)(S * 8 + G), S, G,
                         (int)pl->chunks, (int)pl->waves_per_block, lds, &resident);
+  }
+  if (!rc && kernel == MG_KERNEL_STREAM_SPLIT) {
+    // One block = one DFS wave + one streaming wave; parts count blocks.
+    const int S = stream_s(pl->chunks);
+    const int G = (int)pl->keys_per_wave;
+    pl->waves_per_block = 2;
+    const size_t lds = split_lds_bytes(G, pl->kl);
+    if (lds > dev->max_lds)
+      return fail(MG_ERR_INVALID_ARG, "split plan needs %zu B of LDS per block (max %zu)", lds,
+                  dev->max_lds);
+    int res_waves = 0;
+    rc = resident_waves(dev, MG_KERNEL_STREAM_SPLIT * 100 + (uint32_t)(S * 8 + G), S, G,
+                        (int)pl->chunks, 2, lds, &res_waves);
+    // Three blocks per CU (768 on 256 CUs) stream fastest; a fourth block on some CUs
+    // unbalances them (four keys per group: 14.9 ms at 768 blocks, 26.5 ms at 832).
+    resident = std::min(res_waves / 2, 3 * dev->cus);  // blocks
   }
   if (rc) return rc;
   uint64_t parts = o->waves_per_key;
@@ -318,7 +352,7 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       p.parts = pl.parts;
       p.partial = b.partial;
       MG_HIP(launch_lane_dfs((int)pl.chunks, p, (int)pl.waves_per_block, s));
-    } else if (pl.kernel == MG_KERNEL_STREAM) {
+    } else if (pl.kernel == MG_KERNEL_STREAM || pl.kernel == MG_KERNEL_STREAM_SPLIT) {
       StreamParams p{};
       p.rows = (const uint4*)pl.rows;
       p.n_rows = pl.row_end;
@@ -333,8 +367,11 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       p.parts = pl.parts;
       p.partial = b.partial;
       p.zero_rows = (const uint4*)dev->zero_rows.p;
-      MG_HIP(launch_stream(stream_s(pl.chunks), (int)pl.keys_per_wave, p,
-                           (int)pl.waves_per_block, s));
+      if (pl.kernel == MG_KERNEL_STREAM_SPLIT)
+        MG_HIP(launch_split(stream_s(pl.chunks), (int)pl.keys_per_wave, p, s));
+      else
+        MG_HIP(launch_stream(stream_s(pl.chunks), (int)pl.keys_per_wave, p,
+                             (int)pl.waves_per_block, s));
     } else if (pl.kernel == MG_KERNEL_SUFFIX) {
       SuffixParams p{};
       p.rows = (const uint4*)pl.rows;
@@ -491,12 +528,13 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
     t->total_ms = mg::now_ms() - t0;
     t->kernel = pl.kernel;
     t->lane_depth = pl.kl;
-    t->keys_per_wave = pl.kernel == MG_KERNEL_STREAM ? pl.keys_per_wave : 1;
+    const bool stream = pl.kernel == MG_KERNEL_STREAM || pl.kernel == MG_KERNEL_STREAM_SPLIT;
+    t->keys_per_wave = stream ? pl.keys_per_wave : 1;
     t->waves = (pl.kernel == MG_KERNEL_POINT || pl.kernel == MG_KERNEL_SUFFIX)
                    ? pl.parts * n_keys
-                   : pl.parts * (pl.kernel == MG_KERNEL_STREAM
-                                     ? (n_keys + pl.keys_per_wave - 1) / pl.keys_per_wave
-                                     : n_keys);
+                   : pl.parts * (stream ? (n_keys + pl.keys_per_wave - 1) / pl.keys_per_wave
+                                        : n_keys) *
+                         (pl.kernel == MG_KERNEL_STREAM_SPLIT ? 2 : 1);
   }
   return MG_OK;
 }

@@ -106,6 +106,30 @@ def test_stream_kernel_matches_oracle(mg, scanner, n, w, d):
     m.close()
 
 
+@pytest.mark.parametrize("n,w,d", [(3000, 1024, 12), (5000, 2048, 13), (777, 4096, 10),
+                                   (2500, 272, 12), (600, 32, 10), (70_001, 1024, 17)])
+def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
+    # Producer/consumer blocks: masks from a DFS wave, rows from a streaming wave.
+    print(f"case split n={n} w={w} d={d}", flush=True)
+    rows = oracle.fill_rows(seed=n * 5 + w, first_row=0, n_rows=n, row_bytes=w)
+    m = upload(mg, scanner, rows)
+    targets = [0, n - 1, n // 3, n // 2, n // 5]
+    keys = [p[i % 2] for i, p in enumerate(pairs_for(d, targets, seed=w + 1))]
+    ref = oracle.scan_tiled(keys, rows, THREADS)
+    max_g = 4 if w <= 1024 else (3 if w <= 2048 else 2)
+    for g in range(1, max_g + 1):
+        for kl in sorted({1, 3, d - 6}):
+            got, t = scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT,
+                                                            lane_depth=kl, keys_per_wave=g),
+                                       with_timing=True)
+            assert t["kernel"] == "stream_split"
+            assert np.array_equal(got, ref), (g, kl)
+    # Too shallow a domain for 64 live lanes.
+    with pytest.raises(mg.MgError):
+        scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=d - 5))
+    m.close()
+
+
 def test_point_kernel_and_zero_domain(mg, scanner):
     rows = oracle.fill_rows(seed=4, first_row=0, n_rows=700, row_bytes=48)
     m = upload(mg, scanner, rows)
