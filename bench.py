@@ -95,7 +95,8 @@ def main():
     ap.add_argument("--waves-per-block", type=int, default=0)
     ap.add_argument("--row-bytes", type=int, default=32)
     ap.add_argument("--domain-bits", type=int, default=0, help="0 = rows-log2")
-    ap.add_argument("--keys", type=int, default=24, help="keys per batched pass (even)")
+    ap.add_argument("--keys", type=int, default=24,
+                    help="keys per batched pass: two-party pairs, plus one single key when odd")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--lane-depth", type=int, default=0)
     ap.add_argument("--waves-per-key", type=int, default=0)
@@ -108,6 +109,12 @@ def main():
     ap.add_argument("--delta-rows", type=int, default=1 << 16,
                     help="epoch DeltaBuffer entries applied in every pass (uniform rows, random "
                          "diffs); 0 = none")
+    ap.add_argument("--tuned", action="store_true",
+                    help="time the autotuner's stored setting for this shape "
+                         "(morphogenesis_amd/tuned.json) when there is one")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune this shape first (every candidate checked bit for bit against "
+                         "the first) and time the winner")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a 4 GiB matrix once (known bytes for counter scaling)")
     args = ap.parse_args()
@@ -126,7 +133,7 @@ def main():
     n_rows = args.rows or (1 << args.rows_log2)
     w = args.row_bytes
     d = args.domain_bits or max(1, (n_rows - 1).bit_length())
-    q = max(2, args.keys - args.keys % 2)
+    q = max(1, args.keys)
     scanner = mg.GpuScanner(0)
     info = scanner.info()
     log(f"device {info['name']} CUs={info['compute_units']} free={info['free_bytes'] / 2**30:.1f} GiB")
@@ -147,9 +154,11 @@ def main():
         f"{time.perf_counter() - t0:.1f} s")
 
     rng = np.random.default_rng(args.seed)
-    targets = [int(x) for x in rng.integers(0, n_rows, size=q // 2)]
+    # q keys: q // 2 two-party pairs, and with odd q one more key whose
+    # partner is scanned once, untimed, to check it.
+    targets = [int(x) for x in rng.integers(0, n_rows, size=(q + 1) // 2)]
     pairs = [mg.generate_keys(d, t, seed=1000 + i) for i, t in enumerate(targets)]
-    keys = [k for p in pairs for k in p]
+    keys = [k for p in pairs for k in p][:q]
     # The epoch DeltaBuffer every pass applies on top of the matrix: uniform
     # rows with replacement, random diffs; the first target is forced into it
     # so the bit-exact check covers the delta path. Answers are for the
@@ -176,15 +185,30 @@ def main():
     kernel = {"auto": mg.KERNEL_AUTO, "lane_dfs": mg.KERNEL_LANE_DFS,
               "stream": mg.KERNEL_STREAM, "point": mg.KERNEL_POINT,
               "suffix": mg.KERNEL_SUFFIX, "split": mg.KERNEL_STREAM_SPLIT}[args.kernel]
-    opts = mg.make_opts(kernel=kernel, lane_depth=args.lane_depth,
-                        keys_per_wave=args.keys_per_wave, waves_per_key=args.waves_per_key,
-                        waves_per_block=args.waves_per_block, use_graph=args.use_graph,
-                        delta=delta)
+    tuned = None
+    if args.autotune or args.tuned:
+        from morphogenesis_amd import autotune
+        if args.autotune:
+            best, _ = autotune.tune(scanner, m, keys, reps=2, log=log)
+            tuned = {k: best[k] for k in autotune.SETTING_KEYS if k in best} if best else None
+        else:
+            tuned = autotune.lookup(autotune.device_arch(scanner), w, n_rows, q)
+        log(f"tuned setting: {tuned}")
+    if tuned:
+        opts = mg.make_opts(**tuned, use_graph=args.use_graph, delta=delta)
+    else:
+        opts = mg.make_opts(kernel=kernel, lane_depth=args.lane_depth,
+                            keys_per_wave=args.keys_per_wave, waves_per_key=args.waves_per_key,
+                            waves_per_block=args.waves_per_block, use_graph=args.use_graph,
+                            delta=delta)
+    odd_partner = scanner.scan_keys(m, [pairs[-1][1]], opts)[0] if q % 2 else None
 
     def check(ans):
-        for i, t in enumerate(targets):
+        for i in range(q // 2):
             if not np.array_equal(ans[2 * i] ^ ans[2 * i + 1], target_rows[i]):
-                raise SystemExit(f"bit-exactness failure: pair {i} target {t}")
+                raise SystemExit(f"bit-exactness failure: pair {i} target {targets[i]}")
+        if q % 2 and not np.array_equal(ans[q - 1] ^ odd_partner, target_rows[-1]):
+            raise SystemExit(f"bit-exactness failure: single key, target {targets[-1]}")
 
     for i in range(args.warmup):
         ans, t = scanner.scan_keys(m, keys, opts, with_timing=True)
@@ -206,7 +230,7 @@ def main():
 
     # Plain path: one query (3 keys) per pass over the whole matrix.
     plain_ms = []
-    plain_keys = keys[:KEYS_PER_QUERY]
+    plain_keys = (keys * KEYS_PER_QUERY)[:KEYS_PER_QUERY]
     for i in range(args.plain_steps):
         s0 = time.perf_counter()
         scanner.scan_keys(m, plain_keys, opts)
@@ -220,7 +244,7 @@ def main():
         ref_opts = mg.make_opts(kernel=mg.KERNEL_SUFFIX, delta=delta)
         for i in range(args.ref_plain_steps):
             s0 = time.perf_counter()
-            ans_ref = scanner.scan_keys(m, keys[:2], ref_opts)
+            ans_ref = scanner.scan_keys(m, list(pairs[0]), ref_opts)
             ref_ms.append((time.perf_counter() - s0) * 1e3 * KEYS_PER_QUERY / 2)
             if not np.array_equal(ans_ref[0] ^ ans_ref[1], target_rows[0]):
                 raise SystemExit("bit-exactness failure in the suffix-walk baseline")
@@ -301,11 +325,13 @@ def main():
                 "uniform targets",
         "config": {"workload": f"{rows_label} rows x {w} B + a {args.delta_rows}-row epoch "
                                f"delta, {q} DPF keys per pass over a {d}-bit domain "
-                               f"({q // 2} two-party pairs)",
+                               f"({q // 2} two-party pairs"
+                               f"{' and 1 single key' if q % 2 else ''})",
                    "rows": n_rows, "row_bytes": w, "domain_bits": d, "keys_per_pass": q,
                    "delta_rows": args.delta_rows,
                    "kernel": timing["kernel"], "lane_depth": timing["lane_depth"],
-                   "waves": timing["waves"], "use_graph": bool(args.use_graph)},
+                   "waves": timing["waves"], "use_graph": bool(args.use_graph),
+                   "tuned": tuned},
         "p50_ms": percentile(step_ms, 50),
         "p99_ms": percentile(step_ms, 99),
         "keys_per_s": q * args.steps / wall,

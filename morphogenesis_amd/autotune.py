@@ -21,10 +21,12 @@ import time
 
 import numpy as np
 
-from . import (KERNEL_LANE_DFS, KERNEL_STREAM, KERNEL_STREAM_SPLIT, GpuPageMatrix, GpuScanner,
-               generate_keys, make_opts)
+from . import (KERNEL_AUTO, KERNEL_LANE_DFS, KERNEL_STREAM, KERNEL_STREAM_SPLIT, GpuPageMatrix,
+               GpuScanner, generate_keys, make_opts)
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
+# The make_opts fields a stored setting holds.
+SETTING_KEYS = ("kernel", "lane_depth", "keys_per_wave", "waves_per_block", "waves_per_key")
 
 
 def keys_bucket(n_keys: int) -> int:
@@ -50,30 +52,52 @@ def lookup(arch, row_bytes, n_rows, n_keys, path=TABLE):
     return load_table(path).get(table_key(arch, row_bytes, n_rows, n_keys))
 
 
+def device_arch(scanner) -> str:
+    """The gfx target of the scanner's device, e.g. 'gfx950'."""
+    return scanner.info()["name"].split("(")[-1].split(":")[0]
+
+
+def tuned_opts(scanner, matrix, n_keys, delta=None, path=TABLE):
+    """make_opts of the stored winner for this device, matrix shape and key
+    count, or None when the table has no entry (the scan's built-in defaults
+    then apply)."""
+    entry = lookup(device_arch(scanner), matrix.row_bytes, matrix.n_rows, n_keys, path)
+    return make_opts(**entry, delta=delta) if entry else None
+
+
 def candidates(row_bytes: int, n_keys: int):
+    """The grid for one shape. The first candidate is the scan's built-in
+    choice (kernel AUTO), so a stored winner is never slower than the default
+    on the shape it was tuned on."""
     chunks = row_bytes // 16
-    out = []
+    out = [dict(kernel=KERNEL_AUTO)]
     if chunks <= 8:
         for kl, wpb in itertools.product((7, 8, 9, 10, 11, 12), (1, 2, 4)):
             out.append(dict(kernel=KERNEL_LANE_DFS, lane_depth=kl, waves_per_block=wpb))
     if chunks <= 256:
         gmax = 4 if chunks <= 64 else (3 if chunks <= 128 else 2)
-        for kl, g, wpb in itertools.product((4, 5, 6, 7), range(1, gmax + 1), (1, 2)):
+        for kl, g, wpb in itertools.product((4, 5, 6, 7, 8), range(1, gmax + 1), (1, 2)):
             if g > n_keys:
                 continue
             out.append(dict(kernel=KERNEL_STREAM, lane_depth=kl, keys_per_wave=g,
                             waves_per_block=wpb))
-        for kl, g in itertools.product((4, 5, 6, 7, 8), range(1, gmax + 1)):
+        for kl, g in itertools.product((5, 6, 7, 8), range(1, gmax + 1)):
             if g > n_keys:
                 continue
-            out.append(dict(kernel=KERNEL_STREAM_SPLIT, lane_depth=kl, keys_per_wave=g,
-                            waves_per_block=2))
+            out.append(dict(kernel=KERNEL_STREAM_SPLIT, lane_depth=kl, keys_per_wave=g))
     return out
 
 
-def tune(scanner, matrix, keys, reps=2, grid=None, log=print):
-    """Times each candidate; returns (best option dict, result rows)."""
+def tune(scanner, matrix, keys, reps=2, grid=None, log=print, warmup=2):
+    """Times each candidate; returns (best option dict, result rows).
+
+    `warmup` untimed passes with the default setting go first: the first
+    passes over a freshly written matrix run slower (78,643,200 x 1 KiB rows,
+    3 keys: 18.7 ms, then 12.8-12.9 ms), which would otherwise count against
+    whichever candidate is timed first."""
     grid = grid or candidates(matrix.row_bytes, len(keys))
+    for _ in range(warmup):
+        scanner.scan_keys(matrix, keys)
     ref = None
     rows = []
     for c in grid:
@@ -109,14 +133,15 @@ def main():
     ap.add_argument("--keys", type=int, default=3)
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--kernel", choices=["all", "lane_dfs", "stream", "split"], default="all")
-    ap.add_argument("--save", action="store_true", help="store the winner in tuned.json")
+    ap.add_argument("--save", action="store_true", help="store the winner in the table")
+    ap.add_argument("--table", default=TABLE, help="table file (default: the package's tuned.json)")
     ap.add_argument("--config", default="",
                     help="run one setting only: kernel,lane_depth,keys_per_wave,waves_per_block")
     args = ap.parse_args()
     n_rows = args.rows or (1 << args.rows_log2)
     d = args.domain_bits or max(1, (n_rows - 1).bit_length())
     scanner = GpuScanner(0)
-    arch = scanner.info()["name"].split("(")[-1].split(":")[0]
+    arch = device_arch(scanner)
     m = GpuPageMatrix(scanner, n_rows, args.row_bytes)
     m.fill_seeded(5)
     rng = np.random.default_rng(1)
@@ -126,7 +151,7 @@ def main():
     if args.kernel != "all":
         kid = {"lane_dfs": KERNEL_LANE_DFS, "stream": KERNEL_STREAM,
                "split": KERNEL_STREAM_SPLIT}[args.kernel]
-        grid = [c for c in grid if c["kernel"] == kid]
+        grid = [c for c in grid if c["kernel"] in (kid, KERNEL_AUTO)]
     if args.config:
         k, kl, g, wpb = (int(x) for x in args.config.split(","))
         grid = [dict(kernel=k, lane_depth=kl, keys_per_wave=g, waves_per_block=wpb)]
@@ -138,11 +163,10 @@ def main():
               "best": best, "rows": rows, "seconds": time.time() - t0}
     print(json.dumps(result))
     if args.save and best:
-        table = load_table()
-        entry = {k: best[k] for k in ("kernel", "lane_depth", "keys_per_wave", "waves_per_block")
-                 if k in best}
+        table = load_table(args.table)
+        entry = {k: best[k] for k in SETTING_KEYS if k in best}
         table[table_key(arch, args.row_bytes, n_rows, args.keys)] = entry
-        with open(TABLE, "w") as f:
+        with open(args.table, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)
     m.close()
     scanner.close()

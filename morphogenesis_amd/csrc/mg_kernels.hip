@@ -8,9 +8,10 @@
 //                    each lane walks its own 2^kl-leaf subtree depth-first,
 //                    so the GGM tree costs one PRG expansion per leaf pair
 //                    plus a short per-tile path walk.
-// scan_stream<S,G>   wide rows (W >= 256 B): same lane-DFS PRG for G keys per
-//                    wave; leaf masks go through LDS and each row is read
-//                    once, coalesced, for all G keys.
+// scan_stream<S,G,PK> rows wider than 128 B: same lane-DFS PRG for G keys
+//                    per wave; leaf masks go through LDS and each row is read
+//                    once, coalesced, for all G keys (PK rows per load when a
+//                    row has at most 64 / PK chunks).
 // scan_point         point-eval fallback (d = 0, tiny inputs) and the delta
 //                    scan: one lane per row, full path walk per row.
 // reduce_partials    XOR of per-wave partial answers into the output.
@@ -46,6 +47,17 @@ __device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
 
 __device__ __forceinline__ uint4 wave_xor4(uint4 v) {
   return make_uint4(wave_xor(v.x), wave_xor(v.y), wave_xor(v.z), wave_xor(v.w));
+}
+
+// XOR of v over the lanes l, l ^ from, l ^ 2*from, ... (from a power of two).
+__device__ __forceinline__ uint4 xor_lanes_from(uint4 v, int from) {
+  for (int off = from; off < 64; off <<= 1) {
+    v.x ^= __shfl_xor(v.x, off, 64);
+    v.y ^= __shfl_xor(v.y, off, 64);
+    v.z ^= __shfl_xor(v.z, off, 64);
+    v.w ^= __shfl_xor(v.w, off, 64);
+  }
+  return v;
 }
 
 // Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
@@ -266,10 +278,18 @@ __global__ __launch_bounds__(256) void scan_suffix(SuffixParams p) {
 // the lane DFS leaves 2 masks per lane per key in LDS; the wave then reads
 // those 128 rows once each, coalesced (lane = chunk), and folds them into G
 // accumulators. Accumulator slot j of lane l holds chunk l + 64*j.
-template <int S, int G>
+// Narrow rows (C <= 64 / PK, S = 1) are packed PK to a load: lane l reads
+// chunk l % (64 / PK) of row l / (64 / PK) of each group of PK rows, so a
+// load instruction, an LDS mask read and a fold carry PK rows instead of
+// one. The PK lanes that hold one chunk are XOR-combined at the end.
+template <int S, int G, int PK>
 __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
+  static_assert(PK == 1 || S == 1, "rows are packed only when one load covers a row");
+  constexpr int P = 64 / PK;  // lanes per row
   extern __shared__ uint4 lds[];
   const int lane = threadIdx.x & 63;
+  const uint32_t sub = PK > 1 ? (uint32_t)lane / P : 0;  // this lane's row within a load
+  const uint32_t lane_chunk = PK > 1 ? (uint32_t)lane % P : (uint32_t)lane;
   const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
   const uint32_t w = xcd_logical_wave(p.parts * n_groups, blockDim.x >> 6, wib);
@@ -317,65 +337,69 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
 
   // Rows of one leaf-pair step: 2 per live source lane s, tile_base + s*2^kl +
   // 2*pair + {0,1}; row r of the step uses mask slot r. They are read in
-  // batches of U rows (8 KiB of 1 KiB rows), two batches in flight: the first
-  // two batches of a step are issued before that step's PRG work, so loads
-  // overlap the ChaCha rounds.
+  // batches of U rows (IU load instructions: 8 KiB of 1 KiB rows), two batches
+  // in flight: the first two batches of a step are issued before that step's
+  // PRG work, so loads overlap the ChaCha rounds.
   // Loads are branch-free: a row's base address is wave-uniform (a row past
   // the step or the matrix reads the zero buffer instead), and each lane's
   // chunk offset is fixed for the whole kernel. A lane past the row's last
   // chunk re-reads chunk 0; its accumulator is never written out.
-  constexpr int U = 8 / S;
+  constexpr int IU = 8 / S;   // load instructions per batch
+  constexpr int U = IU * PK;  // rows per batch
   const uint32_t step_rows = 2u * (uint32_t)lanes_live;
   uint32_t lane_off[S];
 #pragma unroll
   for (int j = 0; j < S; j++) {
-    const uint32_t c = lane + 64 * j;
+    const uint32_t c = lane_chunk + 64 * j;
     lane_off[j] = c < C ? c : 0;
   }
   // In a tile that lies wholly inside the matrix (every tile but a ragged
   // last one) a batch's rows sit at fixed distances from its first row: row
-  // r0 + u (r0 even) is (u >> 1) lane subtrees of 2^kl rows and (u & 1) rows
-  // further on. Those byte offsets, with this lane's chunk, are computed once,
-  // so a batch costs one uniform base address and U x S plain loads.
-  uint32_t row_off[U][S];
+  // r0 + q (r0 even) is (q >> 1) lane subtrees of 2^kl rows and (q & 1) rows
+  // further on. This lane reads row q = i*PK + sub with load i. Those byte
+  // offsets, with this lane's chunk, are computed once, so a batch costs one
+  // uniform base address and IU x S plain loads.
+  uint32_t row_off[IU][S];
 #pragma unroll
-  for (int u = 0; u < U; u++)
+  for (int i = 0; i < IU; i++)
 #pragma unroll
-    for (int j = 0; j < S; j++)
-      row_off[u][j] =
-          (uint32_t)(((((uint64_t)(u >> 1) << kl) + (u & 1)) * C + lane_off[j]) * 16);
-  uint4 va[U][S], vb[U][S];
-  auto load_batch = [&](uint64_t tile_base, uint32_t pair, uint32_t r0, uint4 (&v)[U][S],
+    for (int j = 0; j < S; j++) {
+      const uint32_t q = (uint32_t)i * PK + sub;
+      row_off[i][j] =
+          (uint32_t)(((((uint64_t)(q >> 1) << kl) + (q & 1)) * C + lane_off[j]) * 16);
+    }
+  uint4 va[IU][S], vb[IU][S];
+  auto load_batch = [&](uint64_t tile_base, uint32_t pair, uint32_t r0, uint4 (&v)[IU][S],
                         bool whole_tile) {
     if (whole_tile) {
       const char* base = reinterpret_cast<const char*>(
           p.rows + (tile_base + ((uint64_t)(r0 >> 1) << kl) + 2 * (uint64_t)pair) * C);
 #pragma unroll
-      for (int u = 0; u < U; u++)
+      for (int i = 0; i < IU; i++)
 #pragma unroll
         for (int j = 0; j < S; j++)
-          v[u][j] = load_nt(reinterpret_cast<const uint4*>(base + row_off[u][j]));
+          v[i][j] = load_nt(reinterpret_cast<const uint4*>(base + row_off[i][j]));
       return;
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const uint32_t r = r0 + u;
+    for (int i = 0; i < IU; i++) {
+      const uint32_t r = r0 + (uint32_t)i * PK + sub;
       const uint64_t x = tile_base + ((uint64_t)(r >> 1) << kl) + 2 * (uint64_t)pair + (r & 1);
       const uint4* src = (r < step_rows && x < p.n_rows) ? p.rows + x * C : p.zero_rows;
 #pragma unroll
-      for (int j = 0; j < S; j++) v[u][j] = load_nt(src + lane_off[j]);
+      for (int j = 0; j < S; j++) v[i][j] = load_nt(src + lane_off[j]);
     }
   };
-  // Slot indices stay below 128 (r0 + u < max(step_rows, U) <= 128), so they
-  // need no wrap and become immediate LDS offsets.
-  auto fold_batch = [&](uint32_t r0, const uint4 (&v)[U][S]) {
+  // Slot indices stay below 128 (r0 + q < max(step_rows, U) <= 128), so they
+  // need no wrap.
+  auto fold_batch = [&](uint32_t r0, const uint4 (&v)[IU][S]) {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int i = 0; i < IU; i++) {
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        const uint4 m = masks[g * 128 + r0 + u];
+        const uint4 m = masks[g * 128 + r0 + i * PK + sub];
 #pragma unroll
-        for (int j = 0; j < S; j++) xor_masked3(acc[g][j], v[u][j], m);
+        for (int j = 0; j < S; j++) xor_masked3(acc[g][j], v[i][j], m);
       }
     }
   };
@@ -441,11 +465,12 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
 
 #pragma unroll
   for (int g = 0; g < G; g++) {
+    if (PK > 1) acc[g][0] = xor_lanes_from(acc[g][0], P);
     if (!key_live[g]) continue;
 #pragma unroll
     for (int j = 0; j < S; j++) {
-      const uint32_t c = lane + 64 * j;
-      if (c < C) p.partial[((size_t)part * p.n_keys + key0 + g) * C + c] = acc[g][j];
+      const uint32_t c = lane_chunk + 64 * j;
+      if (sub == 0 && c < C) p.partial[((size_t)part * p.n_keys + key0 + g) * C + c] = acc[g][j];
     }
   }
 }
@@ -893,21 +918,27 @@ size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block) {
   return (size_t)waves_per_block * ((size_t)G * kl * 80 + (size_t)G * 128) * sizeof(uint4);
 }
 
-hipError_t launch_stream(int S, int G, const StreamParams& p, int waves_per_block,
+// (S, G, PK) instances of scan_stream: PK > 1 only with S = 1.
+#define MG_STREAM_SHAPES                                                      \
+  MG_SG(1, 1, 1) MG_SG(1, 2, 1) MG_SG(1, 3, 1) MG_SG(1, 4, 1)                 \
+  MG_SG(1, 1, 2) MG_SG(1, 2, 2) MG_SG(1, 3, 2) MG_SG(1, 4, 2)                 \
+  MG_SG(1, 1, 4) MG_SG(1, 2, 4) MG_SG(1, 3, 4) MG_SG(1, 4, 4)                 \
+  MG_SG(2, 1, 1) MG_SG(2, 2, 1) MG_SG(2, 3, 1)                                \
+  MG_SG(4, 1, 1) MG_SG(4, 2, 1)
+
+hipError_t launch_stream(int S, int G, int PK, const StreamParams& p, int waves_per_block,
                          hipStream_t s) {
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
   const uint32_t total = p.parts * n_groups;
   const dim3 block(64 * waves_per_block);
   const dim3 grid(xcd_grid(total, waves_per_block));
   const size_t lds = stream_lds_bytes(G, p.kl, waves_per_block);
-#define MG_SG(SS, GG)                                                         \
-  if (S == SS && G == GG) {                                                   \
-    hipLaunchKernelGGL((scan_stream<SS, GG>), grid, block, lds, s, p);        \
+#define MG_SG(SS, GG, KK)                                                     \
+  if (S == SS && G == GG && PK == KK) {                                       \
+    hipLaunchKernelGGL((scan_stream<SS, GG, KK>), grid, block, lds, s, p);    \
     return hipGetLastError();                                                 \
   }
-  MG_SG(1, 1) MG_SG(1, 2) MG_SG(1, 3) MG_SG(1, 4)
-  MG_SG(2, 1) MG_SG(2, 2) MG_SG(2, 3)
-  MG_SG(4, 1) MG_SG(4, 2)
+  MG_STREAM_SHAPES
 #undef MG_SG
   return hipErrorInvalidValue;
 }
@@ -925,14 +956,13 @@ hipError_t lane_dfs_blocks_per_cu(int chunks, int block_threads, size_t lds, int
   }
 }
 
-hipError_t stream_blocks_per_cu(int S, int G, int block_threads, size_t lds, int* blocks) {
-#define MG_SG(SS, GG)                                                                       \
-  if (S == SS && G == GG)                                                                   \
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_stream<SS, GG>,        \
+hipError_t stream_blocks_per_cu(int S, int G, int PK, int block_threads, size_t lds,
+                                int* blocks) {
+#define MG_SG(SS, GG, KK)                                                                   \
+  if (S == SS && G == GG && PK == KK)                                                       \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_stream<SS, GG, KK>,    \
                                                         block_threads, lds);
-  MG_SG(1, 1) MG_SG(1, 2) MG_SG(1, 3) MG_SG(1, 4)
-  MG_SG(2, 1) MG_SG(2, 2) MG_SG(2, 3)
-  MG_SG(4, 1) MG_SG(4, 2)
+  MG_STREAM_SHAPES
 #undef MG_SG
   return hipErrorInvalidValue;
 }

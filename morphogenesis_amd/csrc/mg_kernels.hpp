@@ -75,7 +75,7 @@ struct SuffixParams {
 hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s);
 hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_block,
                            hipStream_t s);
-hipError_t launch_stream(int S, int G, const StreamParams& p, int waves_per_block,
+hipError_t launch_stream(int S, int G, int PK, const StreamParams& p, int waves_per_block,
                          hipStream_t s);
 size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block);
 // The producer/consumer variant of the stream kernel: blocks of one DFS wave
@@ -85,7 +85,8 @@ size_t split_lds_bytes(int G, uint32_t kl);
 hipError_t split_blocks_per_cu(int S, int G, size_t lds, int* blocks);
 // Resident blocks per CU for the given block size and dynamic LDS.
 hipError_t lane_dfs_blocks_per_cu(int chunks, int block_threads, size_t lds, int* blocks);
-hipError_t stream_blocks_per_cu(int S, int G, int block_threads, size_t lds, int* blocks);
+hipError_t stream_blocks_per_cu(int S, int G, int PK, int block_threads, size_t lds,
+                                int* blocks);
 hipError_t launch_point(const PointParams& p, hipStream_t s);
 // out[out_idx ? out_idx[k] : k] (^)= XOR of the parts' partials of key k.
 hipError_t launch_reduce(const uint4* partial, uint32_t parts, uint32_t n_keys, uint32_t chunks,

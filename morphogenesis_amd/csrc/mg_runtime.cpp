@@ -141,9 +141,10 @@ void dedupe_delta(const uint64_t* idx, const uint8_t* diffs, uint64_t n, uint32_
 namespace {
 
 // Resident waves the device holds for a kernel variant (cached per variant).
-int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int chunks, int waves_per_block,
-                   size_t lds, int* out) {
-  auto key = std::make_pair(kind * 64 + (uint32_t)waves_per_block, (uint32_t)lds);
+int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chunks,
+                   int waves_per_block, size_t lds, int* out) {
+  auto key = std::make_pair((kind * 8 + (uint32_t)PK) * 64 + (uint32_t)waves_per_block,
+                            (uint32_t)lds);
   auto it = dev->occupancy.find(key);
   int blocks = 0;
   if (it != dev->occupancy.end()) {
@@ -154,7 +155,7 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int chunks, int 
     else if (kind / 100 == MG_KERNEL_STREAM_SPLIT)
       MG_HIP(split_blocks_per_cu(S, G, lds, &blocks));
     else
-      MG_HIP(stream_blocks_per_cu(S, G, 64 * waves_per_block, lds, &blocks));
+      MG_HIP(stream_blocks_per_cu(S, G, PK, 64 * waves_per_block, lds, &blocks));
     dev->occupancy[key] = blocks;
   }
   *out = std::max(1, blocks) * waves_per_block * dev->cus;
@@ -164,6 +165,24 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int chunks, int 
 int max_keys_per_wave(int S) { return S == 1 ? 4 : (S == 2 ? 3 : 2); }
 
 int stream_s(uint32_t chunks) { return chunks <= 64 ? 1 : (chunks <= 128 ? 2 : 4); }
+
+// Rows per load instruction of the stream kernel: rows of at most 16 (32)
+// chunks go 4 (2) to a load. A 144-byte row then fills 36 lanes of a load
+// instead of 9.
+int stream_pk(uint32_t chunks) { return chunks <= 16 ? 4 : (chunks <= 32 ? 2 : 1); }
+
+// Keys per wave (stream) or per block (split) when the caller leaves it open. Keys go evenly
+// into the fewest groups of at most gmax keys: a pass costs about as much as its widest group,
+// so 4 keys run as 2 + 2 (32 GiB of 272-byte rows, stream kernel: 44.0 ms as 3 + 1). The split
+// kernel's cost per key falls up to its widest group (1 KiB rows: 11.7 ms for one key, 14.9 ms
+// for four), the stream kernel's up to 3.
+int default_keys_per_wave(uint32_t kernel, int S, uint32_t n_keys) {
+  const int gmax = kernel == MG_KERNEL_STREAM_SPLIT ? max_keys_per_wave(S)
+                                                    : std::min(3, max_keys_per_wave(S));
+  const int nk = std::max(1, (int)n_keys);
+  const int ng = (nk + gmax - 1) / gmax;
+  return (nk + ng - 1) / ng;
+}
 
 // Splits a d-bit domain as tb + lb + kl (see LaneDfsParams).
 void split_levels(uint32_t d, uint32_t kl_req, Plan* pl) {
@@ -201,8 +220,17 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   if (kernel > MG_KERNEL_STREAM_SPLIT)
     return fail(MG_ERR_INVALID_ARG, "unknown kernel %u", kernel);
   if (kernel == MG_KERNEL_AUTO) {
+    // Split against stream over 32 GiB (measured in one process): the split kernel wins every
+    // key count at 1 KiB rows (1 / 3 / 8 keys: 5.08 / 5.79 / 11.77 ms against 5.32 / 6.33 /
+    // 18.74 ms) and three or more keys at 528-1008-byte rows (768 B, 3 keys: 7.30 against
+    // 8.24 ms), but loses one or two keys there (768 B, 1 key: 6.04 against 5.26 ms) and
+    // everywhere below 528 B, where rows pack several to a load. It needs d >= kl + 6.
+    const uint32_t kl_split = o->lane_depth ? o->lane_depth : 6;
     if (d == 0) kernel = MG_KERNEL_POINT;
     else if (pl->chunks <= 8) kernel = MG_KERNEL_LANE_DFS;
+    else if (pl->chunks > 32 && pl->chunks <= 64 && (n_keys >= 3 || pl->chunks == 64) &&
+             d >= kl_split + 6)
+      kernel = MG_KERNEL_STREAM_SPLIT;
     else if (pl->chunks <= 256) kernel = MG_KERNEL_STREAM;
     else kernel = MG_KERNEL_POINT;
   }
@@ -238,7 +266,9 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   // as long as the keys still have at least ~4096 tiles of 2^(kl+6) rows between them to spread
   // over the resident waves. The stream kernel at 78,643,200 x 1 KiB: one key per wave runs best
   // at kl = 4 (11.97 ms; 12.08 ms at kl = 6), two keys per wave at kl = 6 (13.04 ms; 13.14 ms
-  // at kl = 7), three at kl = 4 to 7 alike (15.02 to 15.13 ms).
+  // at kl = 7), three at kl = 4 to 7 alike (15.02 to 15.13 ms). Narrower rows carry more PRG
+  // work per byte and want fewer lane-root walks: 32 GiB of 144-byte rows, one key, 10.4 ms at
+  // kl = 4 and 9.2 ms at kl = 6; 272-byte rows 7.04 / 6.66 ms; 512-byte rows 5.36 / 5.25 ms.
   uint32_t kl_default;
   if (kernel == MG_KERNEL_LANE_DFS) {
     const uint64_t work = (pl->row_end - pl->row_begin) * (uint64_t)n_keys;
@@ -249,8 +279,9 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     // 14.92 ms for one to four keys per group (kl = 7: 11.42 / 11.94 / 13.21 / 15.08 ms).
     kl_default = 6;
   } else {
-    const uint32_t g = o->keys_per_wave ? o->keys_per_wave : std::min<uint32_t>(3, n_keys);
-    kl_default = g <= 1 ? 4 : (g == 2 ? 6 : 7);
+    const int g = o->keys_per_wave ? (int)o->keys_per_wave
+                                   : default_keys_per_wave(kernel, stream_s(pl->chunks), n_keys);
+    kl_default = g <= 2 ? 6 : 7;
   }
   split_levels(d, o->lane_depth ? o->lane_depth : kl_default, pl);
   if (kernel == MG_KERNEL_STREAM_SPLIT && pl->lb != 6)
@@ -276,19 +307,11 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     if (lds > dev->max_lds)
       return fail(MG_ERR_INVALID_ARG, "lane-DFS plan needs %zu B of LDS per block (max %zu)", lds,
                   dev->max_lds);
-    rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl->chunks, 0, 0, (int)pl->chunks,
+    rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl->chunks, 0, 0, 1, (int)pl->chunks,
                         (int)pl->waves_per_block, lds, &resident);
   } else {
     const int S = stream_s(pl->chunks);
-    int G = o->keys_per_wave ? (int)o->keys_per_wave : std::min<int>(3, (int)n_keys);
-    if (!o->keys_per_wave && kernel == MG_KERNEL_STREAM_SPLIT) {
-      // The split kernel's cost per key falls up to the widest group (1 KiB rows: 11.7 ms for
-      // one key, 14.9 ms for four); spread the keys evenly over the fewest groups.
-      const int gmax = max_keys_per_wave(S);
-      const int nk = std::max(1, (int)n_keys);
-      const int ng = (nk + gmax - 1) / gmax;
-      G = (nk + ng - 1) / ng;
-    }
+    int G = o->keys_per_wave ? (int)o->keys_per_wave : default_keys_per_wave(kernel, S, n_keys);
     G = std::max(1, std::min(G, max_keys_per_wave(S)));
     pl->keys_per_wave = (uint32_t)G;
     groups = (n_keys + G - 1) / G;
@@ -298,7 +321,8 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
       return fail(MG_ERR_INVALID_ARG, "stream plan needs %zu B of LDS per block (max %zu)", lds,
                   dev->max_lds);
     rc = resident_waves(dev, MG_KERNEL_STREAM * 100 + (uint32_t)(S * 8 + G), S, G,
-                        (int)pl->chunks, (int)pl->waves_per_block, lds, &resident);
+                        stream_pk(pl->chunks), (int)pl->chunks, (int)pl->waves_per_block, lds,
+                        &resident);
   }
   if (!rc && kernel == MG_KERNEL_STREAM_SPLIT) {
     // One block = one DFS wave + one streaming wave; parts count blocks.
@@ -310,7 +334,7 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
       return fail(MG_ERR_INVALID_ARG, "split plan needs %zu B of LDS per block (max %zu)", lds,
                   dev->max_lds);
     int res_waves = 0;
-    rc = resident_waves(dev, MG_KERNEL_STREAM_SPLIT * 100 + (uint32_t)(S * 8 + G), S, G,
+    rc = resident_waves(dev, MG_KERNEL_STREAM_SPLIT * 100 + (uint32_t)(S * 8 + G), S, G, 1,
                         (int)pl->chunks, 2, lds, &res_waves);
     // Three blocks per CU (768 on 256 CUs) stream fastest; a fourth block on some CUs
     // unbalances them (four keys per group: 14.9 ms at 768 blocks, 26.5 ms at 832).
@@ -370,7 +394,7 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       if (pl.kernel == MG_KERNEL_STREAM_SPLIT)
         MG_HIP(launch_split(stream_s(pl.chunks), (int)pl.keys_per_wave, p, s));
       else
-        MG_HIP(launch_stream(stream_s(pl.chunks), (int)pl.keys_per_wave, p,
+        MG_HIP(launch_stream(stream_s(pl.chunks), (int)pl.keys_per_wave, stream_pk(pl.chunks), p,
                              (int)pl.waves_per_block, s));
     } else if (pl.kernel == MG_KERNEL_SUFFIX) {
       SuffixParams p{};

@@ -270,9 +270,10 @@ int sweep_step(mg_server* s, std::vector<uint32_t>* finished) {
       if (s->slots[i].req && s->slots[i].base == b) h_list[off++] = i;
     const uint32_t n = off - start;
     if (!n || row_end <= row_begin) continue;
+    // The scan's own choice for this row width and key count (lane-DFS up to 128-byte rows,
+    // then the stream or split kernel); all of them tile a segment at this lane depth alike.
     mg_scan_opts o{};
-    o.kernel = MG_KERNEL_LANE_DFS;
-    if (s->chunks > 8) o.kernel = MG_KERNEL_STREAM;
+    o.kernel = MG_KERNEL_AUTO;
     o.lane_depth = s->kl;
     o.row_begin = row_begin;
     o.row_end = row_end;

@@ -81,8 +81,12 @@ def test_lane_depths_agree(mg, scanner, kl):
     m.close()
 
 
+# Rows of <= 16 chunks go 4 to a load instruction, of 17-32 chunks 2 to a
+# load (144, 256, 16, 32 / 272, 512 B); 528 B rows and wider go one to a load.
 @pytest.mark.parametrize("n,w,d", [(1000, 256, 10), (3000, 1024, 12), (777, 4096, 10),
-                                   (5000, 2048, 13), (2500, 272, 12), (600, 32, 10)])
+                                   (5000, 2048, 13), (2500, 272, 12), (600, 32, 10),
+                                   (3001, 144, 12), (2000, 512, 11), (4000, 16, 12),
+                                   (1500, 528, 11), (70_001, 144, 17), (40_000, 272, 16)])
 def test_stream_kernel_matches_oracle(mg, scanner, n, w, d):
     print(f"case n={n} w={w} d={d}", flush=True)
     rows = oracle.fill_rows(seed=n * 3 + w, first_row=0, n_rows=n, row_bytes=w)
