@@ -73,9 +73,13 @@ typedef struct {
 } mg_timing;
 
 enum {
-  MG_KERNEL_AUTO = 0,
+  MG_KERNEL_AUTO = 0,     /* lane-DFS up to 128 B rows; the split kernel
+                             for 528-1024 B rows when a pass has >= 3 keys
+                             or the rows are 1024 B (and d >= kl + 6); the
+                             stream kernel for other rows up to 4096 B    */
   MG_KERNEL_LANE_DFS = 1, /* per-lane subtree DFS, rows of <= 128 B       */
-  MG_KERNEL_STREAM = 2,   /* LDS-staged masks, coalesced wide rows         */
+  MG_KERNEL_STREAM = 2,   /* LDS-staged masks, coalesced wide rows (rows of
+                             <= 512 B packed several to a load)           */
   MG_KERNEL_POINT = 3,    /* one full path walk per row                    */
   MG_KERNEL_SUFFIX = 4,   /* comparison baseline shaped like the reference
                              kernel (fused_kernel.cu:309-360): 2048-row
