@@ -81,10 +81,12 @@ def candidates(row_bytes: int, n_keys: int):
                 continue
             out.append(dict(kernel=KERNEL_STREAM, lane_depth=kl, keys_per_wave=g,
                             waves_per_block=wpb))
-        for kl, g in itertools.product((5, 6, 7, 8), range(1, gmax + 1)):
+        # Split blocks of 1 or 4 producer/consumer pairs (2 or 8 waves).
+        for kl, g, wpb in itertools.product((5, 6, 7, 8), range(1, gmax + 1), (2, 8)):
             if g > n_keys:
                 continue
-            out.append(dict(kernel=KERNEL_STREAM_SPLIT, lane_depth=kl, keys_per_wave=g))
+            out.append(dict(kernel=KERNEL_STREAM_SPLIT, lane_depth=kl, keys_per_wave=g,
+                            waves_per_block=wpb))
     return out
 
 

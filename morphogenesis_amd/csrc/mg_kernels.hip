@@ -476,29 +476,37 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
 }
 
 // ------------------------------------------- split producer/consumer scan --
-// Wide rows, several keys per pass. The two waves of a block split the work:
-// wave 0 runs the keys' lane DFS exactly as scan_stream does and publishes
-// each leaf-pair step's 2 x 64 masks per key into one of two LDS mask
-// buffers; wave 1 only streams the step's 128 rows (NB batches of U rows in
-// flight, including the next step's first batches while it waits) and folds
-// them. A block barrier per step hands a buffer over, so wave 0 runs at most
-// one step ahead and never overwrites a buffer wave 1 still reads. Both waves
-// walk the same step sequence of the block's tile range and meet at every
-// barrier. Needs all 64 lanes live (lb = 6), so a step is always 128 rows.
+// Wide rows, several keys per pass. A block holds NP producer/consumer pairs
+// of waves: waves 0 .. NP-1 run the keys' lane DFS exactly as scan_stream
+// does and publish each leaf-pair step's 2 x 64 masks per key into one of
+// two LDS mask buffers of their pair; waves NP .. 2NP-1 only stream their
+// pair's rows (NB batches of U rows in flight, including the next step's
+// first batches while they wait) and fold them. A block barrier per step
+// hands the buffers over, so a producer runs at most one step ahead and
+// never overwrites a buffer its consumer still reads. Every wave of the
+// block takes the same number of barriers: pairs with fewer steps (the last
+// part, or pairs past the grid's work) run empty steps. With NP = 4 the
+// waves of one block land one producer and one consumer on each SIMD, where
+// single-pair blocks can stack two DFS waves on one SIMD.
+// Needs all 64 lanes live (lb = 6), so a step is always 128 rows.
 constexpr int kSplitNB = 4;
 
-template <int S, int G>
-__global__ __launch_bounds__(128) void scan_split(StreamParams p) {
+template <int S, int G, int NP>
+__global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
   extern __shared__ uint4 lds[];
   const int lane = threadIdx.x & 63;
-  const bool producer = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0;
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool producer = wib < NP;
+  const int pi = wib % NP;  // this wave's pair in the block
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
-  const uint32_t total = p.parts * n_groups;
+  const uint32_t total = p.parts * n_groups;  // pairs with work
   const uint32_t per = (gridDim.x + kXcds - 1) / kXcds;
   const uint32_t blk = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;  // XCD-aware
-  if (blk >= total) return;  // the whole block
-  const uint32_t grp = blk % n_groups;
-  const uint32_t part = blk / n_groups;
+  if ((uint64_t)blk * NP >= total) return;  // the whole block
+  const uint32_t pid = blk * NP + (uint32_t)pi;
+  const bool live = pid < total;
+  const uint32_t grp = live ? pid % n_groups : 0;
+  const uint32_t part = live ? pid / n_groups : 0;
   const uint32_t key0 = grp * G;
   const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
   const uint32_t C = p.chunks;
@@ -506,12 +514,14 @@ __global__ __launch_bounds__(128) void scan_split(StreamParams p) {
   uint64_t t_end = t_begin + p.tiles_per_wave;
   if (t_end > p.tile_end) t_end = p.tile_end;
   const uint32_t n_pairs = 1u << (kl - 1);
-  const uint64_t n_steps = t_end > t_begin ? (t_end - t_begin) * n_pairs : 0;
+  const uint64_t n_steps = live && t_end > t_begin ? (t_end - t_begin) * n_pairs : 0;
+  const uint64_t block_steps = p.tiles_per_wave * n_pairs;  // barriers every wave takes
   // Right children wait at depths 1 .. kl-1 (depth 0 is the lane root,
   // never pushed), so a key's stack has kl - 1 slots of 64 lanes.
   const int slots = kl - 1;
-  uint4* stacks = lds;                              // [G][slots][80]
-  uint4* masks = lds + (size_t)G * slots * 80;      // [2][G][128]
+  uint4* pair_lds = lds + (size_t)pi * ((size_t)G * slots * 80 + (size_t)2 * G * 128);
+  uint4* stacks = pair_lds;                              // [G][slots][80]
+  uint4* masks = pair_lds + (size_t)G * slots * 80;      // [2][G][128]
 
   if (producer) {
     const DevKey* kp[G];
@@ -528,16 +538,19 @@ __global__ __launch_bounds__(128) void scan_split(StreamParams p) {
       path[g].s3 = r.s3; path[g].t = r.t;
     }
     const int leaf_level = d - kl;
-    uint64_t s = 0;
-    for (uint64_t tile = t_begin; tile < t_end; ++tile) {
-      Node node[G];
+    Node node[G];
+    for (uint64_t s = 0; s < block_steps; ++s) {
+      if (s < n_steps) {
+        const uint64_t tile = t_begin + (s >> (kl - 1));
+        const uint32_t pair = (uint32_t)(s & (n_pairs - 1));
+        if (pair == 0) {
 #pragma unroll
-      for (int g = 0; g < G; g++)
-        node[g] = tile_lane_root(kp[g], path[g], lane, tile, tile - 1, tile == t_begin, tb, lb);
-      int lvl = 0;
-      for (uint32_t pair = 0; pair < n_pairs; ++pair, ++s) {
+          for (int g = 0; g < G; g++)
+            node[g] =
+                tile_lane_root(kp[g], path[g], lane, tile, tile - 1, tile == t_begin, tb, lb);
+        }
         uint4* mbuf = masks + (size_t)(s & 1) * G * 128;
-        if (pair) lvl = kl - 1 - __builtin_ctz(pair);
+        const int lvl = pair ? kl - 1 - __builtin_ctz(pair) : 0;
 #pragma unroll
         for (int g = 0; g < G; g++) {
           uint4* st_seed = stacks + (size_t)g * slots * 80;
@@ -558,8 +571,8 @@ __global__ __launch_bounds__(128) void scan_split(StreamParams p) {
           mbuf[g * 128 + 2 * lane] = leaf_mask(kp[g], L);
           mbuf[g * 128 + 2 * lane + 1] = leaf_mask(kp[g], R);
         }
-        __syncthreads();  // step s's masks are published
       }
+      __syncthreads();  // step s's masks are published
     }
     return;
   }
@@ -616,8 +629,9 @@ __global__ __launch_bounds__(128) void scan_split(StreamParams p) {
 #pragma unroll
     for (int k = 0; k < NB; k++) load(0, (uint32_t)k, ring[k]);
   }
-  for (uint64_t s = 0; s < n_steps; ++s) {
+  for (uint64_t s = 0; s < block_steps; ++s) {
     __syncthreads();  // step s's masks are in buffer s & 1
+    if (s >= n_steps) continue;
     const uint4* mbuf = masks + (size_t)(s & 1) * G * 128;
     for (uint32_t b0 = 0; b0 < n_batches; b0 += NB) {
 #pragma unroll
@@ -644,6 +658,7 @@ __global__ __launch_bounds__(128) void scan_split(StreamParams p) {
     }
   }
 
+  if (!live) return;
 #pragma unroll
   for (int g = 0; g < G; g++) {
     if (key0 + g >= p.n_keys) continue;
@@ -881,35 +896,39 @@ hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_split(int S, int G, const StreamParams& p, hipStream_t s) {
+// (S, G) instances of scan_split, each with NP = 1, 2 and 4 pairs per block.
+#define MG_SPLIT_SHAPES(NN)                                                   \
+  MG_SG(1, 1, NN) MG_SG(1, 2, NN) MG_SG(1, 3, NN) MG_SG(1, 4, NN)             \
+  MG_SG(2, 1, NN) MG_SG(2, 2, NN) MG_SG(2, 3, NN)                             \
+  MG_SG(4, 1, NN) MG_SG(4, 2, NN)
+
+hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t s) {
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
   const uint32_t total = p.parts * n_groups;
-  const dim3 block(128);
-  const dim3 grid((total + kXcds - 1) / kXcds * kXcds);
-  const size_t lds = split_lds_bytes(G, p.kl);
-#define MG_SG(SS, GG)                                                         \
-  if (S == SS && G == GG) {                                                   \
-    hipLaunchKernelGGL((scan_split<SS, GG>), grid, block, lds, s, p);         \
+  const uint32_t blocks = (total + NP - 1) / NP;
+  const dim3 block(128 * NP);
+  const dim3 grid((blocks + kXcds - 1) / kXcds * kXcds);
+  const size_t lds = split_lds_bytes(G, p.kl, NP);
+#define MG_SG(SS, GG, NN)                                                     \
+  if (S == SS && G == GG && NP == NN) {                                       \
+    hipLaunchKernelGGL((scan_split<SS, GG, NN>), grid, block, lds, s, p);     \
     return hipGetLastError();                                                 \
   }
-  MG_SG(1, 1) MG_SG(1, 2) MG_SG(1, 3) MG_SG(1, 4)
-  MG_SG(2, 1) MG_SG(2, 2) MG_SG(2, 3)
-  MG_SG(4, 1) MG_SG(4, 2)
+  MG_SPLIT_SHAPES(1) MG_SPLIT_SHAPES(2) MG_SPLIT_SHAPES(4)
 #undef MG_SG
   return hipErrorInvalidValue;
 }
 
-size_t split_lds_bytes(int G, uint32_t kl) {
-  return ((size_t)G * (kl - 1) * 80 + (size_t)2 * G * 128) * sizeof(uint4);
+size_t split_lds_bytes(int G, uint32_t kl, int NP) {
+  return (size_t)NP * ((size_t)G * (kl - 1) * 80 + (size_t)2 * G * 128) * sizeof(uint4);
 }
 
-hipError_t split_blocks_per_cu(int S, int G, size_t lds, int* blocks) {
-#define MG_SG(SS, GG)                                                                       \
-  if (S == SS && G == GG)                                                                   \
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_split<SS, GG>, 128, lds);
-  MG_SG(1, 1) MG_SG(1, 2) MG_SG(1, 3) MG_SG(1, 4)
-  MG_SG(2, 1) MG_SG(2, 2) MG_SG(2, 3)
-  MG_SG(4, 1) MG_SG(4, 2)
+hipError_t split_blocks_per_cu(int S, int G, int NP, size_t lds, int* blocks) {
+#define MG_SG(SS, GG, NN)                                                                   \
+  if (S == SS && G == GG && NP == NN)                                                       \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_split<SS, GG, NN>,     \
+                                                        128 * NN, lds);
+  MG_SPLIT_SHAPES(1) MG_SPLIT_SHAPES(2) MG_SPLIT_SHAPES(4)
 #undef MG_SG
   return hipErrorInvalidValue;
 }

@@ -78,11 +78,11 @@ hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_blo
 hipError_t launch_stream(int S, int G, int PK, const StreamParams& p, int waves_per_block,
                          hipStream_t s);
 size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block);
-// The producer/consumer variant of the stream kernel: blocks of one DFS wave
-// and one row-streaming wave; needs lb = 6.
-hipError_t launch_split(int S, int G, const StreamParams& p, hipStream_t s);
-size_t split_lds_bytes(int G, uint32_t kl);
-hipError_t split_blocks_per_cu(int S, int G, size_t lds, int* blocks);
+// The producer/consumer variant of the stream kernel: blocks of NP (1, 2, 4)
+// pairs of a DFS wave and a row-streaming wave; needs lb = 6.
+hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t s);
+size_t split_lds_bytes(int G, uint32_t kl, int NP);
+hipError_t split_blocks_per_cu(int S, int G, int NP, size_t lds, int* blocks);
 // Resident blocks per CU for the given block size and dynamic LDS.
 hipError_t lane_dfs_blocks_per_cu(int chunks, int block_threads, size_t lds, int* blocks);
 hipError_t stream_blocks_per_cu(int S, int G, int PK, int block_threads, size_t lds,

@@ -153,7 +153,7 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chun
     if (kind == MG_KERNEL_LANE_DFS * 100 + (uint32_t)chunks)
       MG_HIP(lane_dfs_blocks_per_cu(chunks, 64 * waves_per_block, lds, &blocks));
     else if (kind / 100 == MG_KERNEL_STREAM_SPLIT)
-      MG_HIP(split_blocks_per_cu(S, G, lds, &blocks));
+      MG_HIP(split_blocks_per_cu(S, G, waves_per_block / 2, lds, &blocks));
     else
       MG_HIP(stream_blocks_per_cu(S, G, PK, 64 * waves_per_block, lds, &blocks));
     dev->occupancy[key] = blocks;
@@ -163,6 +163,10 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chun
 }
 
 int max_keys_per_wave(int S) { return S == 1 ? 4 : (S == 2 ? 3 : 2); }
+
+// Split kernel: the most producer/consumer pairs per CU to keep resident, indexed by pairs
+// per block (1, 2, 4).
+constexpr int kSplitPairsPerCu[5] = {0, 3, 4, 0, 4};
 
 int stream_s(uint32_t chunks) { return chunks <= 64 ? 1 : (chunks <= 128 ? 2 : 4); }
 
@@ -212,7 +216,9 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   if (d < 64 && (n_rows > (1ull << d)))
     return fail(MG_ERR_ROWS, "%llu rows exceed the 2^%u-leaf key domain",
                 (unsigned long long)n_rows, d);
-  if (o->waves_per_block > 4) return fail(MG_ERR_INVALID_ARG, "waves_per_block must be 0-4");
+  if (o->waves_per_block > 8 ||
+      (o->waves_per_block > 4 && o->kernel != MG_KERNEL_STREAM_SPLIT))
+    return fail(MG_ERR_INVALID_ARG, "waves_per_block must be 0-4 (split kernel: 2, 4 or 8)");
   if (o->keys_per_wave > 4) return fail(MG_ERR_INVALID_ARG, "keys_per_wave must be 0-4");
   if (o->lane_depth > 16) return fail(MG_ERR_INVALID_ARG, "lane_depth must be 0-16");
 
@@ -224,8 +230,9 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     // key count at 1 KiB rows (1 / 3 / 8 keys: 5.08 / 5.79 / 11.77 ms against 5.32 / 6.33 /
     // 18.74 ms) and three or more keys at 528-1008-byte rows (768 B, 3 keys: 7.30 against
     // 8.24 ms), but loses one or two keys there (768 B, 1 key: 6.04 against 5.26 ms) and
-    // everywhere below 528 B, where rows pack several to a load. It needs d >= kl + 6.
-    const uint32_t kl_split = o->lane_depth ? o->lane_depth : 6;
+    // everywhere below 528 B, where rows pack several to a load. It needs d >= kl + 6; its
+    // default kl (5 to 7) shrinks to d - 6 on smaller domains.
+    const uint32_t kl_split = o->lane_depth ? o->lane_depth : 5;
     if (d == 0) kernel = MG_KERNEL_POINT;
     else if (pl->chunks <= 8) kernel = MG_KERNEL_LANE_DFS;
     else if (pl->chunks > 32 && pl->chunks <= 64 && (n_keys >= 3 || pl->chunks == 64) &&
@@ -275,9 +282,13 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     kl_default = 6;
     while (kl_default < 10 && (work >> (kl_default + 7)) >= 4096) kl_default++;
   } else if (kernel == MG_KERNEL_STREAM_SPLIT) {
-    // Split kernel, same matrix, three blocks per CU: kl = 6 gives 11.70 / 11.93 / 13.02 /
-    // 14.92 ms for one to four keys per group (kl = 7: 11.42 / 11.94 / 13.21 / 15.08 ms).
-    kl_default = 6;
+    // Split kernel, same matrix: one or two keys per group run best at kl = 7 with one pair
+    // per block (1 key: 11.39 ms; 11.69 ms at kl = 6), three at kl = 6 and four at kl = 5 with
+    // four pairs per block (12.90 and 14.65 ms), whose LDS stacks then still fit.
+    const int g = o->keys_per_wave ? (int)o->keys_per_wave
+                                   : default_keys_per_wave(kernel, stream_s(pl->chunks), n_keys);
+    kl_default = g <= 2 ? 7 : (g == 3 ? 6 : 5);
+    if (d > 6) kl_default = std::min(kl_default, d - 6);
   } else {
     const int g = o->keys_per_wave ? (int)o->keys_per_wave
                                    : default_keys_per_wave(kernel, stream_s(pl->chunks), n_keys);
@@ -297,7 +308,7 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
 
   uint32_t groups = n_keys;
   int resident = 0;
-  int rc;
+  int rc = MG_OK;
   if (kernel == MG_KERNEL_LANE_DFS) {
     if (!o->waves_per_block)
       while (pl->waves_per_block > 1 &&
@@ -315,30 +326,46 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     G = std::max(1, std::min(G, max_keys_per_wave(S)));
     pl->keys_per_wave = (uint32_t)G;
     groups = (n_keys + G - 1) / G;
-    if (!o->waves_per_block) pl->waves_per_block = 1;
-    const size_t lds = stream_lds_bytes(G, pl->kl, (int)pl->waves_per_block);
-    if (lds > dev->max_lds)
-      return fail(MG_ERR_INVALID_ARG, "stream plan needs %zu B of LDS per block (max %zu)", lds,
-                  dev->max_lds);
-    rc = resident_waves(dev, MG_KERNEL_STREAM * 100 + (uint32_t)(S * 8 + G), S, G,
-                        stream_pk(pl->chunks), (int)pl->chunks, (int)pl->waves_per_block, lds,
-                        &resident);
+    if (kernel == MG_KERNEL_STREAM) {
+      if (!o->waves_per_block) pl->waves_per_block = 1;
+      const size_t lds = stream_lds_bytes(G, pl->kl, (int)pl->waves_per_block);
+      if (lds > dev->max_lds)
+        return fail(MG_ERR_INVALID_ARG, "stream plan needs %zu B of LDS per block (max %zu)",
+                    lds, dev->max_lds);
+      rc = resident_waves(dev, MG_KERNEL_STREAM * 100 + (uint32_t)(S * 8 + G), S, G,
+                          stream_pk(pl->chunks), (int)pl->chunks, (int)pl->waves_per_block, lds,
+                          &resident);
+    }
   }
-  if (!rc && kernel == MG_KERNEL_STREAM_SPLIT) {
-    // One block = one DFS wave + one streaming wave; parts count blocks.
+  if (kernel == MG_KERNEL_STREAM_SPLIT) {
+    // A block holds NP pairs of a DFS wave and a streaming wave; parts count pairs.
     const int S = stream_s(pl->chunks);
     const int G = (int)pl->keys_per_wave;
-    pl->waves_per_block = 2;
-    const size_t lds = split_lds_bytes(G, pl->kl);
+    // Default: four pairs per block from three keys per group on, when their LDS fits. A
+    // block's eight waves then put one DFS wave on each SIMD; single-pair blocks, three to a
+    // CU, can stack two DFS waves on one SIMD (78,643,200 x 1 KiB, 3 keys: 12.90 against
+    // 13.06 ms; 4 keys: 14.65 against 15.15 ms). Two pairs per block measured slower than
+    // either. One or two keys per group: single-pair blocks (1 key: 11.39 against 11.51 ms).
+    uint32_t wpb = o->waves_per_block;
+    if (wpb != 2 && wpb != 4 && wpb != 8) {
+      // 0, or a block size meant for the kernel AUTO would otherwise have picked.
+      if (wpb && o->kernel != MG_KERNEL_AUTO)
+        return fail(MG_ERR_INVALID_ARG, "split kernel blocks hold 2, 4 or 8 waves (got %u)", wpb);
+      wpb = G >= 3 && split_lds_bytes(G, pl->kl, 4) <= dev->max_lds ? 8 : 2;
+    }
+    pl->waves_per_block = wpb;
+    const int NP = (int)wpb / 2;
+    const size_t lds = split_lds_bytes(G, pl->kl, NP);
     if (lds > dev->max_lds)
       return fail(MG_ERR_INVALID_ARG, "split plan needs %zu B of LDS per block (max %zu)", lds,
                   dev->max_lds);
     int res_waves = 0;
     rc = resident_waves(dev, MG_KERNEL_STREAM_SPLIT * 100 + (uint32_t)(S * 8 + G), S, G, 1,
-                        (int)pl->chunks, 2, lds, &res_waves);
-    // Three blocks per CU (768 on 256 CUs) stream fastest; a fourth block on some CUs
-    // unbalances them (four keys per group: 14.9 ms at 768 blocks, 26.5 ms at 832).
-    resident = std::min(res_waves / 2, 3 * dev->cus);  // blocks
+                        (int)pl->chunks, (int)wpb, lds, &res_waves);
+    // Three pairs per CU (768 on 256 CUs) stream fastest with single-pair blocks; a fourth
+    // block on some CUs unbalances them (four keys: 14.9 ms at 768 blocks, 26.5 ms at 832).
+    // Four-pair blocks: one per CU.
+    resident = std::min(res_waves / 2, kSplitPairsPerCu[NP] * dev->cus);  // pairs
   }
   if (rc) return rc;
   uint64_t parts = o->waves_per_key;
@@ -392,7 +419,8 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       p.partial = b.partial;
       p.zero_rows = (const uint4*)dev->zero_rows.p;
       if (pl.kernel == MG_KERNEL_STREAM_SPLIT)
-        MG_HIP(launch_split(stream_s(pl.chunks), (int)pl.keys_per_wave, p, s));
+        MG_HIP(launch_split(stream_s(pl.chunks), (int)pl.keys_per_wave,
+                            (int)pl.waves_per_block / 2, p, s));
       else
         MG_HIP(launch_stream(stream_s(pl.chunks), (int)pl.keys_per_wave, stream_pk(pl.chunks), p,
                              (int)pl.waves_per_block, s));

@@ -121,16 +121,59 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
     keys = [p[i % 2] for i, p in enumerate(pairs_for(d, targets, seed=w + 1))]
     ref = oracle.scan_tiled(keys, rows, THREADS)
     max_g = 4 if w <= 1024 else (3 if w <= 2048 else 2)
+    # 2, 4, 8 waves per block = 1, 2, 4 producer/consumer pairs; pairs past the
+    # work and pairs with fewer tiles than the block's others run empty steps.
     for g in range(1, max_g + 1):
         for kl in sorted({1, 3, d - 6}):
-            got, t = scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT,
-                                                            lane_depth=kl, keys_per_wave=g),
-                                       with_timing=True)
-            assert t["kernel"] == "stream_split"
-            assert np.array_equal(got, ref), (g, kl)
+            for wpb in (0, 4, 8):
+                opts = mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=kl,
+                                    keys_per_wave=g, waves_per_block=wpb)
+                # Per pair: g stacks of kl - 1 levels (80 x 16 B each) and two
+                # 128-mask buffers per key; a block gets 160 KiB of LDS.
+                if (wpb or 2) // 2 * (g * (kl - 1) * 80 + 256 * g) * 16 > 160 * 1024:
+                    with pytest.raises(mg.MgError):
+                        scanner.scan_keys(m, keys, opts)
+                    continue
+                got, t = scanner.scan_keys(m, keys, opts, with_timing=True)
+                assert t["kernel"] == "stream_split"
+                assert np.array_equal(got, ref), (g, kl, wpb)
+    # Uneven parts: a waves-per-key count that leaves the last part short.
+    for wpb in (2, 8):
+        got = scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=1,
+                                                     waves_per_key=7, waves_per_block=wpb))
+        assert np.array_equal(got, ref), wpb
+    for bad in (1, 3, 6):
+        with pytest.raises(mg.MgError):
+            scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT,
+                                                    waves_per_block=bad))
     # Too shallow a domain for 64 live lanes.
     with pytest.raises(mg.MgError):
         scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=d - 5))
+    m.close()
+
+
+# Kernel AUTO picks: lane-DFS up to 128-byte rows; the split kernel for
+# 528-1024-byte rows with >= 3 keys or 1024-byte rows, when d >= 11; the
+# stream kernel otherwise. Small domains shrink the split kernel's lane depth.
+@pytest.mark.parametrize("n,w,d,q,kernel", [
+    (3000, 1024, 12, 1, "stream_split"), (2000, 1024, 11, 1, "stream_split"),
+    (1000, 1024, 10, 1, "stream"), (3000, 768, 12, 3, "stream_split"),
+    (3000, 768, 12, 2, "stream"), (3000, 512, 12, 3, "stream"), (3000, 128, 12, 3, "lane_dfs"),
+    (3000, 2048, 12, 3, "stream"), (9000, 1024, 14, 3, "stream_split"),
+    (9000, 1024, 14, 4, "stream_split"), (9000, 1024, 14, 8, "stream_split")])
+def test_auto_kernel_choice(mg, scanner, n, w, d, q, kernel):
+    print(f"case auto n={n} w={w} d={d} q={q}", flush=True)
+    rows = oracle.fill_rows(seed=n + w + q, first_row=0, n_rows=n, row_bytes=w)
+    m = upload(mg, scanner, rows)
+    keys = [p[i % 2] for i, p in enumerate(pairs_for(d, [(n * (i + 1)) // (q + 1)
+                                                         for i in range(q)], seed=d))]
+    ref = oracle.scan_tiled(keys, rows, THREADS)
+    got, t = scanner.scan_keys(m, keys, with_timing=True)
+    assert t["kernel"] == kernel
+    assert np.array_equal(got, ref)
+    # A block size given for another kernel does not stop AUTO's choice.
+    got = scanner.scan_keys(m, keys, mg.make_opts(waves_per_block=1))
+    assert np.array_equal(got, ref)
     m.close()
 
 
