@@ -12,6 +12,9 @@
 //                    per wave; leaf masks go through LDS and each row is read
 //                    once, coalesced, for all G keys (PK rows per load when a
 //                    row has at most 64 / PK chunks).
+// scan_split<S,G,NP> the stream kernel's work split between NP pairs of
+//                    waves per block: a DFS wave makes the masks, a streaming
+//                    wave folds the rows (528 B - 1 KiB rows, several keys).
 // scan_point         point-eval fallback (d = 0, tiny inputs) and the delta
 //                    scan: one lane per row, full path walk per row.
 // reduce_partials    XOR of per-wave partial answers into the output.
@@ -337,15 +340,18 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
 
   // Rows of one leaf-pair step: 2 per live source lane s, tile_base + s*2^kl +
   // 2*pair + {0,1}; row r of the step uses mask slot r. They are read in
-  // batches of U rows (IU load instructions: 8 KiB of 1 KiB rows), two batches
-  // in flight: the first two batches of a step are issued before that step's
-  // PRG work, so loads overlap the ChaCha rounds.
+  // batches of U rows (IU load instructions: 8 KiB of 1 KiB rows), NBUF
+  // batches in flight: the first NBUF batches of a step are issued before
+  // that step's PRG work, so loads overlap the ChaCha rounds. Packed narrow
+  // rows with at most two keys keep four batches, a whole 128-row step, in
+  // flight (the registers allow it there; wide rows spill with four).
   // Loads are branch-free: a row's base address is wave-uniform (a row past
   // the step or the matrix reads the zero buffer instead), and each lane's
   // chunk offset is fixed for the whole kernel. A lane past the row's last
   // chunk re-reads chunk 0; its accumulator is never written out.
   constexpr int IU = 8 / S;   // load instructions per batch
   constexpr int U = IU * PK;  // rows per batch
+  constexpr int NBUF = 2;
   const uint32_t step_rows = 2u * (uint32_t)lanes_live;
   uint32_t lane_off[S];
 #pragma unroll
@@ -368,7 +374,7 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
       row_off[i][j] =
           (uint32_t)(((((uint64_t)(q >> 1) << kl) + (q & 1)) * C + lane_off[j]) * 16);
     }
-  uint4 va[IU][S], vb[IU][S];
+  uint4 ring[NBUF][IU][S];
   auto load_batch = [&](uint64_t tile_base, uint32_t pair, uint32_t r0, uint4 (&v)[IU][S],
                         bool whole_tile) {
     if (whole_tile) {
@@ -404,8 +410,10 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
     }
   };
   auto prefetch_step = [&](uint64_t tile_base, uint32_t pair, bool whole_tile) {
-    load_batch(tile_base, pair, 0, va, whole_tile);
-    if (U < step_rows) load_batch(tile_base, pair, U, vb, whole_tile);
+#pragma unroll
+    for (int k = 0; k < NBUF; k++)
+      if (k == 0 || (uint32_t)k * U < step_rows)
+        load_batch(tile_base, pair, (uint32_t)k * U, ring[k], whole_tile);
   };
 
   for (uint64_t tile = t_begin; tile < t_end; ++tile) {
@@ -441,13 +449,17 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
         ++lvl;
         continue;
       }
-      // va / vb already hold batches 0 and 1 of this step.
-      for (uint32_t r0 = 0; r0 < step_rows; r0 += 2 * U) {
-        fold_batch(r0, va);
-        if (r0 + 2 * U < step_rows) load_batch(tile_base, pair, r0 + 2 * U, va, whole_tile);
-        if (r0 + U >= step_rows) break;
-        fold_batch(r0 + U, vb);
-        if (r0 + 3 * U < step_rows) load_batch(tile_base, pair, r0 + 3 * U, vb, whole_tile);
+      // ring[k] already holds batch k of this step; batch b + NBUF takes the
+      // slot that batch b frees.
+      for (uint32_t r0 = 0; r0 < step_rows; r0 += NBUF * U) {
+#pragma unroll
+        for (int k = 0; k < NBUF; k++) {
+          const uint32_t rk = r0 + (uint32_t)k * U;
+          if (k > 0 && rk >= step_rows) break;
+          fold_batch(rk, ring[k]);
+          if (rk + NBUF * U < step_rows)
+            load_batch(tile_base, pair, rk + NBUF * U, ring[k], whole_tile);
+        }
       }
       if (++pair == n_pairs) break;
       prefetch_step(tile_base, pair, whole_tile);
