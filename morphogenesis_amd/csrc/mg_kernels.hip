@@ -340,11 +340,13 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
 
   // Rows of one leaf-pair step: 2 per live source lane s, tile_base + s*2^kl +
   // 2*pair + {0,1}; row r of the step uses mask slot r. They are read in
-  // batches of U rows (IU load instructions: 8 KiB of 1 KiB rows), NBUF
-  // batches in flight: the first NBUF batches of a step are issued before
-  // that step's PRG work, so loads overlap the ChaCha rounds. Packed narrow
-  // rows with at most two keys keep four batches, a whole 128-row step, in
-  // flight (the registers allow it there; wide rows spill with four).
+  // batches of U rows (IU load instructions: 8 KiB of 1 KiB rows), NBUF = 2
+  // batches in flight: the first two batches of a step are issued before
+  // that step's PRG work, so loads overlap the ChaCha rounds. Three or four
+  // batches keep more of a step in flight but hold their registers through
+  // the ChaCha rounds: with four, one wave fits per SIMD instead of two, and
+  // 144-byte rows with one key slow from 9.1 to 11.5 ms (three: 8.9 ms for
+  // one key, 15.9 against 14.4 ms for two).
   // Loads are branch-free: a row's base address is wave-uniform (a row past
   // the step or the matrix reads the zero buffer instead), and each lane's
   // chunk offset is fixed for the whole kernel. A lane past the row's last
