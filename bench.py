@@ -26,9 +26,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "batched queries/s + p50/p99 ms at 2^31x32B DB; scan GB/s vs HBM peak"
 KEYS_PER_QUERY = 3
-# MI355X published peaks: HBM3E 8 TB/s; int32 VALU = 256 CU x 64 lanes x 2.4 GHz.
+# MI355X peaks: HBM3E 8 TB/s (published). int32 VALU: 256 CUs x 128 lanes per
+# clock x 2.4 GHz = 78.6 TOP/s, the rate of the published 157.3 TFLOP/s FP32
+# vector peak counted per op; simple int32 ops (v_add_u32, v_xor_b32) reach it:
+# the microbenchmark's independent add/xor chains run at 0.74 of it at the
+# 2.1 GHz clock they hold (python -m morphogenesis_amd.microbench --forms).
+# Rotates (v_alignbit_b32), a third of ChaCha's ops, issue at 64 lanes per
+# clock, so the ChaCha mix (2 of 3 ops at 128, 1 at 64 lanes per clock) is
+# bounded at 96 lanes per clock: 59.0 TOP/s, reported beside the peak.
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
+VALU_MIX_PEAK_TOPS = 256 * 96 * 2.4e9 / 1e12
 # Algorithmic int32 ops per leaf: one GGM node (2 ChaCha8 blocks after dead-code
 # elimination, correction word included) = 700, plus one v_bitop3 per 32-bit
 # word of the row for the masked fold.
@@ -290,7 +298,8 @@ def main():
     # integer VALU for 32-byte rows, HBM for wide rows at a few keys per pass.
     valu_side = {"achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
                  "frac": valu_tops / VALU_PEAK_TOPS, "ops_per_leaf": OPS_PER_NODE + w // 4,
-                 "leaves_per_launch": leaves}
+                 "leaves_per_launch": leaves,
+                 "mix_peak": VALU_MIX_PEAK_TOPS, "mix_frac": valu_tops / VALU_MIX_PEAK_TOPS}
     hbm_side = {"achieved": scan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": scan_gbs / HBM_PEAK_GBS, "bytes_per_launch": n_rows * w}
     if alg_ops / (VALU_PEAK_TOPS * 1e12) >= n_rows * w / (HBM_PEAK_GBS * 1e9):

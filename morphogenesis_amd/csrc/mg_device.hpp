@@ -72,6 +72,129 @@ __device__ __forceinline__ void prg_expand(uint32_t s0, uint32_t s1,
   rt = (y1 + kC1) & 1u;
 }
 
+// The same expansion with rounds 2-8 issued in a fixed order: each of the 12
+// stages of a quarter-round is issued for all eight quarter-rounds of the
+// round (4 of each block) before the next stage, so consecutive instructions
+// never depend on each other. gfx950 issues v_add_u32 / v_xor_b32 faster than
+// one wave instruction per 4 cycles only when the next instruction does not
+// wait for them; the compiler's own order keeps a quarter-round's dependent
+// ops close together. Each op is its own asm statement (volatile asm keeps
+// the order) in SSA form, so words the two blocks still share need no copy.
+// Round 1 stays in C: there the two blocks share three column quarter-rounds
+// and the compiler folds the zero words. In round 8 block 1 keeps only what
+// its words 0 and 1 need.
+__device__ __forceinline__ uint32_t op_add(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm volatile("v_add_u32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ uint32_t op_xor(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm volatile("v_xor_b32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+// rotl(x, 32 - N) = alignbit(x, x, N)
+#define MG_ROTR_ASM(N)                                                         \
+  __device__ __forceinline__ uint32_t op_rotr##N(uint32_t a) {                 \
+    uint32_t r;                                                                \
+    asm volatile("v_alignbit_b32 %0, %1, %1, " #N : "=v"(r) : "v"(a));         \
+    return r;                                                                  \
+  }
+MG_ROTR_ASM(16)
+MG_ROTR_ASM(20)
+MG_ROTR_ASM(24)
+MG_ROTR_ASM(25)
+#undef MG_ROTR_ASM
+
+// Stages 1..LAST of NQ quarter-rounds (a[q], b[q], c[q], d[q]) in op-major
+// order.
+template <int NQ, int LAST>
+__device__ __forceinline__ void qr_stages(uint32_t* a[], uint32_t* b[], uint32_t* c[],
+                                          uint32_t* d[]) {
+#define MG_STAGE(K, EXPR)                                                      \
+  if (LAST >= K) {                                                             \
+    _Pragma("unroll") for (int q = 0; q < NQ; q++) { EXPR; }                   \
+  }
+  MG_STAGE(1, *a[q] = op_add(*a[q], *b[q]))
+  MG_STAGE(2, *d[q] = op_xor(*d[q], *a[q]))
+  MG_STAGE(3, *d[q] = op_rotr16(*d[q]))
+  MG_STAGE(4, *c[q] = op_add(*c[q], *d[q]))
+  MG_STAGE(5, *b[q] = op_xor(*b[q], *c[q]))
+  MG_STAGE(6, *b[q] = op_rotr20(*b[q]))
+  MG_STAGE(7, *a[q] = op_add(*a[q], *b[q]))
+  MG_STAGE(8, *d[q] = op_xor(*d[q], *a[q]))
+  MG_STAGE(9, *d[q] = op_rotr24(*d[q]))
+  MG_STAGE(10, *c[q] = op_add(*c[q], *d[q]))
+  MG_STAGE(11, *b[q] = op_xor(*b[q], *c[q]))
+  MG_STAGE(12, *b[q] = op_rotr25(*b[q]))
+#undef MG_STAGE
+}
+
+__device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uint32_t s2,
+                                                   uint32_t s3, uint32_t l[4], uint32_t r[4],
+                                                   uint32_t& lt, uint32_t& rt) {
+  uint32_t x0 = kC0, x1 = kC1, x2 = kC2, x3 = kC3;
+  uint32_t x4 = s0, x5 = s1, x6 = s2, x7 = s3;
+  uint32_t x8 = s0, x9 = s1, x10 = s2, x11 = s3;
+  uint32_t x12 = 0, x13 = 0, x14 = 0, x15 = 0;
+  uint32_t y0 = kC0, y4 = s0, y8 = s0, y12 = 1;
+  // Round 1 (columns): block 1 differs from block 0 only in column 0.
+  MG_QR(x0, x4, x8, x12);
+  MG_QR(x1, x5, x9, x13);
+  MG_QR(x2, x6, x10, x14);
+  MG_QR(x3, x7, x11, x15);
+  MG_QR(y0, y4, y8, y12);
+  uint32_t y1 = x1, y5 = x5, y9 = x9, y13 = x13;
+  uint32_t y2 = x2, y6 = x6, y10 = x10, y14 = x14;
+  uint32_t y3 = x3, y7 = x7, y11 = x11, y15 = x15;
+  // Rounds 2..7: diagonal, column, diagonal, column, diagonal, column.
+#pragma unroll
+  for (int rd = 2; rd <= 7; rd++) {
+    if (rd % 2 == 0) {
+      uint32_t* a[8] = {&x0, &x1, &x2, &x3, &y0, &y1, &y2, &y3};
+      uint32_t* b[8] = {&x5, &x6, &x7, &x4, &y5, &y6, &y7, &y4};
+      uint32_t* c[8] = {&x10, &x11, &x8, &x9, &y10, &y11, &y8, &y9};
+      uint32_t* d[8] = {&x15, &x12, &x13, &x14, &y15, &y12, &y13, &y14};
+      qr_stages<8, 12>(a, b, c, d);
+    } else {
+      uint32_t* a[8] = {&x0, &x1, &x2, &x3, &y0, &y1, &y2, &y3};
+      uint32_t* b[8] = {&x4, &x5, &x6, &x7, &y4, &y5, &y6, &y7};
+      uint32_t* c[8] = {&x8, &x9, &x10, &x11, &y8, &y9, &y10, &y11};
+      uint32_t* d[8] = {&x12, &x13, &x14, &x15, &y12, &y13, &y14, &y15};
+      qr_stages<8, 12>(a, b, c, d);
+    }
+  }
+  // Round 8 (diagonal): block 0 in full; block 1's quarter-rounds through
+  // words 0 and 1 up to stage 7, which sets them.
+  {
+    uint32_t* a[6] = {&x0, &x1, &x2, &x3, &y0, &y1};
+    uint32_t* b[6] = {&x5, &x6, &x7, &x4, &y5, &y6};
+    uint32_t* c[6] = {&x10, &x11, &x8, &x9, &y10, &y11};
+    uint32_t* d[6] = {&x15, &x12, &x13, &x14, &y15, &y12};
+    qr_stages<6, 7>(a, b, c, d);
+    uint32_t* a4[4] = {&x0, &x1, &x2, &x3};
+    uint32_t* b4[4] = {&x5, &x6, &x7, &x4};
+    uint32_t* c4[4] = {&x10, &x11, &x8, &x9};
+    uint32_t* d4[4] = {&x15, &x12, &x13, &x14};
+    // Stages 8..12 of block 0: run stages 1..12 would redo 1..7, so the tail
+    // is written out.
+#pragma unroll
+    for (int q = 0; q < 4; q++) *d4[q] = op_xor(*d4[q], *a4[q]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) *d4[q] = op_rotr24(*d4[q]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) *c4[q] = op_add(*c4[q], *d4[q]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) *b4[q] = op_xor(*b4[q], *c4[q]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) *b4[q] = op_rotr25(*b4[q]);
+  }
+  l[0] = x0 + kC0; l[1] = x1 + kC1; l[2] = x2 + kC2; l[3] = x3 + kC3;
+  r[0] = x4 + s0; r[1] = x5 + s1; r[2] = x6 + s2; r[3] = x7 + s3;
+  lt = (y0 + kC0) & 1u;
+  rt = (y1 + kC1) & 1u;
+}
+
 // All-ones if t == 1, else zero, kept opaque to the optimiser so that
 // `x ^ (uniform & m)` stays one v_bitop3_b32 with the uniform operand in an
 // SGPR, instead of being rewritten as a select of the uniform value (which
@@ -82,14 +205,22 @@ __device__ __forceinline__ uint32_t t_mask(uint32_t t) {
   return m;
 }
 
+// Which expansion the kernels use (the ordered one issues rounds 2-8 in a
+// fixed op-major order).
+constexpr bool kPrgOrdered = true;
+
 // Both children of `n` at tree level `level`, correction word applied when
 // n.t == 1 (dpf.rs:221-239). `k` is wave-uniform, so the correction word is
 // read with scalar loads.
+template <bool kOrdered = kPrgOrdered>
 __device__ __forceinline__ void expand_node(const DevKey* __restrict__ k,
                                             int level, const Node& n,
                                             Node& left, Node& right) {
   uint32_t l[4], r[4], lt, rt;
-  prg_expand(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
+  if constexpr (kOrdered)
+    prg_expand_ordered(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
+  else
+    prg_expand(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
   const uint32_t m = t_mask(n.t);
   const uint32_t c0 = k->cw_seed[level][0], c1 = k->cw_seed[level][1];
   const uint32_t c2 = k->cw_seed[level][2], c3 = k->cw_seed[level][3];

@@ -1,0 +1,69 @@
+"""The PRG microbenchmark computes the real node chain: its checksum equals the
+oracle's for a small launch, so the timed loop is the PRG and not something
+the compiler thinned out."""
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+M32 = 0xFFFFFFFF
+
+
+def prg_checksum_reference(blocks, iters):
+    """The kernel's checksum from the oracle's PRG expand. Lane gid starts at
+    seed (gid * 0x9E3779B9, gid ^ 0x85EBCA6B, gid + 0xC2B2AE35, ~gid), t =
+    gid & 1, and follows left children; the key's correction words are
+    cw_seed[l][w] = 0x01234567 * (4l + w + 1), both control bits set, applied
+    at level i & 15 when t == 1. The checksum XORs every right child and each
+    lane's last node."""
+    total = 0
+    for gid in range(blocks * 256):
+        s = [(gid * 0x9E3779B9) & M32, gid ^ 0x85EBCA6B, (gid + 0xC2B2AE35) & M32, ~gid & M32]
+        t = gid & 1
+        acc = 0
+        for i in range(iters):
+            lvl = i & 15
+            left, right, lt, rt = oracle.prg_expand(s)
+            left = [int(x) for x in left]
+            right = [int(x) for x in right]
+            if t == 1:
+                cw = [(0x01234567 * (lvl * 4 + w + 1)) & M32 for w in range(4)]
+                left = [a ^ c for a, c in zip(left, cw)]
+                right = [a ^ c for a, c in zip(right, cw)]
+                lt ^= 1
+                rt ^= 1
+            acc ^= right[0] ^ right[1] ^ right[2] ^ right[3] ^ rt
+            s, t = left, lt
+        total ^= acc ^ s[0] ^ s[1] ^ s[2] ^ s[3] ^ t
+    return total
+
+
+def test_prg_microbench_matches_oracle():
+    from morphogenesis_amd import microbench
+    for iters in (1, 5, 17):
+        want = prg_checksum_reference(1, iters)
+        ms, chk = microbench.prg(1, iters)
+        assert ms > 0
+        assert chk == want, iters
+        # Both expansions: compiler-scheduled and op-major ordered.
+        for ordered in (0, 1):
+            assert microbench.prg_variant(ordered, 1, iters)[1] == want, (iters, ordered)
+
+
+def test_valu_microbench_matches_its_definition():
+    from morphogenesis_amd import microbench
+    # x -> (x + y) ^ y per chain and iteration, y = gid * 0x9E3779B9.
+    iters = 3
+    ms, chk = microbench.valu(1, iters)
+    want = 0
+    for gid in range(256):
+        y = (gid * 0x9E3779B9) & M32
+        acc = 0
+        for j in range(8):
+            x = gid + j
+            for _ in range(iters):
+                x = ((x + y) & M32) ^ y
+            acc ^= x
+        want ^= acc
+    assert chk == want
