@@ -141,9 +141,14 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   const DevKey* __restrict__ k = p.keys + (p.key_idx ? p.key_idx[key_i] : key_i);
   const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
 
-  // Per-wave DFS stack: right children waiting at each subtree level.
-  uint4* st_seed = lds + (size_t)wib * kl * 80;  // [kl][64] seeds
-  uint32_t* st_t = reinterpret_cast<uint32_t*>(st_seed + kl * 64);  // [kl][64]
+  // Per-wave DFS stack: the right children waiting at depths 1 .. kl-1 of
+  // the lane subtree (depth 0, the lane root, is never pushed). Seeds sit in
+  // LDS, [kl-1][64]; each child's control byte is kept as the one bit that
+  // matters, t == 1 (dpf.rs:221-239 and 136-141 test nothing else), in bit
+  // `depth` of a per-lane register. At kl = 10 a wave needs 9 KiB of LDS, so
+  // four 4-wave blocks fit per CU.
+  uint4* st_seed = lds + (size_t)wib * (kl - 1) * 64;  // [kl-1][64] seeds
+  uint32_t st_tbits = 0;
 
   uint4 acc[C];
 #pragma unroll
@@ -173,11 +178,11 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
     // (or of an idle lane) is read from a zero buffer instead.
     int lvl = 0;
     for (uint32_t pair = 0; pair < n_pairs; ++pair) {
-      if (pair) {
+      if (pair) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
         lvl = kl - 1 - __builtin_ctz(pair);
-        const uint4 sv = st_seed[lvl * 64 + lane];
+        const uint4 sv = st_seed[(lvl - 1) * 64 + lane];
         node.s0 = sv.x; node.s1 = sv.y; node.s2 = sv.z; node.s3 = sv.w;
-        node.t = st_t[lvl * 64 + lane];
+        node.t = (st_tbits >> lvl) & 1u;
       }
       const uint64_t x = base + 2 * (uint64_t)pair;
       const uint4* src0 = (lane_active && x < p.n_rows) ? p.rows + x * C : p.zero_rows;
@@ -191,8 +196,9 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
       for (; lvl < kl - 1; ++lvl) {
         Node L, R;
         expand_node(k, leaf_level + lvl, node, L, R);
-        st_seed[(lvl + 1) * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);
-        st_t[(lvl + 1) * 64 + lane] = R.t;
+        st_seed[lvl * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth lvl + 1
+        const uint32_t bit = 1u << (lvl + 1);
+        st_tbits = R.t == 1u ? (st_tbits | bit) : (st_tbits & ~bit);
         node = L;
       }
       Node L, R;
@@ -876,12 +882,16 @@ static int grid_for(uint64_t items, int block) {
   return (int)g;
 }
 
+size_t lane_dfs_lds_bytes(int waves_per_block, uint32_t kl) {
+  return (size_t)waves_per_block * (kl - 1) * 64 * sizeof(uint4);
+}
+
 hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_block,
                            hipStream_t s) {
   const uint32_t total = p.parts * p.n_keys;
   const dim3 block(64 * waves_per_block);
   const dim3 grid(xcd_grid(total, waves_per_block));
-  const size_t lds = (size_t)waves_per_block * p.kl * 80 * sizeof(uint4);
+  const size_t lds = lane_dfs_lds_bytes(waves_per_block, p.kl);
   switch (chunks) {
 #define MG_CASE(C)                                                            \
   case C:                                                                     \

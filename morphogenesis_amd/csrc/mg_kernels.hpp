@@ -75,6 +75,8 @@ struct SuffixParams {
 hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s);
 hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_block,
                            hipStream_t s);
+// Dynamic LDS of a lane-DFS block: (kl - 1) levels of 64 seeds per wave.
+size_t lane_dfs_lds_bytes(int waves_per_block, uint32_t kl);
 hipError_t launch_stream(int S, int G, int PK, const StreamParams& p, int waves_per_block,
                          hipStream_t s);
 size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block);

@@ -216,12 +216,12 @@ __global__ __launch_bounds__(256) void fresh_bench(uint32_t iters, uint32_t* __r
 #define MGB_ADD(x, y) asm volatile("v_add_u32 %0, %0, %1" : "+v"(x) : "v"(y))
 #define MGB_XOR(x, y) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(x) : "v"(y))
 #define MGB_ROT(x, n) asm volatile("v_alignbit_b32 %0, %0, %0, " #n : "+v"(x))
-template <int ORDER>
+template <int ORDER, int NQ = 8>
 __global__ __launch_bounds__(256) void qr_bench(uint32_t iters, uint32_t* __restrict__ out) {
   const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
-  uint32_t a[8], b[8], c[8], d[8];
+  uint32_t a[NQ], b[NQ], c[NQ], d[NQ];
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
+  for (int q = 0; q < NQ; q++) {
     a[q] = gid + (uint32_t)q;
     b[q] = gid * 3u + (uint32_t)q;
     c[q] = gid * 5u + (uint32_t)q;
@@ -230,32 +230,32 @@ __global__ __launch_bounds__(256) void qr_bench(uint32_t iters, uint32_t* __rest
   for (uint32_t i = 0; i < iters; ++i) {
     if constexpr (ORDER == 0) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_ADD(a[q], b[q]);
+      for (int q = 0; q < NQ; q++) MGB_ADD(a[q], b[q]);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_XOR(d[q], a[q]);
+      for (int q = 0; q < NQ; q++) MGB_XOR(d[q], a[q]);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_ROT(d[q], 16);
+      for (int q = 0; q < NQ; q++) MGB_ROT(d[q], 16);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_ADD(c[q], d[q]);
+      for (int q = 0; q < NQ; q++) MGB_ADD(c[q], d[q]);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_XOR(b[q], c[q]);
+      for (int q = 0; q < NQ; q++) MGB_XOR(b[q], c[q]);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_ROT(b[q], 20);
+      for (int q = 0; q < NQ; q++) MGB_ROT(b[q], 20);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_ADD(a[q], b[q]);
+      for (int q = 0; q < NQ; q++) MGB_ADD(a[q], b[q]);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_XOR(d[q], a[q]);
+      for (int q = 0; q < NQ; q++) MGB_XOR(d[q], a[q]);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_ROT(d[q], 24);
+      for (int q = 0; q < NQ; q++) MGB_ROT(d[q], 24);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_ADD(c[q], d[q]);
+      for (int q = 0; q < NQ; q++) MGB_ADD(c[q], d[q]);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_XOR(b[q], c[q]);
+      for (int q = 0; q < NQ; q++) MGB_XOR(b[q], c[q]);
 #pragma unroll
-      for (int q = 0; q < 8; q++) MGB_ROT(b[q], 25);
+      for (int q = 0; q < NQ; q++) MGB_ROT(b[q], 25);
     } else {
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < NQ; q++) {
         MGB_ADD(a[q], b[q]); MGB_XOR(d[q], a[q]); MGB_ROT(d[q], 16);
         MGB_ADD(c[q], d[q]); MGB_XOR(b[q], c[q]); MGB_ROT(b[q], 20);
         MGB_ADD(a[q], b[q]); MGB_XOR(d[q], a[q]); MGB_ROT(d[q], 24);
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void qr_bench(uint32_t iters, uint32_t* __rest
   }
   uint32_t r = 0;
 #pragma unroll
-  for (int q = 0; q < 8; q++) r ^= a[q] ^ b[q] ^ c[q] ^ d[q];
+  for (int q = 0; q < NQ; q++) r ^= a[q] ^ b[q] ^ c[q] ^ d[q];
   out[gid] = r;
 }
 #undef MGB_ADD
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void qr_bench(uint32_t iters, uint32_t* __rest
 
 constexpr int kOpForms = 14;
 // mgb_mix forms: (K, MIX).
-constexpr int kMixForms = 18;
+constexpr int kMixForms = 19;
 
 }  // namespace mg
 
@@ -389,7 +389,8 @@ int mgb_op(int device, uint32_t op, uint32_t blocks, uint32_t iters, double* ms,
 // also 48 adds per iteration. Forms 12 and 13: forms 6 and 9 with the loop
 // body unrolled 15 times (720 instructions, the size of the PRG loop); iters
 // must be a multiple of 15. Forms 14 and 15: fresh_bench<0>, <1>. Forms 16 and
-// 17: qr_bench<0>, <1> (iters / 2 iterations of 96).
+// 17: qr_bench<0>, <1> (iters / 2 iterations of 96). Form 18: 16 quarter-rounds
+// op-major (iters / 4 iterations of 192).
 int mgb_mix(int device, uint32_t form, uint32_t blocks, uint32_t iters, double* ms,
             uint32_t* checksum) {
   if (!ms || !checksum || !blocks || form >= (uint32_t)mg::kMixForms)
@@ -418,6 +419,8 @@ int mgb_mix(int device, uint32_t form, uint32_t blocks, uint32_t iters, double* 
           // 96 instructions per qr_bench iteration: half the iterations.
           case 16: hipLaunchKernelGGL((mg::qr_bench<0>), g, b, 0, s, iters / 2, out); break;
           case 17: hipLaunchKernelGGL((mg::qr_bench<1>), g, b, 0, s, iters / 2, out); break;
+          // 16 quarter-rounds op-major: 192 instructions per iteration.
+          case 18: hipLaunchKernelGGL((mg::qr_bench<0, 16>), g, b, 0, s, iters / 4, out); break;
         }
         return hipGetLastError();
       },

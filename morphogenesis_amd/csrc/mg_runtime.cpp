@@ -197,6 +197,90 @@ void split_levels(uint32_t d, uint32_t kl_req, Plan* pl) {
   pl->tb = d - kl - lb;
 }
 
+// Lane-DFS launch shape. Blocks of `wpb` waves are dealt to the CUs evenly and
+// their waves to the 4 SIMDs, so the most loaded SIMD runs ceil(waves / SIMDs)
+// waves of `tiles_per_wave` tiles. A tile costs each lane 2^kl leaf
+// expansions plus about 8 more (its lb-level path to the lane root and the
+// tile prefix). The SIMD's rate depends on how many waves share it (the
+// ChaCha node rate at 1 .. 8 waves per SIMD, measured: 0.754, 0.918, 0.943,
+// 0.967, 0.978, 0.989, 0.995, 1.0 of the 8-wave rate). The shape minimising
+// waves x tiles x (2^kl + 8) / rate on the most loaded SIMD wins; kl is
+// searched when `kl_fixed` is 0, tiles per wave when `parts_fixed` is 0.
+// 2^27 x 32 B, 3 keys: kl = 9 as 1366 parts x 3 tiles gives 4098 waves, one
+// CU takes a fifth block, and the pass runs 9.43 ms against 7.35 ms for kl = 7
+// as 6144 waves of 8 tiles, which this picks.
+struct LaneShape {
+  uint32_t kl = 0;
+  uint32_t wpb = 0;
+  uint64_t tiles_per_wave = 0;
+  double cost = 0;
+};
+
+int choose_lane_shape(mg_device* dev, const Plan& pl0, uint32_t d, uint32_t n_keys, uint32_t kl_fixed,
+                      uint32_t wpb_req, uint64_t parts_fixed, LaneShape* out) {
+  static const double kRate[9] = {0, 0.754, 0.918, 0.943, 0.967, 0.978, 0.989, 0.995, 1.0};
+  const uint64_t simds = (uint64_t)dev->cus * 4;
+  LaneShape best;
+  const uint32_t kl_lo = kl_fixed ? std::min(kl_fixed, d) : 1;
+  const uint32_t kl_hi = kl_fixed ? kl_lo : std::min(10u, d);
+  for (uint32_t kl = kl_lo; kl <= kl_hi; ++kl) {
+    const uint32_t lb = std::min(6u, d - kl);
+    const uint64_t tile_rows = 1ull << (kl + lb);
+    // A range start off this depth's tiles rules it out; a fixed depth is kept, so that the
+    // plan's own alignment check reports it.
+    if (!kl_fixed && pl0.row_begin % tile_rows != 0) continue;
+    const uint64_t n_tiles = std::max<uint64_t>(
+        1, (pl0.row_end + tile_rows - 1) / tile_rows - pl0.row_begin / tile_rows);
+    uint32_t wpb = wpb_req;
+    while (wpb > 1 && lane_dfs_lds_bytes((int)wpb, kl) > dev->max_lds) wpb--;
+    const size_t lds = lane_dfs_lds_bytes((int)wpb, kl);
+    if (lds > dev->max_lds) continue;
+    int slots = 0;
+    int rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl0.chunks, 0, 0, 1, (int)pl0.chunks,
+                            (int)wpb, lds, &slots);
+    if (rc) return rc;
+    const uint64_t cap = std::max<uint64_t>((uint64_t)slots, 1);
+    // Candidate waves per key: the most that stay resident and a few fewer, the counts that
+    // fill a whole number of waves per SIMD, and those of the smallest tiles per wave.
+    uint64_t cand[32];
+    int nc = 0;
+    if (parts_fixed) {
+      cand[nc++] = parts_fixed;
+    } else {
+      const uint64_t p_hi = std::max<uint64_t>(1, cap / n_keys);
+      for (uint64_t j = 0; j < 9 && j < p_hi; ++j) cand[nc++] = p_hi - j;
+      for (uint64_t m = 1; m <= 8; ++m)
+        if (m * simds >= n_keys) cand[nc++] = m * simds / n_keys;
+      const uint64_t t_lo = std::max<uint64_t>(1, (n_tiles * n_keys + cap - 1) / cap);
+      for (uint64_t t = t_lo; t < t_lo + 8; ++t) cand[nc++] = (n_tiles + t - 1) / t;
+    }
+    for (int ci = 0; ci < nc; ++ci) {
+      const uint64_t p0 = std::max<uint64_t>(1, std::min(cand[ci], n_tiles));
+      const uint64_t tpw = (n_tiles + p0 - 1) / p0;
+      const uint64_t parts = (n_tiles + tpw - 1) / tpw;
+      const uint64_t waves = parts * n_keys;
+      // Waves beyond the resident ones run in later rounds.
+      const uint64_t rounds = (waves + cap - 1) / cap;
+      const uint64_t per_round = (waves + rounds - 1) / rounds;
+      const uint64_t wps = (per_round + simds - 1) / simds;
+      const double rate = kRate[std::min<uint64_t>(wps, 8)];
+      const double cost = (double)rounds * (double)wps * (double)tpw *
+                          (double)((1ull << kl) + 8) / rate;
+      if (best.kl == 0 || cost < best.cost * 0.999) {
+        best.kl = kl;
+        best.wpb = wpb;
+        best.tiles_per_wave = tpw;
+        best.cost = cost;
+      }
+    }
+  }
+  if (best.kl == 0)
+    return fail(MG_ERR_ROWS, "row_begin %llu is not a multiple of any lane-DFS tile",
+                (unsigned long long)pl0.row_begin);
+  *out = best;
+  return MG_OK;
+}
+
 }  // namespace
 
 int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_bytes,
@@ -269,18 +353,22 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     pl->parts = (uint32_t)std::max<uint64_t>(n_tiles, 1);
     return MG_OK;
   }
-  // Default lane depth (measured on MI355X): lane-DFS wants deep lane subtrees (kl up to 10)
-  // as long as the keys still have at least ~4096 tiles of 2^(kl+6) rows between them to spread
-  // over the resident waves. The stream kernel at 78,643,200 x 1 KiB: one key per wave runs best
-  // at kl = 4 (11.97 ms; 12.08 ms at kl = 6), two keys per wave at kl = 6 (13.04 ms; 13.14 ms
-  // at kl = 7), three at kl = 4 to 7 alike (15.02 to 15.13 ms). Narrower rows carry more PRG
-  // work per byte and want fewer lane-root walks: 32 GiB of 144-byte rows, one key, 10.4 ms at
-  // kl = 4 and 9.2 ms at kl = 6; 272-byte rows 7.04 / 6.66 ms; 512-byte rows 5.36 / 5.25 ms.
+  // Default lane depth (measured on MI355X). Lane-DFS: the cost model of choose_lane_shape
+  // over kl and tiles per wave (2^31 x 32 B, 24 keys: kl = 9, 921 ms; 935 / 937 / 940 ms at
+  // kl = 8 / 10 / 11. 2^27 x 32 B: 1 key 2.56 ms at kl = 9 against 3.25 ms at kl = 6; 2 keys
+  // 4.94 ms at kl = 10; 3 keys 7.35 ms at kl = 7). The stream kernel at 78,643,200 x 1 KiB:
+  // one key per wave runs best at kl = 4 (11.97 ms; 12.08 ms at kl = 6), two keys per wave at
+  // kl = 6 (13.04 ms; 13.14 ms at kl = 7), three at kl = 4 to 7 alike (15.02 to 15.13 ms).
+  // Narrower rows carry more PRG work per byte and want fewer lane-root walks: 32 GiB of
+  // 144-byte rows, one key, 10.4 ms at kl = 4 and 9.2 ms at kl = 6; 272-byte rows 7.04 / 6.66
+  // ms; 512-byte rows 5.36 / 5.25 ms.
   uint32_t kl_default;
+  LaneShape lane_shape;
   if (kernel == MG_KERNEL_LANE_DFS) {
-    const uint64_t work = (pl->row_end - pl->row_begin) * (uint64_t)n_keys;
-    kl_default = 6;
-    while (kl_default < 10 && (work >> (kl_default + 7)) >= 4096) kl_default++;
+    int rc0 = choose_lane_shape(dev, *pl, d, n_keys, o->lane_depth,
+                                (uint32_t)pl->waves_per_block, o->waves_per_key, &lane_shape);
+    if (rc0) return rc0;
+    kl_default = lane_shape.kl;
   } else if (kernel == MG_KERNEL_STREAM_SPLIT) {
     // Split kernel, same matrix: one or two keys per group run best at kl = 7 with one pair
     // per block (1 key: 11.39 ms; 11.69 ms at kl = 6), three at kl = 6 and four at kl = 5 with
@@ -310,11 +398,8 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   int resident = 0;
   int rc = MG_OK;
   if (kernel == MG_KERNEL_LANE_DFS) {
-    if (!o->waves_per_block)
-      while (pl->waves_per_block > 1 &&
-             (size_t)pl->waves_per_block * pl->kl * 80 * 16 > dev->max_lds)
-        pl->waves_per_block--;
-    const size_t lds = (size_t)pl->waves_per_block * pl->kl * 80 * 16;
+    if (!o->waves_per_block) pl->waves_per_block = lane_shape.wpb;
+    const size_t lds = lane_dfs_lds_bytes((int)pl->waves_per_block, pl->kl);
     if (lds > dev->max_lds)
       return fail(MG_ERR_INVALID_ARG, "lane-DFS plan needs %zu B of LDS per block (max %zu)", lds,
                   dev->max_lds);
@@ -369,6 +454,8 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   }
   if (rc) return rc;
   uint64_t parts = o->waves_per_key;
+  if (!parts && kernel == MG_KERNEL_LANE_DFS && lane_shape.tiles_per_wave)
+    parts = (n_tiles + lane_shape.tiles_per_wave - 1) / lane_shape.tiles_per_wave;
   if (!parts) parts = std::max<uint64_t>(1, (uint64_t)resident / groups);
   parts = std::min<uint64_t>(parts, n_tiles ? n_tiles : 1);
   parts = std::min<uint64_t>(parts, 1u << 20);

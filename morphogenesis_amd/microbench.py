@@ -42,7 +42,8 @@ MIX_FORMS = ["1 chain add", "2 chains add", "4 chains add", "8 chains add", "16 
              "8 chains add/xor, 720-instruction loop",
              "8 chains x += z, z ^= x (both operands fresh)",
              "8 chains x += z, z ^= x, z = rotl(z) (both operands fresh)",
-             "8 quarter-rounds, op-major order", "8 quarter-rounds, one QR at a time"]
+             "8 quarter-rounds, op-major order", "8 quarter-rounds, one QR at a time",
+             "16 quarter-rounds, op-major order"]
 
 # Instruction forms of mgb_op (csrc/mg_microbench.hip, op_once).
 OP_FORMS = ["v_add_u32 (VOP2)", "v_xor_b32 (VOP2)", "v_alignbit_b32 x,x,x,25",
