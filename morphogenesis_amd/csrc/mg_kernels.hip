@@ -134,10 +134,16 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   extern __shared__ uint4 lds[];
   const int lane = threadIdx.x & 63;
   const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t w = xcd_logical_wave(p.parts * p.n_keys, blockDim.x >> 6, wib);
-  if (w >= p.parts * p.n_keys) return;
-  const uint32_t key_i = w % p.n_keys;
-  const uint32_t part = w / p.n_keys;
+  const uint32_t total = p.parts * p.n_keys;
+  const uint32_t w = xcd_logical_wave(total, blockDim.x >> 6, wib);
+  if (w - (uint32_t)wib >= total) return;  // the whole block is past the work
+  // The block's waves (consecutive keys of one part) meet at a barrier after
+  // every tile, so they stay in step and their identical row reads hit in
+  // L2 instead of each fetching the rows again. Waves past the work, and
+  // waves of a part with fewer tiles, take the same number of barriers.
+  const bool live = w < total;
+  const uint32_t key_i = live ? w % p.n_keys : 0;
+  const uint32_t part = live ? w / p.n_keys : 0;
   const DevKey* __restrict__ k = p.keys + (p.key_idx ? p.key_idx[key_i] : key_i);
   const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
 
@@ -157,6 +163,7 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   const uint64_t t_begin = p.tile_begin + (uint64_t)part * p.tiles_per_wave;
   uint64_t t_end = t_begin + p.tiles_per_wave;
   if (t_end > p.tile_end) t_end = p.tile_end;
+  if (!live || t_end < t_begin) t_end = t_begin;
   const bool lane_active = lane < (1 << lb);
   const uint32_t n_pairs = 1u << (kl - 1);
   const int leaf_level = d - kl;
@@ -166,7 +173,10 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
     const Node r = key_root(k);
     path.s0 = r.s0; path.s1 = r.s1; path.s2 = r.s2; path.s3 = r.s3; path.t = r.t;
   }
-  for (uint64_t tile = t_begin; tile < t_end; ++tile) {
+  for (uint64_t ti = 0; ti < p.tiles_per_wave; ++ti) {
+    const uint64_t tile = t_begin + ti;
+    if (ti) __syncthreads();
+    if (tile >= t_end) continue;
     Node node = tile_lane_root(k, path, lane, tile, tile - 1, tile == t_begin, tb, lb);
     const uint64_t base = (tile << (kl + lb)) | ((uint64_t)lane << kl);
 
@@ -215,7 +225,7 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
 #pragma unroll
   for (int c = 0; c < C; c++) {
     const uint4 v = wave_xor4(acc[c]);
-    if (lane == 0) p.partial[((size_t)part * p.n_keys + key_i) * C + c] = v;
+    if (live && lane == 0) p.partial[((size_t)part * p.n_keys + key_i) * C + c] = v;
   }
 }
 
