@@ -271,10 +271,12 @@ int sweep_step(mg_server* s, std::vector<uint32_t>* finished) {
     const uint32_t n = off - start;
     if (!n || row_end <= row_begin) continue;
     // The scan's own choice for this row width and key count (lane-DFS up to 128-byte rows,
-    // then the stream or split kernel); all of them tile a segment at this lane depth alike.
+    // then the stream or split kernel). Segments are multiples of the 2^(kl + lb)-row tile of
+    // the loop's lane depth; lane-DFS rows let the plan pick the depth for this step's key
+    // count among those whose tiles divide that (every smaller one does).
     mg_scan_opts o{};
     o.kernel = MG_KERNEL_AUTO;
-    o.lane_depth = s->kl;
+    o.lane_depth = s->chunks <= 8 ? 0 : s->kl;
     o.row_begin = row_begin;
     o.row_end = row_end;
     mg::Plan pl;
