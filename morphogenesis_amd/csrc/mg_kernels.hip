@@ -319,10 +319,15 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
   const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
   const uint32_t C = p.chunks;
 
-  // LDS per wave: G stacks [kl][64] seeds + [kl][64] t, then masks [G][128].
-  const size_t per_wave = (size_t)G * kl * 80 + (size_t)G * 128;
+  // LDS per wave: G stacks of seeds at depths 1 .. kl-1, [kl-1][64] each,
+  // then masks [G][128]. Each key's pushed control bytes are kept as the bit
+  // t == 1 in a register (bit `depth`), as in scan_lane_dfs.
+  const size_t per_wave = (size_t)G * (kl - 1) * 64 + (size_t)G * 128;
   uint4* base_lds = lds + (size_t)wib * per_wave;
-  uint4* masks = base_lds + (size_t)G * kl * 80;
+  uint4* masks = base_lds + (size_t)G * (kl - 1) * 64;
+  uint32_t st_tbits[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) st_tbits[g] = 0;
 
   const DevKey* kp[G];
   bool key_live[G];
@@ -450,16 +455,16 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
     for (;;) {
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        uint4* st_seed = base_lds + (size_t)g * kl * 80;
-        uint32_t* st_t = reinterpret_cast<uint32_t*>(st_seed + kl * 64);
+        uint4* st_seed = base_lds + (size_t)g * (kl - 1) * 64;
         Node L, R;
         expand_node(kp[g], leaf_level + lvl, node[g], L, R);
         if (lvl == kl - 1) {
           masks[g * 128 + 2 * lane] = leaf_mask(kp[g], L);
           masks[g * 128 + 2 * lane + 1] = leaf_mask(kp[g], R);
         } else {
-          st_seed[(lvl + 1) * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);
-          st_t[(lvl + 1) * 64 + lane] = R.t;
+          st_seed[lvl * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth lvl + 1
+          const uint32_t bit = 1u << (lvl + 1);
+          st_tbits[g] = R.t == 1u ? (st_tbits[g] | bit) : (st_tbits[g] & ~bit);
           node[g] = L;
         }
       }
@@ -483,12 +488,11 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
       prefetch_step(tile_base, pair, whole_tile);
       lvl = kl - 1 - __builtin_ctz(pair);
 #pragma unroll
-      for (int g = 0; g < G; g++) {
-        uint4* st_seed = base_lds + (size_t)g * kl * 80;
-        uint32_t* st_t = reinterpret_cast<uint32_t*>(st_seed + kl * 64);
-        const uint4 sv = st_seed[lvl * 64 + lane];
+      for (int g = 0; g < G; g++) {  // the right child at depth lvl >= 1: slot lvl - 1
+        const uint4* st_seed = base_lds + (size_t)g * (kl - 1) * 64;
+        const uint4 sv = st_seed[(lvl - 1) * 64 + lane];
         node[g].s0 = sv.x; node[g].s1 = sv.y; node[g].s2 = sv.z; node[g].s3 = sv.w;
-        node[g].t = st_t[lvl * 64 + lane];
+        node[g].t = (st_tbits[g] >> lvl) & 1u;
       }
     }
   }
@@ -547,11 +551,12 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
   const uint64_t n_steps = live && t_end > t_begin ? (t_end - t_begin) * n_pairs : 0;
   const uint64_t block_steps = p.tiles_per_wave * n_pairs;  // barriers every wave takes
   // Right children wait at depths 1 .. kl-1 (depth 0 is the lane root,
-  // never pushed), so a key's stack has kl - 1 slots of 64 lanes.
+  // never pushed), so a key's stack has kl - 1 slots of 64 seeds; their
+  // control bytes are kept as the bit t == 1 in a register per key.
   const int slots = kl - 1;
-  uint4* pair_lds = lds + (size_t)pi * ((size_t)G * slots * 80 + (size_t)2 * G * 128);
-  uint4* stacks = pair_lds;                              // [G][slots][80]
-  uint4* masks = pair_lds + (size_t)G * slots * 80;      // [2][G][128]
+  uint4* pair_lds = lds + (size_t)pi * ((size_t)G * slots * 64 + (size_t)2 * G * 128);
+  uint4* stacks = pair_lds;                              // [G][slots][64]
+  uint4* masks = pair_lds + (size_t)G * slots * 64;      // [2][G][128]
 
   if (producer) {
     const DevKey* kp[G];
@@ -569,6 +574,9 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
     }
     const int leaf_level = d - kl;
     Node node[G];
+    uint32_t st_tbits[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) st_tbits[g] = 0;
     for (uint64_t s = 0; s < block_steps; ++s) {
       if (s < n_steps) {
         const uint64_t tile = t_begin + (s >> (kl - 1));
@@ -583,18 +591,18 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
         const int lvl = pair ? kl - 1 - __builtin_ctz(pair) : 0;
 #pragma unroll
         for (int g = 0; g < G; g++) {
-          uint4* st_seed = stacks + (size_t)g * slots * 80;
-          uint32_t* st_t = reinterpret_cast<uint32_t*>(st_seed + slots * 64);
+          uint4* st_seed = stacks + (size_t)g * slots * 64;
           if (pair) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
             const uint4 sv = st_seed[(lvl - 1) * 64 + lane];
             node[g].s0 = sv.x; node[g].s1 = sv.y; node[g].s2 = sv.z; node[g].s3 = sv.w;
-            node[g].t = st_t[(lvl - 1) * 64 + lane];
+            node[g].t = (st_tbits[g] >> lvl) & 1u;
           }
           Node L, R;
           for (int l = lvl; l < kl - 1; ++l) {
             expand_node(kp[g], leaf_level + l, node[g], L, R);
             st_seed[l * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth l + 1
-            st_t[l * 64 + lane] = R.t;
+            const uint32_t bit = 1u << (l + 1);
+            st_tbits[g] = R.t == 1u ? (st_tbits[g] | bit) : (st_tbits[g] & ~bit);
             node[g] = L;
           }
           expand_node(kp[g], leaf_level + kl - 1, node[g], L, R);
@@ -954,7 +962,7 @@ hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t
 }
 
 size_t split_lds_bytes(int G, uint32_t kl, int NP) {
-  return (size_t)NP * ((size_t)G * (kl - 1) * 80 + (size_t)2 * G * 128) * sizeof(uint4);
+  return (size_t)NP * ((size_t)G * (kl - 1) * 64 + (size_t)2 * G * 128) * sizeof(uint4);
 }
 
 hipError_t split_blocks_per_cu(int S, int G, int NP, size_t lds, int* blocks) {
@@ -968,7 +976,7 @@ hipError_t split_blocks_per_cu(int S, int G, int NP, size_t lds, int* blocks) {
 }
 
 size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block) {
-  return (size_t)waves_per_block * ((size_t)G * kl * 80 + (size_t)G * 128) * sizeof(uint4);
+  return (size_t)waves_per_block * ((size_t)G * (kl - 1) * 64 + (size_t)G * 128) * sizeof(uint4);
 }
 
 // (S, G, PK) instances of scan_stream: PK > 1 only with S = 1.

@@ -371,11 +371,11 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     kl_default = lane_shape.kl;
   } else if (kernel == MG_KERNEL_STREAM_SPLIT) {
     // Split kernel, same matrix: one or two keys per group run best at kl = 7 with one pair
-    // per block (1 key: 11.39 ms; 11.69 ms at kl = 6), three at kl = 6 and four at kl = 5 with
-    // four pairs per block (12.90 and 14.65 ms), whose LDS stacks then still fit.
+    // per block (1 key: 11.39 ms; 11.69 ms at kl = 6), three and four at kl = 6 with four pairs
+    // per block (12.33 and 13.66 ms; four keys at kl = 5: 14.15 ms).
     const int g = o->keys_per_wave ? (int)o->keys_per_wave
                                    : default_keys_per_wave(kernel, stream_s(pl->chunks), n_keys);
-    kl_default = g <= 2 ? 7 : (g == 3 ? 6 : 5);
+    kl_default = g <= 2 ? 7 : 6;
     if (d > 6) kl_default = std::min(kl_default, d - 6);
   } else {
     const int g = o->keys_per_wave ? (int)o->keys_per_wave
@@ -420,6 +420,10 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
       rc = resident_waves(dev, MG_KERNEL_STREAM * 100 + (uint32_t)(S * 8 + G), S, G,
                           stream_pk(pl->chunks), (int)pl->chunks, (int)pl->waves_per_block, lds,
                           &resident);
+      // With two or more keys per wave, more than two waves per SIMD run slower (32 GiB of
+      // 272-byte rows, 2 keys: 10.15 ms at 1928 waves, 13.5 ms at 2804); one key per wave
+      // gains from the third (6.58 against 7.21 ms).
+      if (G >= 2) resident = std::min(resident, 2 * 4 * dev->cus);
     }
   }
   if (kernel == MG_KERNEL_STREAM_SPLIT) {

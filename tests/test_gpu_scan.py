@@ -128,9 +128,9 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
             for wpb in (0, 4, 8):
                 opts = mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=kl,
                                     keys_per_wave=g, waves_per_block=wpb)
-                # Per pair: g stacks of kl - 1 levels (80 x 16 B each) and two
+                # Per pair: g stacks of kl - 1 levels (64 x 16 B each) and two
                 # 128-mask buffers per key; a block gets 160 KiB of LDS.
-                if (wpb or 2) // 2 * (g * (kl - 1) * 80 + 256 * g) * 16 > 160 * 1024:
+                if (wpb or 2) // 2 * (g * (kl - 1) * 64 + 256 * g) * 16 > 160 * 1024:
                     with pytest.raises(mg.MgError):
                         scanner.scan_keys(m, keys, opts)
                     continue
