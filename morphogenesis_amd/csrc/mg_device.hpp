@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <utility>
 
 #include "mg_key.hpp"
 
@@ -106,9 +107,32 @@ MG_ROTR_ASM(24)
 MG_ROTR_ASM(25)
 #undef MG_ROTR_ASM
 
+// Rotate by 16 as a packed 16-bit add of zero with the halves swapped.
+__device__ __forceinline__ uint32_t op_rotr16_pk(uint32_t a) {
+  uint32_t r;
+  asm volatile("v_pk_add_u16 %0, %1, 0 op_sel:[1,0] op_sel_hi:[0,0]" : "=v"(r) : "v"(a));
+  return r;
+}
+// rotr16(d ^ a) in two word-selecting xors: the low half from the high halves
+// (upper bits cleared), then the high half from the low halves.
+__device__ __forceinline__ uint32_t op_xor_rotr16_lo(uint32_t d, uint32_t a) {
+  uint32_t r;
+  asm volatile("v_xor_b32_sdwa %0, %1, %2 dst_sel:WORD_0 dst_unused:UNUSED_PAD "
+               "src0_sel:WORD_1 src1_sel:WORD_1"
+               : "=v"(r) : "v"(d), "v"(a));
+  return r;
+}
+__device__ __forceinline__ uint32_t op_xor_rotr16_hi(uint32_t lo, uint32_t d, uint32_t a) {
+  asm volatile("v_xor_b32_sdwa %0, %1, %2 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE "
+               "src0_sel:WORD_0 src1_sel:WORD_0"
+               : "+v"(lo) : "v"(d), "v"(a));
+  return lo;
+}
+
 // Stages 1..LAST of NQ quarter-rounds (a[q], b[q], c[q], d[q]) in op-major
-// order.
-template <int NQ, int LAST>
+// order. R16 picks stages 2-3 (d ^= a; d = rotr16(d)): 0 xor + alignbit, 1 xor
+// + packed-add swap, 2 two SDWA xors.
+template <int NQ, int LAST, int R16 = 0>
 __device__ __forceinline__ void qr_stages(uint32_t* a[], uint32_t* b[], uint32_t* c[],
                                           uint32_t* d[]) {
 #define MG_STAGE(K, EXPR)                                                      \
@@ -116,8 +140,14 @@ __device__ __forceinline__ void qr_stages(uint32_t* a[], uint32_t* b[], uint32_t
     _Pragma("unroll") for (int q = 0; q < NQ; q++) { EXPR; }                   \
   }
   MG_STAGE(1, *a[q] = op_add(*a[q], *b[q]))
-  MG_STAGE(2, *d[q] = op_xor(*d[q], *a[q]))
-  MG_STAGE(3, *d[q] = op_rotr16(*d[q]))
+  if constexpr (R16 == 2) {
+    uint32_t t[NQ];
+    MG_STAGE(2, t[q] = op_xor_rotr16_lo(*d[q], *a[q]))
+    MG_STAGE(3, *d[q] = op_xor_rotr16_hi(t[q], *d[q], *a[q]))
+  } else {
+    MG_STAGE(2, *d[q] = op_xor(*d[q], *a[q]))
+    MG_STAGE(3, *d[q] = R16 == 1 ? op_rotr16_pk(*d[q]) : op_rotr16(*d[q]))
+  }
   MG_STAGE(4, *c[q] = op_add(*c[q], *d[q]))
   MG_STAGE(5, *b[q] = op_xor(*b[q], *c[q]))
   MG_STAGE(6, *b[q] = op_rotr20(*b[q]))
@@ -130,6 +160,7 @@ __device__ __forceinline__ void qr_stages(uint32_t* a[], uint32_t* b[], uint32_t
 #undef MG_STAGE
 }
 
+template <int R16 = 0>
 __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uint32_t s2,
                                                    uint32_t s3, uint32_t l[4], uint32_t r[4],
                                                    uint32_t& lt, uint32_t& rt) {
@@ -155,13 +186,13 @@ __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uin
       uint32_t* b[8] = {&x5, &x6, &x7, &x4, &y5, &y6, &y7, &y4};
       uint32_t* c[8] = {&x10, &x11, &x8, &x9, &y10, &y11, &y8, &y9};
       uint32_t* d[8] = {&x15, &x12, &x13, &x14, &y15, &y12, &y13, &y14};
-      qr_stages<8, 12>(a, b, c, d);
+      qr_stages<8, 12, R16>(a, b, c, d);
     } else {
       uint32_t* a[8] = {&x0, &x1, &x2, &x3, &y0, &y1, &y2, &y3};
       uint32_t* b[8] = {&x4, &x5, &x6, &x7, &y4, &y5, &y6, &y7};
       uint32_t* c[8] = {&x8, &x9, &x10, &x11, &y8, &y9, &y10, &y11};
       uint32_t* d[8] = {&x12, &x13, &x14, &x15, &y12, &y13, &y14, &y15};
-      qr_stages<8, 12>(a, b, c, d);
+      qr_stages<8, 12, R16>(a, b, c, d);
     }
   }
   // Round 8 (diagonal): block 0 in full; block 1's quarter-rounds through
@@ -171,7 +202,7 @@ __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uin
     uint32_t* b[6] = {&x5, &x6, &x7, &x4, &y5, &y6};
     uint32_t* c[6] = {&x10, &x11, &x8, &x9, &y10, &y11};
     uint32_t* d[6] = {&x15, &x12, &x13, &x14, &y15, &y12};
-    qr_stages<6, 7>(a, b, c, d);
+    qr_stages<6, 7, R16>(a, b, c, d);
     uint32_t* a4[4] = {&x0, &x1, &x2, &x3};
     uint32_t* b4[4] = {&x5, &x6, &x7, &x4};
     uint32_t* c4[4] = {&x10, &x11, &x8, &x9};
@@ -195,6 +226,83 @@ __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uin
   rt = (y1 + kC1) & 1u;
 }
 
+// Quarter-round stage K (0-11) on the words a, b, c, d.
+template <int K>
+__device__ __forceinline__ void qr_op(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d) {
+  if constexpr (K == 0 || K == 6) a = op_add(a, b);
+  else if constexpr (K == 1 || K == 7) d = op_xor(d, a);
+  else if constexpr (K == 2) d = op_rotr16(d);
+  else if constexpr (K == 3 || K == 9) c = op_add(c, d);
+  else if constexpr (K == 4 || K == 10) b = op_xor(b, c);
+  else if constexpr (K == 5) b = op_rotr20(b);
+  else if constexpr (K == 8) d = op_rotr24(d);
+  else b = op_rotr25(b);
+}
+
+// Quarter-round Q's words (a, b, c, d) in round RD: columns in odd rounds,
+// diagonals in even ones.
+template <int RD, int Q>
+struct QrWords {
+  static constexpr int a = Q;
+  static constexpr int b = RD % 2 ? 4 + Q : 4 + (Q + 1) % 4;
+  static constexpr int c = RD % 2 ? 8 + Q : 8 + (Q + 2) % 4;
+  static constexpr int d = RD % 2 ? 12 + Q : 12 + (Q + 3) % 4;
+};
+
+template <int T, int Q>
+__device__ __forceinline__ void skew_qr(uint32_t (&x)[16], uint32_t (&y)[16]) {
+  {
+    using W = QrWords<2 + T / 12, Q>;
+    qr_op<T % 12>(x[W::a], x[W::b], x[W::c], x[W::d]);
+  }
+  if constexpr (T >= 1) {
+    constexpr int U = T - 1, RD = 2 + U / 12, K = U % 12;
+    if constexpr (RD < 8 || (Q < 2 && K <= 6)) {
+      using W = QrWords<RD, Q>;
+      qr_op<K>(y[W::a], y[W::b], y[W::c], y[W::d]);
+    }
+  }
+}
+
+// Step T of the skewed rounds: stage T of block 0 and stage T - 1 of block 1
+// (stage t is round 2 + t / 12, op t % 12), the two blocks' ops alternating.
+template <size_t... T>
+__device__ __forceinline__ void skew_steps(uint32_t (&x)[16], uint32_t (&y)[16],
+                                           std::index_sequence<T...>) {
+  ((skew_qr<(int)T, 0>(x, y), skew_qr<(int)T, 1>(x, y), skew_qr<(int)T, 2>(x, y),
+    skew_qr<(int)T, 3>(x, y)),
+   ...);
+}
+
+// As prg_expand_ordered, with block 1 one quarter-round stage behind block 0
+// through rounds 2-8 and the two blocks' ops interleaved one by one, so each
+// rotate (4-cycle issue) sits beside an add or xor of the other block. Block
+// 1's round 8 runs only its quarter-rounds through words 0 and 1, up to the
+// stage that sets them.
+__device__ __forceinline__ void prg_expand_skewed(uint32_t s0, uint32_t s1, uint32_t s2,
+                                                  uint32_t s3, uint32_t l[4], uint32_t r[4],
+                                                  uint32_t& lt, uint32_t& rt) {
+  uint32_t x[16] = {kC0, kC1, kC2, kC3, s0, s1, s2, s3, s0, s1, s2, s3, 0, 0, 0, 0};
+  uint32_t y[16];
+  y[0] = kC0; y[4] = s0; y[8] = s0; y[12] = 1;
+  // Round 1 (columns): block 1 differs from block 0 only in column 0.
+  MG_QR(x[0], x[4], x[8], x[12]);
+  MG_QR(x[1], x[5], x[9], x[13]);
+  MG_QR(x[2], x[6], x[10], x[14]);
+  MG_QR(x[3], x[7], x[11], x[15]);
+  MG_QR(y[0], y[4], y[8], y[12]);
+#pragma unroll
+  for (int c = 1; c < 4; c++) {
+    y[c] = x[c]; y[4 + c] = x[4 + c]; y[8 + c] = x[8 + c]; y[12 + c] = x[12 + c];
+  }
+  // 84 stages of block 0; block 1's last (round 8, stage 6) is step 79.
+  skew_steps(x, y, std::make_index_sequence<84>());
+  l[0] = x[0] + kC0; l[1] = x[1] + kC1; l[2] = x[2] + kC2; l[3] = x[3] + kC3;
+  r[0] = x[4] + s0; r[1] = x[5] + s1; r[2] = x[6] + s2; r[3] = x[7] + s3;
+  lt = (y[0] + kC0) & 1u;
+  rt = (y[1] + kC1) & 1u;
+}
+
 // All-ones if t == 1, else zero, kept opaque to the optimiser so that
 // `x ^ (uniform & m)` stays one v_bitop3_b32 with the uniform operand in an
 // SGPR, instead of being rewritten as a select of the uniform value (which
@@ -205,26 +313,61 @@ __device__ __forceinline__ uint32_t t_mask(uint32_t t) {
   return m;
 }
 
-// Which expansion the kernels use (the ordered one issues rounds 2-8 in a
-// fixed op-major order).
-constexpr bool kPrgOrdered = true;
+// Which expansion the kernels use: 0 the compiler's order, 1 rounds 2-8
+// op-major, 2 op-major with block 1 one stage behind block 0, 3 and 4
+// op-major with the rotate by 16 as a packed add (3) or SDWA xors (4).
+constexpr int kPrgVariant = 1;
 
 // Both children of `n` at tree level `level`, correction word applied when
 // n.t == 1 (dpf.rs:221-239). `k` is wave-uniform, so the correction word is
 // read with scalar loads.
-template <bool kOrdered = kPrgOrdered>
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Key reads go through the constant address space: no kernel writes a key,
+// and a wave-uniform read then becomes a scalar load into SGPRs (a divergent
+// one stays a vector load). Through plain pointers the compiler loads them
+// with vector loads again after every barrier. Kernels whose waves hand
+// masks over through LDS keep the vector loads (kScalarKey = false): a wave
+// waiting for a scalar load also waits for its own LDS stores, which share
+// the same counter.
+typedef __attribute__((address_space(4))) const u32x4 key_u32x4;
+typedef __attribute__((address_space(4))) const uint32_t key_u32;
+__device__ __forceinline__ u32x4 key_load4(const uint32_t* p) {
+  return *(key_u32x4*)p;  // an address-space cast, which reinterpret_cast cannot express
+}
+__device__ __forceinline__ uint32_t key_load(const uint32_t* p) {
+  return *(key_u32*)p;
+}
+template <bool kScalarKey>
+__device__ __forceinline__ u32x4 key_read4(const uint32_t* p) {
+  if constexpr (kScalarKey) return key_load4(p);
+  else return *reinterpret_cast<const u32x4*>(p);
+}
+template <bool kScalarKey>
+__device__ __forceinline__ uint32_t key_read(const uint32_t* p) {
+  if constexpr (kScalarKey) return key_load(p);
+  else return *p;
+}
+
+template <int kVariant = kPrgVariant, bool kScalarKey = true>
 __device__ __forceinline__ void expand_node(const DevKey* __restrict__ k,
                                             int level, const Node& n,
                                             Node& left, Node& right) {
   uint32_t l[4], r[4], lt, rt;
-  if constexpr (kOrdered)
+  if constexpr (kVariant == 2)
+    prg_expand_skewed(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
+  else if constexpr (kVariant == 3)
+    prg_expand_ordered<1>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
+  else if constexpr (kVariant == 4)
+    prg_expand_ordered<2>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
+  else if constexpr (kVariant == 1)
     prg_expand_ordered(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
   else
     prg_expand(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
   const uint32_t m = t_mask(n.t);
-  const uint32_t c0 = k->cw_seed[level][0], c1 = k->cw_seed[level][1];
-  const uint32_t c2 = k->cw_seed[level][2], c3 = k->cw_seed[level][3];
-  const uint32_t ct = k->cw_t[level];
+  const u32x4 cw = key_read4<kScalarKey>(k->cw_seed[level]);
+  const uint32_t c0 = cw.x, c1 = cw.y, c2 = cw.z, c3 = cw.w;
+  const uint32_t ct = key_read<kScalarKey>(&k->cw_t[level]);
   left.s0 = l[0] ^ (c0 & m); left.s1 = l[1] ^ (c1 & m);
   left.s2 = l[2] ^ (c2 & m); left.s3 = l[3] ^ (c3 & m);
   right.s0 = r[0] ^ (c0 & m); right.s1 = r[1] ^ (c1 & m);
@@ -243,11 +386,12 @@ __device__ __forceinline__ Node ggm_step(const DevKey* __restrict__ k,
 }
 
 // Leaf output (dpf.rs:136-141): seed ^ final_cw iff t == 1.
+template <bool kScalarKey = true>
 __device__ __forceinline__ uint4 leaf_mask(const DevKey* __restrict__ k,
                                            const Node& n) {
   const uint32_t m = t_mask(n.t);
-  return make_uint4(n.s0 ^ (k->final_cw[0] & m), n.s1 ^ (k->final_cw[1] & m),
-                    n.s2 ^ (k->final_cw[2] & m), n.s3 ^ (k->final_cw[3] & m));
+  const u32x4 f = key_read4<kScalarKey>(k->final_cw);
+  return make_uint4(n.s0 ^ (f.x & m), n.s1 ^ (f.y & m), n.s2 ^ (f.z & m), n.s3 ^ (f.w & m));
 }
 
 __device__ __forceinline__ Node key_root(const DevKey* __restrict__ k) {
@@ -268,7 +412,6 @@ __device__ __forceinline__ uint4 point_eval(const DevKey* __restrict__ k,
   return leaf_mask(k, n);
 }
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Streaming 16-byte load that is not kept in the caches (global_load_dwordx4 nt).
 __device__ __forceinline__ uint4 load_nt(const uint4* p) {

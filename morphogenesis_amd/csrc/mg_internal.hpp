@@ -59,6 +59,7 @@ struct Plan {
   uint32_t waves_per_block;
   uint32_t parts;
   uint64_t tile_begin, tile_end, tiles_per_wave;
+  uint64_t steps_per_wave;  // split kernel: leaf-pair steps per DFS/stream pair
   uint64_t row_begin, row_end;
   uint64_t n_delta;
   const void* rows;

@@ -457,10 +457,10 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
       for (int g = 0; g < G; g++) {
         uint4* st_seed = base_lds + (size_t)g * (kl - 1) * 64;
         Node L, R;
-        expand_node(kp[g], leaf_level + lvl, node[g], L, R);
+        expand_node<kPrgVariant, false>(kp[g], leaf_level + lvl, node[g], L, R);
         if (lvl == kl - 1) {
-          masks[g * 128 + 2 * lane] = leaf_mask(kp[g], L);
-          masks[g * 128 + 2 * lane + 1] = leaf_mask(kp[g], R);
+          masks[g * 128 + 2 * lane] = leaf_mask<false>(kp[g], L);
+          masks[g * 128 + 2 * lane + 1] = leaf_mask<false>(kp[g], R);
         } else {
           st_seed[lvl * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth lvl + 1
           const uint32_t bit = 1u << (lvl + 1);
@@ -544,12 +544,15 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
   const uint32_t key0 = grp * G;
   const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
   const uint32_t C = p.chunks;
-  const uint64_t t_begin = p.tile_begin + (uint64_t)part * p.tiles_per_wave;
-  uint64_t t_end = t_begin + p.tiles_per_wave;
-  if (t_end > p.tile_end) t_end = p.tile_end;
+  // This pair's leaf-pair steps: global steps gs0 .. gs0 + n_steps - 1, where global step
+  // gs is pair gs mod 2^(kl-1) of tile tile_begin + gs / 2^(kl-1).
   const uint32_t n_pairs = 1u << (kl - 1);
-  const uint64_t n_steps = live && t_end > t_begin ? (t_end - t_begin) * n_pairs : 0;
-  const uint64_t block_steps = p.tiles_per_wave * n_pairs;  // barriers every wave takes
+  const uint64_t total_steps = (p.tile_end - p.tile_begin) << (kl - 1);
+  const uint64_t gs0 = (uint64_t)part * p.steps_per_wave;
+  uint64_t n_steps = 0;
+  if (live && gs0 < total_steps)
+    n_steps = total_steps - gs0 < p.steps_per_wave ? total_steps - gs0 : p.steps_per_wave;
+  const uint64_t block_steps = p.steps_per_wave;  // barriers every wave takes
   // Right children wait at depths 1 .. kl-1 (depth 0 is the lane root,
   // never pushed), so a key's stack has kl - 1 slots of 64 seeds; their
   // control bytes are kept as the bit t == 1 in a register per key.
@@ -579,35 +582,58 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
     for (int g = 0; g < G; g++) st_tbits[g] = 0;
     for (uint64_t s = 0; s < block_steps; ++s) {
       if (s < n_steps) {
-        const uint64_t tile = t_begin + (s >> (kl - 1));
-        const uint32_t pair = (uint32_t)(s & (n_pairs - 1));
-        if (pair == 0) {
+        const uint64_t gs = gs0 + s;
+        const uint64_t tile = p.tile_begin + (gs >> (kl - 1));
+        const uint32_t pair = (uint32_t)(gs & (n_pairs - 1));
+        const bool from_root = pair == 0 || s == 0;  // a new tile, or the run's first step
+        if (from_root) {
 #pragma unroll
           for (int g = 0; g < G; g++)
-            node[g] =
-                tile_lane_root(kp[g], path[g], lane, tile, tile - 1, tile == t_begin, tb, lb);
+            node[g] = tile_lane_root(kp[g], path[g], lane, tile, tile - 1, s == 0, tb, lb);
+        }
+        const bool seek = s == 0 && pair != 0;
+        if (seek) {
+          // The run starts inside a tile: walk each key from the lane root down to leaf
+          // pair `pair`, keeping the right children that lie ahead of it, as the
+          // depth-first walk from pair 0 would have left them.
+#pragma unroll
+          for (int g = 0; g < G; g++) {
+            uint4* st_seed = stacks + (size_t)g * slots * 64;
+            for (int l = 0; l < kl - 1; ++l) {
+              Node L, R;
+              expand_node<kPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
+              if ((pair >> (kl - 2 - l)) & 1u) {
+                node[g] = R;
+              } else {
+                st_seed[l * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);
+                const uint32_t bit = 1u << (l + 1);
+                st_tbits[g] = R.t == 1u ? (st_tbits[g] | bit) : (st_tbits[g] & ~bit);
+                node[g] = L;
+              }
+            }
+          }
         }
         uint4* mbuf = masks + (size_t)(s & 1) * G * 128;
-        const int lvl = pair ? kl - 1 - __builtin_ctz(pair) : 0;
+        const int lvl = seek ? kl - 1 : from_root ? 0 : kl - 1 - __builtin_ctz(pair);
 #pragma unroll
         for (int g = 0; g < G; g++) {
           uint4* st_seed = stacks + (size_t)g * slots * 64;
-          if (pair) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
+          if (!from_root) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
             const uint4 sv = st_seed[(lvl - 1) * 64 + lane];
             node[g].s0 = sv.x; node[g].s1 = sv.y; node[g].s2 = sv.z; node[g].s3 = sv.w;
             node[g].t = (st_tbits[g] >> lvl) & 1u;
           }
           Node L, R;
           for (int l = lvl; l < kl - 1; ++l) {
-            expand_node(kp[g], leaf_level + l, node[g], L, R);
+            expand_node<kPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
             st_seed[l * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth l + 1
             const uint32_t bit = 1u << (l + 1);
             st_tbits[g] = R.t == 1u ? (st_tbits[g] | bit) : (st_tbits[g] & ~bit);
             node[g] = L;
           }
-          expand_node(kp[g], leaf_level + kl - 1, node[g], L, R);
-          mbuf[g * 128 + 2 * lane] = leaf_mask(kp[g], L);
-          mbuf[g * 128 + 2 * lane + 1] = leaf_mask(kp[g], R);
+          expand_node<kPrgVariant, false>(kp[g], leaf_level + kl - 1, node[g], L, R);
+          mbuf[g * 128 + 2 * lane] = leaf_mask<false>(kp[g], L);
+          mbuf[g * 128 + 2 * lane + 1] = leaf_mask<false>(kp[g], R);
         }
       }
       __syncthreads();  // step s's masks are published
@@ -634,8 +660,9 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
       row_off[u][j] =
           (uint32_t)(((((uint64_t)(u >> 1) << kl) + (u & 1)) * C + lane_off[j]) * 16);
   auto load = [&](uint64_t s, uint32_t b, uint4 (&v)[U][S]) {
-    const uint64_t tile_base = (t_begin + (s >> (kl - 1))) << (kl + lb);
-    const uint32_t pair = (uint32_t)(s & (n_pairs - 1));
+    const uint64_t gs = gs0 + s;
+    const uint64_t tile_base = (p.tile_begin + (gs >> (kl - 1))) << (kl + lb);
+    const uint32_t pair = (uint32_t)(gs & (n_pairs - 1));
     const uint32_t r0 = b * U;
     if (tile_base + (1ull << (kl + lb)) <= p.n_rows) {
       const char* base = reinterpret_cast<const char*>(

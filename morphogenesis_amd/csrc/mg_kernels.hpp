@@ -41,6 +41,10 @@ struct StreamParams {
   uint64_t tile_begin;
   uint64_t tile_end;
   uint64_t tiles_per_wave;
+  // scan_split: pair `part` takes leaf-pair steps [part * steps_per_wave, +steps_per_wave)
+  // of the range, step = (tile - tile_begin) * 2^(kl-1) + pair; runs may start and end
+  // inside a tile. scan_stream hands out whole tiles (tiles_per_wave).
+  uint64_t steps_per_wave;
   uint32_t parts;  // waves per key group
   uint4* partial;  // [parts][n_keys][chunks]
   const uint4* zero_rows;  // >= 256 zero chunks, read in place of absent rows

@@ -15,7 +15,7 @@
 //   int mgb_prg(int device, uint32_t blocks, uint32_t iters, double* ms,
 //               uint32_t* checksum)           (the kernels' expansion)
 //   int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters,
-//                double* ms, uint32_t* checksum)  (either expansion)
+//                double* ms, uint32_t* checksum)  (expansion variant 0-4)
 //   int mgb_valu(int device, uint32_t blocks, uint32_t iters, double* ms,
 //                uint32_t* checksum)
 // Both launch `blocks` blocks of 256 threads, time the launch with HIP events
@@ -29,7 +29,7 @@
 
 namespace mg {
 
-template <bool kOrdered>
+template <int kVariant>
 __global__ __launch_bounds__(256) void prg_bench(const DevKey* __restrict__ key, uint32_t iters,
                                                  uint32_t* __restrict__ out) {
   const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void prg_bench(const DevKey* __restrict__ key,
   uint32_t acc = 0;
   for (uint32_t i = 0; i < iters; ++i) {
     Node L, R;
-    expand_node<kOrdered>(key, (int)(i & 15u), n, L, R);
+    expand_node<kVariant>(key, (int)(i & 15u), n, L, R);
     // The right child stays live (the scan kernels push it to their stack).
     acc ^= R.s0 ^ R.s1 ^ R.s2 ^ R.s3 ^ R.t;
     n = L;
@@ -88,6 +88,16 @@ __device__ __forceinline__ void op_once(uint32_t& x, uint32_t y) {
   else if constexpr (OP == 11) asm volatile("v_lshrrev_b32 %0, 25, %0" : "+v"(x));
   else if constexpr (OP == 12) asm volatile("v_or_b32 %0, %0, %1" : "+v"(x) : "v"(y));
   else if constexpr (OP == 13) asm volatile("v_alignbit_b32 %0, %0, %1, 25" : "+v"(x) : "v"(y));
+  // Halves swapped by a packed 16-bit add of zero: a rotate by 16.
+  else if constexpr (OP == 14)
+    asm volatile("v_pk_add_u16 %0, %0, %1 op_sel:[1,0] op_sel_hi:[0,0]" : "+v"(x) : "v"(0u));
+  else if constexpr (OP == 15) asm volatile("v_pk_add_u16 %0, %0, %1" : "+v"(x) : "v"(y));
+  else if constexpr (OP == 16)
+    asm volatile("v_xor_b32_sdwa %0, %0, %1 dst_sel:WORD_0 dst_unused:UNUSED_PAD "
+                 "src0_sel:WORD_1 src1_sel:WORD_1" : "+v"(x) : "v"(y));
+  else if constexpr (OP == 17)
+    asm volatile("v_xor_b32_sdwa %0, %0, %1 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE "
+                 "src0_sel:WORD_0 src1_sel:WORD_0" : "+v"(x) : "v"(y));
 }
 
 template <int OP>
@@ -216,6 +226,23 @@ __global__ __launch_bounds__(256) void fresh_bench(uint32_t iters, uint32_t* __r
 #define MGB_ADD(x, y) asm volatile("v_add_u32 %0, %0, %1" : "+v"(x) : "v"(y))
 #define MGB_XOR(x, y) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(x) : "v"(y))
 #define MGB_ROT(x, n) asm volatile("v_alignbit_b32 %0, %0, %0, " #n : "+v"(x))
+// Rotate by 16 as a packed 16-bit add of zero with the halves swapped.
+#define MGB_ROT16_PK(x) \
+  asm volatile("v_pk_add_u16 %0, %0, %1 op_sel:[1,0] op_sel_hi:[0,0]" : "+v"(x) : "v"(0u))
+// x = rotr16(x ^ y) as two word-selecting xors: low half from the high halves,
+// then the high half from the low halves.
+#define MGB_XOR_ROT16_SDWA(x, y)                                                        \
+  do {                                                                                  \
+    uint32_t t_;                                                                        \
+    asm volatile("v_xor_b32_sdwa %0, %1, %2 dst_sel:WORD_0 dst_unused:UNUSED_PAD "      \
+                 "src0_sel:WORD_1 src1_sel:WORD_1" : "=v"(t_) : "v"(x), "v"(y));        \
+    asm volatile("v_xor_b32_sdwa %0, %1, %2 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE " \
+                 "src0_sel:WORD_0 src1_sel:WORD_0" : "+v"(t_) : "v"(x), "v"(y));        \
+    x = t_;                                                                             \
+  } while (0)
+// ORDER 0: op-major; 1: one quarter-round at a time; 2: op-major with the
+// rotate by 16 as MGB_ROT16_PK; 3: op-major with xor + rotate by 16 as
+// MGB_XOR_ROT16_SDWA. All four compute the same words.
 template <int ORDER, int NQ = 8>
 __global__ __launch_bounds__(256) void qr_bench(uint32_t iters, uint32_t* __restrict__ out) {
   const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
@@ -228,13 +255,23 @@ __global__ __launch_bounds__(256) void qr_bench(uint32_t iters, uint32_t* __rest
     d[q] = gid * 7u + (uint32_t)q;
   }
   for (uint32_t i = 0; i < iters; ++i) {
-    if constexpr (ORDER == 0) {
+    if constexpr (ORDER != 1) {
 #pragma unroll
       for (int q = 0; q < NQ; q++) MGB_ADD(a[q], b[q]);
+      if constexpr (ORDER == 3) {
 #pragma unroll
-      for (int q = 0; q < NQ; q++) MGB_XOR(d[q], a[q]);
+        for (int q = 0; q < NQ; q++) MGB_XOR_ROT16_SDWA(d[q], a[q]);
+      } else {
 #pragma unroll
-      for (int q = 0; q < NQ; q++) MGB_ROT(d[q], 16);
+        for (int q = 0; q < NQ; q++) MGB_XOR(d[q], a[q]);
+        if constexpr (ORDER == 2) {
+#pragma unroll
+          for (int q = 0; q < NQ; q++) MGB_ROT16_PK(d[q]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < NQ; q++) MGB_ROT(d[q], 16);
+        }
+      }
 #pragma unroll
       for (int q = 0; q < NQ; q++) MGB_ADD(c[q], d[q]);
 #pragma unroll
@@ -268,13 +305,75 @@ __global__ __launch_bounds__(256) void qr_bench(uint32_t iters, uint32_t* __rest
   for (int q = 0; q < NQ; q++) r ^= a[q] ^ b[q] ^ c[q] ^ d[q];
   out[gid] = r;
 }
+// Eight quarter-rounds as two groups of four, group B one stage behind
+// group A, their ops interleaved one by one, so a rotate of one group sits
+// beside an add or xor of the other. 96 instructions per iteration.
+template <int K>
+__device__ __forceinline__ void qr_stage(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d) {
+  if constexpr (K == 0) MGB_ADD(a, b);
+  else if constexpr (K == 1) MGB_XOR(d, a);
+  else if constexpr (K == 2) MGB_ROT(d, 16);
+  else if constexpr (K == 3) MGB_ADD(c, d);
+  else if constexpr (K == 4) MGB_XOR(b, c);
+  else if constexpr (K == 5) MGB_ROT(b, 20);
+  else if constexpr (K == 6) MGB_ADD(a, b);
+  else if constexpr (K == 7) MGB_XOR(d, a);
+  else if constexpr (K == 8) MGB_ROT(d, 24);
+  else if constexpr (K == 9) MGB_ADD(c, d);
+  else if constexpr (K == 10) MGB_XOR(b, c);
+  else if constexpr (K == 11) MGB_ROT(b, 25);
+}
+
+template <int KA>
+__device__ __forceinline__ void skew_step(uint32_t (&a)[8], uint32_t (&b)[8], uint32_t (&c)[8],
+                                          uint32_t (&d)[8]) {
+  // Group A (QRs 0-3) at stage KA, group B (QRs 4-7) at stage KA - 1.
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    qr_stage<KA>(a[q], b[q], c[q], d[q]);
+    if constexpr (KA >= 1) qr_stage<KA - 1>(a[q + 4], b[q + 4], c[q + 4], d[q + 4]);
+  }
+}
+
+__global__ __launch_bounds__(256) void qr_skew_bench(uint32_t iters, uint32_t* __restrict__ out) {
+  const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
+  uint32_t a[8], b[8], c[8], d[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    a[q] = gid + (uint32_t)q;
+    b[q] = gid * 3u + (uint32_t)q;
+    c[q] = gid * 5u + (uint32_t)q;
+    d[q] = gid * 7u + (uint32_t)q;
+  }
+  for (uint32_t i = 0; i < iters; ++i) {
+    skew_step<0>(a, b, c, d);
+    skew_step<1>(a, b, c, d);
+    skew_step<2>(a, b, c, d);
+    skew_step<3>(a, b, c, d);
+    skew_step<4>(a, b, c, d);
+    skew_step<5>(a, b, c, d);
+    skew_step<6>(a, b, c, d);
+    skew_step<7>(a, b, c, d);
+    skew_step<8>(a, b, c, d);
+    skew_step<9>(a, b, c, d);
+    skew_step<10>(a, b, c, d);
+    skew_step<11>(a, b, c, d);
+#pragma unroll
+    for (int q = 0; q < 4; q++) qr_stage<11>(a[q + 4], b[q + 4], c[q + 4], d[q + 4]);
+  }
+  uint32_t r = 0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) r ^= a[q] ^ b[q] ^ c[q] ^ d[q];
+  out[gid] = r;
+}
+
 #undef MGB_ADD
 #undef MGB_XOR
 #undef MGB_ROT
 
-constexpr int kOpForms = 14;
+constexpr int kOpForms = 18;
 // mgb_mix forms: (K, MIX).
-constexpr int kMixForms = 19;
+constexpr int kMixForms = 22;
 
 }  // namespace mg
 
@@ -313,10 +412,12 @@ int timed(int device, uint32_t blocks, F launch, double* ms, uint32_t* checksum)
 
 extern "C" {
 
-// ordered: 0 = the compiler-scheduled expansion, 1 = the op-major ordered one.
+// ordered: 0 = the compiler-scheduled expansion, 1 = the op-major ordered one,
+// 2 = op-major with block 1 one stage behind block 0, 3 / 4 = op-major with
+// the rotate by 16 as a packed 16-bit add / two SDWA xors.
 int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters, double* ms,
              uint32_t* checksum) {
-  if (!ms || !checksum || !blocks) return (int)hipErrorInvalidValue;
+  if (!ms || !checksum || !blocks || ordered > 4) return (int)hipErrorInvalidValue;
   // A key with every correction word set, so half the nodes (t == 1) apply it.
   mg::DevKey hk{};
   for (int l = 0; l < mg::kMaxDomainBits; l++) {
@@ -335,10 +436,16 @@ int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters, doub
   const int rc = timed(
       device, blocks,
       [&](uint32_t* out, hipStream_t s) {
-        if (ordered)
-          hipLaunchKernelGGL(mg::prg_bench<true>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
+        if (ordered == 4)
+          hipLaunchKernelGGL(mg::prg_bench<4>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
+        else if (ordered == 3)
+          hipLaunchKernelGGL(mg::prg_bench<3>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
+        else if (ordered == 2)
+          hipLaunchKernelGGL(mg::prg_bench<2>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
+        else if (ordered == 1)
+          hipLaunchKernelGGL(mg::prg_bench<1>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
         else
-          hipLaunchKernelGGL(mg::prg_bench<false>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
+          hipLaunchKernelGGL(mg::prg_bench<0>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
         return hipGetLastError();
       },
       ms, checksum);
@@ -347,7 +454,7 @@ int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters, doub
 }
 
 int mgb_prg(int device, uint32_t blocks, uint32_t iters, double* ms, uint32_t* checksum) {
-  return mgb_prg2(device, mg::kPrgOrdered ? 1 : 0, blocks, iters, ms, checksum);
+  return mgb_prg2(device, (uint32_t)mg::kPrgVariant, blocks, iters, ms, checksum);
 }
 
 int mgb_valu(int device, uint32_t blocks, uint32_t iters, double* ms, uint32_t* checksum) {
@@ -361,7 +468,7 @@ int mgb_valu(int device, uint32_t blocks, uint32_t iters, double* ms, uint32_t* 
       ms, checksum);
 }
 
-// Form `op` (0 .. 13, see op_once) x 16 per lane per iteration.
+// Form `op` (0 .. 17, see op_once) x 16 per lane per iteration.
 int mgb_op(int device, uint32_t op, uint32_t blocks, uint32_t iters, double* ms,
            uint32_t* checksum) {
   if (!ms || !checksum || !blocks || op >= (uint32_t)mg::kOpForms)
@@ -374,7 +481,7 @@ int mgb_op(int device, uint32_t op, uint32_t blocks, uint32_t iters, double* ms,
   case K: hipLaunchKernelGGL(mg::op_bench<K>, dim3(blocks), dim3(256), 0, s, iters, out); break;
           MGB_CASE(0) MGB_CASE(1) MGB_CASE(2) MGB_CASE(3) MGB_CASE(4) MGB_CASE(5) MGB_CASE(6)
           MGB_CASE(7) MGB_CASE(8) MGB_CASE(9) MGB_CASE(10) MGB_CASE(11) MGB_CASE(12)
-          MGB_CASE(13)
+          MGB_CASE(13) MGB_CASE(14) MGB_CASE(15) MGB_CASE(16) MGB_CASE(17)
 #undef MGB_CASE
         }
         return hipGetLastError();
@@ -390,7 +497,10 @@ int mgb_op(int device, uint32_t op, uint32_t blocks, uint32_t iters, double* ms,
 // body unrolled 15 times (720 instructions, the size of the PRG loop); iters
 // must be a multiple of 15. Forms 14 and 15: fresh_bench<0>, <1>. Forms 16 and
 // 17: qr_bench<0>, <1> (iters / 2 iterations of 96). Form 18: 16 quarter-rounds
-// op-major (iters / 4 iterations of 192).
+// op-major (iters / 4 iterations of 192). Form 19: 8 quarter-rounds in two
+// groups skewed by one stage (iters / 2 iterations of 96). Forms 20 and 21:
+// qr_bench<2>, <3>, form 16 with the rotate by 16 as a packed 16-bit add (20)
+// or xor + rotate by 16 as two word-selecting SDWA xors (21).
 int mgb_mix(int device, uint32_t form, uint32_t blocks, uint32_t iters, double* ms,
             uint32_t* checksum) {
   if (!ms || !checksum || !blocks || form >= (uint32_t)mg::kMixForms)
@@ -421,6 +531,9 @@ int mgb_mix(int device, uint32_t form, uint32_t blocks, uint32_t iters, double* 
           case 17: hipLaunchKernelGGL((mg::qr_bench<1>), g, b, 0, s, iters / 2, out); break;
           // 16 quarter-rounds op-major: 192 instructions per iteration.
           case 18: hipLaunchKernelGGL((mg::qr_bench<0, 16>), g, b, 0, s, iters / 4, out); break;
+          case 19: hipLaunchKernelGGL(mg::qr_skew_bench, g, b, 0, s, iters / 2, out); break;
+          case 20: hipLaunchKernelGGL((mg::qr_bench<2>), g, b, 0, s, iters / 2, out); break;
+          case 21: hipLaunchKernelGGL((mg::qr_bench<3>), g, b, 0, s, iters / 2, out); break;
         }
         return hipGetLastError();
       },

@@ -461,11 +461,30 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   if (!parts && kernel == MG_KERNEL_LANE_DFS && lane_shape.tiles_per_wave)
     parts = (n_tiles + lane_shape.tiles_per_wave - 1) / lane_shape.tiles_per_wave;
   if (!parts) parts = std::max<uint64_t>(1, (uint64_t)resident / groups);
-  parts = std::min<uint64_t>(parts, n_tiles ? n_tiles : 1);
-  parts = std::min<uint64_t>(parts, 1u << 20);
-  pl->tiles_per_wave = (n_tiles + parts - 1) / parts;
-  if (pl->tiles_per_wave == 0) pl->tiles_per_wave = 1;
-  pl->parts = (uint32_t)((n_tiles + pl->tiles_per_wave - 1) / pl->tiles_per_wave);
+  if (kernel == MG_KERNEL_STREAM_SPLIT && pl->waves_per_block == 2) {
+    // Single-pair blocks take equal runs of leaf-pair steps, which may start and end inside a
+    // tile, so every resident pair gets work (78,643,200 x 1 KiB at kl 7, 1 key: 9600 tiles are
+    // 13 per pair on 739 of 768 pairs, 800 steps on all 768; 11.4-11.7 against 11.7-12.0 ms).
+    // The pairs of a larger block meet at a barrier every step and a step's work depends on
+    // its place in the tile, so they keep whole tiles, in phase (3 keys: 13.4-13.5 ms against
+    // 14.4 ms with runs out of phase).
+    const uint64_t n_pairs = 1ull << (pl->kl - 1);
+    const uint64_t total_steps = n_tiles * n_pairs;
+    parts = std::min<uint64_t>(parts, total_steps ? total_steps : 1);
+    parts = std::min<uint64_t>(parts, 1u << 20);
+    pl->steps_per_wave = (total_steps + parts - 1) / parts;
+    if (pl->steps_per_wave == 0) pl->steps_per_wave = 1;
+    pl->parts = (uint32_t)((total_steps + pl->steps_per_wave - 1) / pl->steps_per_wave);
+    // Tiles one pair touches at most.
+    pl->tiles_per_wave = (pl->steps_per_wave + n_pairs - 1) / n_pairs + 1;
+  } else {
+    parts = std::min<uint64_t>(parts, n_tiles ? n_tiles : 1);
+    parts = std::min<uint64_t>(parts, 1u << 20);
+    pl->tiles_per_wave = (n_tiles + parts - 1) / parts;
+    if (pl->tiles_per_wave == 0) pl->tiles_per_wave = 1;
+    pl->parts = (uint32_t)((n_tiles + pl->tiles_per_wave - 1) / pl->tiles_per_wave);
+    if (kernel == MG_KERNEL_STREAM_SPLIT) pl->steps_per_wave = pl->tiles_per_wave << (pl->kl - 1);
+  }
   if (pl->parts == 0) pl->parts = 1;
   if ((uint64_t)pl->parts * groups > (1ull << 30))
     return fail(MG_ERR_INVALID_ARG, "%u waves per key x %u key groups is too many", pl->parts,
@@ -506,6 +525,7 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       p.tile_begin = pl.tile_begin;
       p.tile_end = pl.tile_end;
       p.tiles_per_wave = pl.tiles_per_wave;
+      p.steps_per_wave = pl.steps_per_wave;
       p.parts = pl.parts;
       p.partial = b.partial;
       p.zero_rows = (const uint4*)dev->zero_rows.p;

@@ -43,14 +43,23 @@ MIX_FORMS = ["1 chain add", "2 chains add", "4 chains add", "8 chains add", "16 
              "8 chains x += z, z ^= x (both operands fresh)",
              "8 chains x += z, z ^= x, z = rotl(z) (both operands fresh)",
              "8 quarter-rounds, op-major order", "8 quarter-rounds, one QR at a time",
-             "16 quarter-rounds, op-major order"]
+             "16 quarter-rounds, op-major order",
+             "8 quarter-rounds, two groups skewed by one stage",
+             "8 quarter-rounds, op-major, rotate 16 as v_pk_add_u16 op_sel",
+             "8 quarter-rounds, op-major, xor + rotate 16 as two SDWA xors"]
 
 # Instruction forms of mgb_op (csrc/mg_microbench.hip, op_once).
 OP_FORMS = ["v_add_u32 (VOP2)", "v_xor_b32 (VOP2)", "v_alignbit_b32 x,x,x,25",
             "v_perm_b32 x,x,x,sel", "v_lshlrev_b32 7 (VOP2)", "v_bitop3_b32 0x78",
             "v_add3_u32", "v_xad_u32", "v_lshl_or_b32", "v_alignbyte_b32 x,x,x,2",
             "v_add_u32_e64 (VOP3)", "v_lshrrev_b32 25 (VOP2)", "v_or_b32 (VOP2)",
-            "v_alignbit_b32 x,y,25"]
+            "v_alignbit_b32 x,y,25", "v_pk_add_u16 op_sel swap (rotate 16)",
+            "v_pk_add_u16", "v_xor_b32_sdwa WORD_1 -> WORD_0, pad",
+            "v_xor_b32_sdwa WORD_0 -> WORD_1, preserve"]
+
+# Node expansions of mgb_prg2 (csrc/mg_device.hpp, expand_node).
+PRG_VARIANTS = ["compiler_order", "ordered", "skewed", "ordered_rot16_pk_add",
+                "ordered_xor_rot16_sdwa"]
 
 # int32 VALU ops per GGM node expansion, the constant every roofline here uses:
 # two ChaCha8 blocks after dead-code elimination plus the correction word.
@@ -73,7 +82,7 @@ def prg(blocks, iters, device=0):
 
 
 def prg_variant(ordered, blocks, iters, device=0):
-    """prg() with the compiler-scheduled (0) or op-major ordered (1) expansion."""
+    """prg() with expansion `ordered`, an index into PRG_VARIANTS."""
     ms = ctypes.c_double()
     chk = ctypes.c_uint32()
     rc = _lib.mgb_prg2(device, ordered, blocks, iters, ctypes.byref(ms), ctypes.byref(chk))
@@ -127,10 +136,10 @@ def main():
     lanes = blocks * 256
     prg_ms = min(prg(blocks, args.prg_iters, args.device)[0] for _ in range(args.reps + 1))
     variants = {}
-    for v in (0, 1):
+    for v in range(len(PRG_VARIANTS)):
         v_ms = min(prg_variant(v, blocks, args.prg_iters, args.device)[0]
                    for _ in range(args.reps + 1))
-        variants["ordered" if v else "compiler_order"] = {
+        variants[PRG_VARIANTS[v]] = {
             "ms": v_ms, "tops_at_700_ops_per_node":
                 lanes * args.prg_iters / (v_ms * 1e-3) * OPS_PER_NODE / 1e12}
     valu_ms = min(valu(blocks, args.valu_iters, args.device)[0] for _ in range(args.reps + 1))

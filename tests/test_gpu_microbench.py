@@ -46,8 +46,9 @@ def test_prg_microbench_matches_oracle():
         ms, chk = microbench.prg(1, iters)
         assert ms > 0
         assert chk == want, iters
-        # Both expansions: compiler-scheduled and op-major ordered.
-        for ordered in (0, 1):
+        # Every expansion: compiler-scheduled, op-major ordered, skewed, and
+        # op-major with the rotate by 16 as a packed add or SDWA xors.
+        for ordered in (0, 1, 2, 3, 4):
             assert microbench.prg_variant(ordered, 1, iters)[1] == want, (iters, ordered)
 
 
@@ -67,3 +68,16 @@ def test_valu_microbench_matches_its_definition():
             acc ^= x
         want ^= acc
     assert chk == want
+
+
+def test_rotate16_forms_compute_the_quarter_rounds():
+    """The packed-add and SDWA rotate-by-16 quarter-round forms compute the same
+    words as the alignbit form."""
+    from morphogenesis_amd import microbench
+    names = microbench.MIX_FORMS
+    base = names.index("8 quarter-rounds, op-major order")
+    for name in ("8 quarter-rounds, op-major, rotate 16 as v_pk_add_u16 op_sel",
+                 "8 quarter-rounds, op-major, xor + rotate 16 as two SDWA xors"):
+        for iters in (2, 6):
+            assert (microbench.mix(names.index(name), 1, iters)[1]
+                    == microbench.mix(base, 1, iters)[1]), (name, iters)

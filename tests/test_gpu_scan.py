@@ -142,6 +142,19 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
         got = scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=1,
                                                      waves_per_key=7, waves_per_block=wpb))
         assert np.array_equal(got, ref), wpb
+    # Pairs take runs of leaf-pair steps that start and end inside a tile, so a
+    # producer first walks down to its run's first pair: 3 and 7 runs per key,
+    # and 1000 (one or two steps per pair).
+    for g in range(1, max_g + 1):
+        for kl in sorted({3, d - 6}):
+            for wpk in (3, 7, 1000):
+                for wpb in (2, 8):
+                    if wpb // 2 * (g * (kl - 1) * 64 + 256 * g) * 16 > 160 * 1024:
+                        continue
+                    opts = mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=kl,
+                                        keys_per_wave=g, waves_per_key=wpk, waves_per_block=wpb)
+                    got = scanner.scan_keys(m, keys, opts)
+                    assert np.array_equal(got, ref), (g, kl, wpk, wpb)
     for bad in (1, 3, 6):
         with pytest.raises(mg.MgError):
             scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT,
