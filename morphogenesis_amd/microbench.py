@@ -59,7 +59,11 @@ OP_FORMS = ["v_add_u32 (VOP2)", "v_xor_b32 (VOP2)", "v_alignbit_b32 x,x,x,25",
 
 # Node expansions of mgb_prg2 (csrc/mg_device.hpp, expand_node).
 PRG_VARIANTS = ["compiler_order", "ordered", "skewed", "ordered_rot16_pk_add",
-                "ordered_xor_rot16_sdwa"]
+                "ordered_xor_rot16_sdwa", "ordered_round_statements"] + [
+                    f"ordered_round_statements_nop{a}_{r}" for a, r in (
+                        (1, 1), (2, 2), (3, 3), (2, 4), (2, 3), (2, 5), (2, 6), (3, 4),
+                        (3, 5), (1, 4), (1, 3), (3, 2), (4, 4), (2, 8), (1, 5), (3, 6))] + [
+                    "ordered_round_statements_nop2_5_round2_spaced"]
 
 # int32 VALU ops per GGM node expansion, the constant every roofline here uses:
 # two ChaCha8 blocks after dead-code elimination plus the correction word.

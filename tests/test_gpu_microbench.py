@@ -46,9 +46,8 @@ def test_prg_microbench_matches_oracle():
         ms, chk = microbench.prg(1, iters)
         assert ms > 0
         assert chk == want, iters
-        # Every expansion: compiler-scheduled, op-major ordered, skewed, and
-        # op-major with the rotate by 16 as a packed add or SDWA xors.
-        for ordered in (0, 1, 2, 3, 4):
+        # Every expansion variant.
+        for ordered in range(len(microbench.PRG_VARIANTS)):
             assert microbench.prg_variant(ordered, 1, iters)[1] == want, (iters, ordered)
 
 
