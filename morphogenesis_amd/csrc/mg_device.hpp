@@ -110,12 +110,8 @@ MG_ROTR_ASM(25)
 // An s_nop of N wait states (none for N = 0).
 template <int N>
 __device__ __forceinline__ void stage_nop() {
-  if constexpr (N == 1) asm volatile("s_nop 0");
-  else if constexpr (N == 2) asm volatile("s_nop 1");
-  else if constexpr (N == 3) asm volatile("s_nop 2");
-  else if constexpr (N == 4) asm volatile("s_nop 3");
-  else if constexpr (N == 5) asm volatile("s_nop 4");
-  else static_assert(N == 0, "s_nop of 0-5 wait states");
+  static_assert(N >= 0 && N <= 16, "s_nop of 0-16 wait states");
+  if constexpr (N > 0) asm volatile("s_nop %0" ::"i"(N - 1));
 }
 
 // Rotate by 16 as a packed 16-bit add of zero with the halves swapped.
@@ -142,15 +138,15 @@ __device__ __forceinline__ uint32_t op_xor_rotr16_hi(uint32_t lo, uint32_t d, ui
 
 // Stages 1..LAST of NQ quarter-rounds (a[q], b[q], c[q], d[q]) in op-major
 // order. R16 picks stages 2-3 (d ^= a; d = rotr16(d)): 0 xor + alignbit, 1 xor
-// + packed-add swap, 2 two SDWA xors. NA / NR > 0: an s_nop of that many
-// wait states before each add/xor / rotate stage, on top of the one the
+// + packed-add swap, 2 two SDWA xors. NA / NX / NR > 0: an s_nop of that many
+// wait states before each add / xor / rotate stage, on top of the one the
 // compiler puts before a stage whose ops read the previous stage's results.
-template <int NQ, int LAST, int R16 = 0, int NA = 0, int NR = 0>
+template <int NQ, int LAST, int R16 = 0, int NA = 0, int NR = 0, int NX = NA>
 __device__ __forceinline__ void qr_stages(uint32_t* a[], uint32_t* b[], uint32_t* c[],
                                           uint32_t* d[]) {
 #define MG_STAGE(K, EXPR)                                                      \
   if (LAST >= K) {                                                             \
-    stage_nop<(K % 3 == 0) ? NR : NA>();                                       \
+    stage_nop<(K % 3 == 0) ? NR : (K % 3 == 1) ? NA : NX>();                   \
     _Pragma("unroll") for (int q = 0; q < NQ; q++) { EXPR; }                   \
   }
   MG_STAGE(1, *a[q] = op_add(*a[q], *b[q]))
@@ -261,11 +257,11 @@ __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uin
 #define MG_A_DIAGS(S)                                                              \
   S(0, 5, 10, 15) S(1, 6, 11, 12) S(2, 7, 8, 13) S(3, 4, 9, 14) S(16, 21, 26, 31) \
   S(17, 22, 27, 28) S(18, 23, 24, 29) S(19, 20, 25, 30)
-// N: text put before each add/xor stage, R: before each rotate stage (empty,
-// or an s_nop).
-#define MG_A_ROUNDN(QRS, N, R)                                                          \
-  N QRS(MG_A_S1) N QRS(MG_A_S2) R QRS(MG_A_S3) N QRS(MG_A_S4) N QRS(MG_A_S5) R          \
-  QRS(MG_A_S6) N QRS(MG_A_S1) N QRS(MG_A_S2) R QRS(MG_A_S9) N QRS(MG_A_S4) N            \
+// A / X / R: text put before each add / xor / rotate stage (empty, or an
+// s_nop).
+#define MG_A_ROUNDN(QRS, A, X, R)                                                       \
+  A QRS(MG_A_S1) X QRS(MG_A_S2) R QRS(MG_A_S3) A QRS(MG_A_S4) X QRS(MG_A_S5) R          \
+  QRS(MG_A_S6) A QRS(MG_A_S1) X QRS(MG_A_S2) R QRS(MG_A_S9) A QRS(MG_A_S4) X            \
   QRS(MG_A_S5) R QRS(MG_A_S12)
 // Round 8: block 0's diagonals in full; block 1's quarter-rounds through its
 // words 0 and 1, (0, 5, 10, 15) and (1, 6, 11, 12), as %16-%19 and %20-%23,
@@ -273,10 +269,10 @@ __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uin
 #define MG_A_DIAGS6(S) \
   S(0, 5, 10, 15) S(1, 6, 11, 12) S(2, 7, 8, 13) S(3, 4, 9, 14) S(16, 17, 18, 19) S(20, 21, 22, 23)
 #define MG_A_DIAGS4(S) S(0, 5, 10, 15) S(1, 6, 11, 12) S(2, 7, 8, 13) S(3, 4, 9, 14)
-#define MG_A_ROUND8N(N, R)                                                                  \
-  N MG_A_DIAGS6(MG_A_S1) N MG_A_DIAGS6(MG_A_S2) R MG_A_DIAGS6(MG_A_S3) N                    \
-  MG_A_DIAGS6(MG_A_S4) N MG_A_DIAGS6(MG_A_S5) R MG_A_DIAGS6(MG_A_S6) N MG_A_DIAGS6(MG_A_S1) \
-  N MG_A_DIAGS4(MG_A_S2) R MG_A_DIAGS4(MG_A_S9) N MG_A_DIAGS4(MG_A_S4) N                    \
+#define MG_A_ROUND8N(A, X, R)                                                               \
+  A MG_A_DIAGS6(MG_A_S1) X MG_A_DIAGS6(MG_A_S2) R MG_A_DIAGS6(MG_A_S3) A                    \
+  MG_A_DIAGS6(MG_A_S4) X MG_A_DIAGS6(MG_A_S5) R MG_A_DIAGS6(MG_A_S6) A MG_A_DIAGS6(MG_A_S1) \
+  X MG_A_DIAGS4(MG_A_S2) R MG_A_DIAGS4(MG_A_S9) A MG_A_DIAGS4(MG_A_S4) X                    \
   MG_A_DIAGS4(MG_A_S5) R MG_A_DIAGS4(MG_A_S12)
 #define MG_A_X32(x, y)                                                                 \
   "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(x[4]), "+v"(x[5]), "+v"(x[6]),  \
@@ -288,25 +284,18 @@ __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uin
 // As prg_expand_ordered (same op order), with rounds 3-8 as one asm statement
 // each: 6 hazard nops for those rounds instead of about 63. Round 2 keeps one
 // statement per op: its in-place form would need copies of the twelve words
-// the blocks still share after round 1.
-// s_nop text of K wait states (K = 0: none).
-#define MG_NOP_S_0 ""
-#define MG_NOP_S_1 "s_nop 0\n\t"
-#define MG_NOP_S_2 "s_nop 1\n\t"
-#define MG_NOP_S_3 "s_nop 2\n\t"
-#define MG_NOP_S_4 "s_nop 3\n\t"
-#define MG_NOP_S_5 "s_nop 4\n\t"
-#define MG_NOP_S_6 "s_nop 5\n\t"
-#define MG_NOP_S_8 "s_nop 7\n\t"
-
-// NA / NR: wait states of the s_nop put before each add/xor / rotate stage of
-// rounds 3-8 (NA = NR = 1 is the instruction stream of prg_expand_ordered).
-// R2N: also space round 2's stages (NA - 1 / NR - 1 wait states added to the
-// compiler's one).
-template <int NA = 0, int NR = 0, bool R2N = false>
+// the blocks still share after round 1. NA / NX / NR: wait states of the s_nop
+// put before each add / xor / rotate stage of rounds 3-8 (0 for all three: no
+// s_nop; otherwise each at least 1; 1, 1, 1 is the instruction stream of
+// prg_expand_ordered). R2N: space round 2's stages the same way (NA - 1,
+// NX - 1, NR - 1 wait states on top of the compiler's one).
+template <int NA = 0, int NX = 0, int NR = 0, bool R2N = false>
 __device__ __forceinline__ void prg_expand_rounds(uint32_t s0, uint32_t s1, uint32_t s2,
                                                   uint32_t s3, uint32_t l[4], uint32_t r[4],
                                                   uint32_t& lt, uint32_t& rt) {
+  static_assert((NA == 0 && NX == 0 && NR == 0) || (NA >= 1 && NX >= 1 && NR >= 1 &&
+                                                    NA <= 16 && NX <= 16 && NR <= 16),
+                "s_nop spacing of 1-16 wait states, or none");
   uint32_t x0 = kC0, x1 = kC1, x2 = kC2, x3 = kC3;
   uint32_t x4 = s0, x5 = s1, x6 = s2, x7 = s3;
   uint32_t x8 = s0, x9 = s1, x10 = s2, x11 = s3;
@@ -326,36 +315,49 @@ __device__ __forceinline__ void prg_expand_rounds(uint32_t s0, uint32_t s1, uint
     uint32_t* b[8] = {&x5, &x6, &x7, &x4, &y5, &y6, &y7, &y4};
     uint32_t* c[8] = {&x10, &x11, &x8, &x9, &y10, &y11, &y8, &y9};
     uint32_t* d[8] = {&x15, &x12, &x13, &x14, &y15, &y12, &y13, &y14};
-    if constexpr (R2N && NA > 0 && NR > 0)
-      qr_stages<8, 12, 0, NA - 1, NR - 1>(a, b, c, d);
+    if constexpr (R2N && NA > 0)
+      qr_stages<8, 12, 0, NA - 1, NR - 1, NX - 1>(a, b, c, d);
     else
       qr_stages<8, 12>(a, b, c, d);
   }
   uint32_t x[16] = {x0, x1, x2, x3, x4, x5, x6, x7, x8, x9, x10, x11, x12, x13, x14, x15};
   uint32_t y[16] = {y0, y1, y2, y3, y4, y5, y6, y7, y8, y9, y10, y11, y12, y13, y14, y15};
   uint32_t z[8];
-#define MG_A_BODY(N, R)                                                                     \
-  asm volatile(MG_A_ROUNDN(MG_A_COLS, N, R) : MG_A_X32(x, y));  /* round 3 */               \
-  asm volatile(MG_A_ROUNDN(MG_A_DIAGS, N, R) : MG_A_X32(x, y)); /* round 4 */               \
-  asm volatile(MG_A_ROUNDN(MG_A_COLS, N, R) : MG_A_X32(x, y));  /* round 5 */               \
-  asm volatile(MG_A_ROUNDN(MG_A_DIAGS, N, R) : MG_A_X32(x, y)); /* round 6 */               \
-  asm volatile(MG_A_ROUNDN(MG_A_COLS, N, R) : MG_A_X32(x, y));  /* round 7 */               \
-  z[0] = y[0]; z[1] = y[5]; z[2] = y[10]; z[3] = y[15];                                     \
-  z[4] = y[1]; z[5] = y[6]; z[6] = y[11]; z[7] = y[12];                                     \
-  asm volatile(MG_A_ROUND8N(N, R)                                                           \
-               : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(x[4]), "+v"(x[5]),    \
-                 "+v"(x[6]), "+v"(x[7]), "+v"(x[8]), "+v"(x[9]), "+v"(x[10]), "+v"(x[11]),  \
-                 "+v"(x[12]), "+v"(x[13]), "+v"(x[14]), "+v"(x[15]), "+v"(z[0]), "+v"(z[1]), \
-                 "+v"(z[2]), "+v"(z[3]), "+v"(z[4]), "+v"(z[5]), "+v"(z[6]), "+v"(z[7]));
-#define MG_A_TRY(A, R) if constexpr (NA == A && NR == R) { MG_A_BODY(MG_NOP_S_##A, MG_NOP_S_##R) } else
-#define MG_A_TRY_R(A) MG_A_TRY(A, 0) MG_A_TRY(A, 1) MG_A_TRY(A, 2) MG_A_TRY(A, 3) \
-  MG_A_TRY(A, 4) MG_A_TRY(A, 5) MG_A_TRY(A, 6) MG_A_TRY(A, 8)
-  MG_A_TRY_R(0) MG_A_TRY_R(1) MG_A_TRY_R(2) MG_A_TRY_R(3) MG_A_TRY_R(4) {
-    static_assert(NA < 0, "no s_nop text for this setting");
+  if constexpr (NA == 0) {
+    asm volatile(MG_A_ROUNDN(MG_A_COLS, "", "", "") : MG_A_X32(x, y));   // round 3
+    asm volatile(MG_A_ROUNDN(MG_A_DIAGS, "", "", "") : MG_A_X32(x, y));  // round 4
+    asm volatile(MG_A_ROUNDN(MG_A_COLS, "", "", "") : MG_A_X32(x, y));   // round 5
+    asm volatile(MG_A_ROUNDN(MG_A_DIAGS, "", "", "") : MG_A_X32(x, y));  // round 6
+    asm volatile(MG_A_ROUNDN(MG_A_COLS, "", "", "") : MG_A_X32(x, y));   // round 7
+  } else {
+    // The s_nop lengths are operands %32-%34 (immediates).
+#define MG_A_R(QRS)                                                                     \
+  asm volatile(MG_A_ROUNDN(QRS, "s_nop %32\n\t", "s_nop %33\n\t", "s_nop %34\n\t")      \
+               : MG_A_X32(x, y)                                                         \
+               : "i"(NA - 1), "i"(NX - 1), "i"(NR - 1))
+    MG_A_R(MG_A_COLS);   // round 3
+    MG_A_R(MG_A_DIAGS);  // round 4
+    MG_A_R(MG_A_COLS);   // round 5
+    MG_A_R(MG_A_DIAGS);  // round 6
+    MG_A_R(MG_A_COLS);   // round 7
+#undef MG_A_R
   }
-#undef MG_A_TRY_R
-#undef MG_A_TRY
-#undef MG_A_BODY
+  z[0] = y[0]; z[1] = y[5]; z[2] = y[10]; z[3] = y[15];
+  z[4] = y[1]; z[5] = y[6]; z[6] = y[11]; z[7] = y[12];
+#define MG_A_Z24                                                                       \
+  "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(x[4]), "+v"(x[5]), "+v"(x[6]),  \
+  "+v"(x[7]), "+v"(x[8]), "+v"(x[9]), "+v"(x[10]), "+v"(x[11]), "+v"(x[12]),            \
+  "+v"(x[13]), "+v"(x[14]), "+v"(x[15]), "+v"(z[0]), "+v"(z[1]), "+v"(z[2]), "+v"(z[3]), \
+  "+v"(z[4]), "+v"(z[5]), "+v"(z[6]), "+v"(z[7])
+  if constexpr (NA == 0) {
+    asm volatile(MG_A_ROUND8N("", "", "") : MG_A_Z24);
+  } else {
+    // The s_nop lengths are operands %24-%26.
+    asm volatile(MG_A_ROUND8N("s_nop %24\n\t", "s_nop %25\n\t", "s_nop %26\n\t")
+                 : MG_A_Z24
+                 : "i"(NA - 1), "i"(NX - 1), "i"(NR - 1));
+  }
+#undef MG_A_Z24
   l[0] = x[0] + kC0; l[1] = x[1] + kC1; l[2] = x[2] + kC2; l[3] = x[3] + kC3;
   r[0] = x[4] + s0; r[1] = x[5] + s1; r[2] = x[6] + s2; r[3] = x[7] + s3;
   lt = (z[0] + kC0) & 1u;
@@ -453,14 +455,20 @@ __device__ __forceinline__ uint32_t t_mask(uint32_t t) {
 // op-major, 2 op-major with block 1 one stage behind block 0, 3 and 4
 // op-major with the rotate by 16 as a packed add (3) or SDWA xors (4), 5 the
 // op-major order with rounds 3-8 as one asm statement each, 6-21 variant 5
-// with the s_nops of kPrgRoundNops before its stages, 22 variant 11 (two wait
-// states before add/xor stages, five before rotates) with round 2 spaced the
-// same way. The kernels use 22: 6-7 % more node expansions per second than 1
-// at 5-8 waves per SIMD.
-constexpr int kPrgVariant = 22;
-// prg_expand_rounds' (NA, NR) of variants 6, 7, ...
+// with the s_nops of kPrgRoundNops before its stages, 22 and on rounds 2-8
+// spaced as kPrgRoundNops3 says. The kernels use 33 (two wait states before
+// add and xor stages, six before rotates): 4.5 % more node expansions per
+// second than 1 at 4 waves per SIMD, 7 % at 5, 9.5 % at 8.
+constexpr int kPrgVariant = 33;
+// prg_expand_rounds' (NA = NX, NR) of variants 6-21, rounds 3-8 spaced only.
 constexpr int kPrgRoundNops[][2] = {{1, 1}, {2, 2}, {3, 3}, {2, 4}, {2, 3}, {2, 5}, {2, 6}, {3, 4},
                                     {3, 5}, {1, 4}, {1, 3}, {3, 2}, {4, 4}, {2, 8}, {1, 5}, {3, 6}};
+// (NA, NX, NR) of variants 22, 23, ..., rounds 2-8 spaced.
+constexpr int kPrgRoundNops3[][3] = {{2, 2, 5}, {1, 2, 5}, {1, 3, 5}, {2, 3, 5}, {1, 2, 6},
+                                     {1, 3, 6}, {2, 3, 6}, {1, 2, 4}, {2, 3, 4}, {1, 4, 5},
+                                     {3, 2, 5}, {2, 2, 6}, {2, 4, 5}, {2, 2, 7}, {2, 2, 8},
+                                     {3, 3, 7}, {2, 1, 6}, {1, 1, 6}, {3, 2, 6}, {2, 3, 7},
+                                     {1, 2, 7}, {3, 3, 8}, {2, 2, 10}};
 
 // Both children of `n` at tree level `level`, correction word applied when
 // n.t == 1 (dpf.rs:221-239). `k` is wave-uniform, so the correction word is
@@ -505,12 +513,14 @@ __device__ __forceinline__ void expand_node(const DevKey* __restrict__ k,
   else if constexpr (kVariant == 4)
     prg_expand_ordered<2>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
   else if constexpr (kVariant == 5)
-    prg_expand_rounds<0, 0>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
-  else if constexpr (kVariant == 22)
-    prg_expand_rounds<2, 5, true>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
-  else if constexpr (kVariant >= 6)
-    prg_expand_rounds<kPrgRoundNops[kVariant - 6][0], kPrgRoundNops[kVariant - 6][1]>(
-        n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
+    prg_expand_rounds<0, 0, 0>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
+  else if constexpr (kVariant >= 6 && kVariant <= 21)
+    prg_expand_rounds<kPrgRoundNops[kVariant - 6][0], kPrgRoundNops[kVariant - 6][0],
+                      kPrgRoundNops[kVariant - 6][1]>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
+  else if constexpr (kVariant >= 22)
+    prg_expand_rounds<kPrgRoundNops3[kVariant - 22][0], kPrgRoundNops3[kVariant - 22][1],
+                      kPrgRoundNops3[kVariant - 22][2], true>(n.s0, n.s1, n.s2, n.s3, l, r, lt,
+                                                             rt);
   else if constexpr (kVariant == 1)
     prg_expand_ordered(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
   else

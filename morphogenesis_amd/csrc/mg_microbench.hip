@@ -15,7 +15,7 @@
 //   int mgb_prg(int device, uint32_t blocks, uint32_t iters, double* ms,
 //               uint32_t* checksum)           (the kernels' expansion)
 //   int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters,
-//                double* ms, uint32_t* checksum)  (expansion variant 0-22)
+//                double* ms, uint32_t* checksum)  (expansion variant 0-44)
 //   int mgb_valu(int device, uint32_t blocks, uint32_t iters, double* ms,
 //                uint32_t* checksum)
 // Both launch `blocks` blocks of 256 threads, time the launch with HIP events
@@ -416,11 +416,11 @@ extern "C" {
 // 2 = op-major with block 1 one stage behind block 0, 3 / 4 = op-major with
 // the rotate by 16 as a packed 16-bit add / two SDWA xors, 5 = op-major with
 // rounds 3-8 as one asm statement each, 6-21 = 5 with the s_nops of
-// kPrgRoundNops (mg_device.hpp) before its stages, 22 = variant 11 with
-// round 2's stages spaced the same way.
+// kPrgRoundNops (mg_device.hpp) before its stages, 22-44 = rounds 2-8 spaced
+// as kPrgRoundNops3 says (add / xor / rotate stages).
 int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters, double* ms,
              uint32_t* checksum) {
-  if (!ms || !checksum || !blocks || ordered > 22) return (int)hipErrorInvalidValue;
+  if (!ms || !checksum || !blocks || ordered > 44) return (int)hipErrorInvalidValue;
   // A key with every correction word set, so half the nodes (t == 1) apply it.
   mg::DevKey hk{};
   for (int l = 0; l < mg::kMaxDomainBits; l++) {
@@ -445,7 +445,11 @@ int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters, doub
   case V: hipLaunchKernelGGL(mg::prg_bench<V>, dim3(blocks), dim3(256), 0, s, dk, iters, out); break;
             MGB_PRG(6) MGB_PRG(7) MGB_PRG(8) MGB_PRG(9) MGB_PRG(10) MGB_PRG(11) MGB_PRG(12)
             MGB_PRG(13) MGB_PRG(14) MGB_PRG(15) MGB_PRG(16) MGB_PRG(17) MGB_PRG(18)
-            MGB_PRG(19) MGB_PRG(20) MGB_PRG(21) MGB_PRG(22)
+            MGB_PRG(19) MGB_PRG(20) MGB_PRG(21) MGB_PRG(22) MGB_PRG(23) MGB_PRG(24)
+            MGB_PRG(25) MGB_PRG(26) MGB_PRG(27) MGB_PRG(28) MGB_PRG(29) MGB_PRG(30)
+            MGB_PRG(31) MGB_PRG(32) MGB_PRG(33) MGB_PRG(34) MGB_PRG(35) MGB_PRG(36)
+            MGB_PRG(37) MGB_PRG(38) MGB_PRG(39) MGB_PRG(40) MGB_PRG(41) MGB_PRG(42)
+            MGB_PRG(43) MGB_PRG(44)
 #undef MGB_PRG
           }
         } else if (ordered == 5)

@@ -63,7 +63,11 @@ PRG_VARIANTS = ["compiler_order", "ordered", "skewed", "ordered_rot16_pk_add",
                     f"ordered_round_statements_nop{a}_{r}" for a, r in (
                         (1, 1), (2, 2), (3, 3), (2, 4), (2, 3), (2, 5), (2, 6), (3, 4),
                         (3, 5), (1, 4), (1, 3), (3, 2), (4, 4), (2, 8), (1, 5), (3, 6))] + [
-                    "ordered_round_statements_nop2_5_round2_spaced"]
+                    f"ordered_rounds_2_to_8_spaced_add{a}_xor{x}_rot{r}" for a, x, r in (
+                        (2, 2, 5), (1, 2, 5), (1, 3, 5), (2, 3, 5), (1, 2, 6), (1, 3, 6),
+                        (2, 3, 6), (1, 2, 4), (2, 3, 4), (1, 4, 5), (3, 2, 5), (2, 2, 6),
+                        (2, 4, 5), (2, 2, 7), (2, 2, 8), (3, 3, 7), (2, 1, 6), (1, 1, 6),
+                        (3, 2, 6), (2, 3, 7), (1, 2, 7), (3, 3, 8), (2, 2, 10))]
 
 # int32 VALU ops per GGM node expansion, the constant every roofline here uses:
 # two ChaCha8 blocks after dead-code elimination plus the correction word.
