@@ -460,6 +460,13 @@ __device__ __forceinline__ uint32_t t_mask(uint32_t t) {
 // add and xor stages, six before rotates): 4.5 % more node expansions per
 // second than 1 at 4 waves per SIMD, 7 % at 5, 9.5 % at 8.
 constexpr int kPrgVariant = 33;
+// The node routine of the stream and split kernels' DFS waves: the unspaced
+// op-major one. They run one to three DFS waves per SIMD, where the spacing
+// s_nops cost more than they save (78,643,200 x 1 KiB, 4 keys: 14.40-14.61
+// against 14.87-15.45 ms; 238,609,294 x 144 B, 1 key: 9.57-9.61
+// against 9.83-9.89 ms).
+constexpr int kStreamPrgVariant = 1;
+constexpr int kSplitPrgVariant = 1;
 // prg_expand_rounds' (NA = NX, NR) of variants 6-21, rounds 3-8 spaced only.
 constexpr int kPrgRoundNops[][2] = {{1, 1}, {2, 2}, {3, 3}, {2, 4}, {2, 3}, {2, 5}, {2, 6}, {3, 4},
                                     {3, 5}, {1, 4}, {1, 3}, {3, 2}, {4, 4}, {2, 8}, {1, 5}, {3, 6}};

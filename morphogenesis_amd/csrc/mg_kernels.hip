@@ -457,7 +457,7 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
       for (int g = 0; g < G; g++) {
         uint4* st_seed = base_lds + (size_t)g * (kl - 1) * 64;
         Node L, R;
-        expand_node<kPrgVariant, false>(kp[g], leaf_level + lvl, node[g], L, R);
+        expand_node<kStreamPrgVariant, false>(kp[g], leaf_level + lvl, node[g], L, R);
         if (lvl == kl - 1) {
           masks[g * 128 + 2 * lane] = leaf_mask<false>(kp[g], L);
           masks[g * 128 + 2 * lane + 1] = leaf_mask<false>(kp[g], R);
@@ -601,7 +601,7 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
             uint4* st_seed = stacks + (size_t)g * slots * 64;
             for (int l = 0; l < kl - 1; ++l) {
               Node L, R;
-              expand_node<kPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
+              expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
               if ((pair >> (kl - 2 - l)) & 1u) {
                 node[g] = R;
               } else {
@@ -625,13 +625,13 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
           }
           Node L, R;
           for (int l = lvl; l < kl - 1; ++l) {
-            expand_node<kPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
+            expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
             st_seed[l * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth l + 1
             const uint32_t bit = 1u << (l + 1);
             st_tbits[g] = R.t == 1u ? (st_tbits[g] | bit) : (st_tbits[g] & ~bit);
             node[g] = L;
           }
-          expand_node<kPrgVariant, false>(kp[g], leaf_level + kl - 1, node[g], L, R);
+          expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + kl - 1, node[g], L, R);
           mbuf[g * 128 + 2 * lane] = leaf_mask<false>(kp[g], L);
           mbuf[g * 128 + 2 * lane + 1] = leaf_mask<false>(kp[g], R);
         }
