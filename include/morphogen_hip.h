@@ -70,6 +70,8 @@ typedef struct {
   uint32_t lane_depth;   /* kl actually used                               */
   uint32_t keys_per_wave;
   uint32_t waves;        /* waves launched for the fused scan              */
+  uint32_t stack_global; /* lane-DFS stack levels kept in global memory    */
+  uint32_t reserved;
 } mg_timing;
 
 enum {
@@ -105,6 +107,11 @@ typedef struct {
                             /* whole matrix; row_end 0 = n_rows            */
   const mg_delta *delta;    /* if set: answers ^= delta contribution
                                (scan_delta_for_gpu, scan.rs:434-485)       */
+  uint32_t stack_global;    /* lane-DFS: 0 = tuned, else 1 + the number of
+                               top stack levels kept in global memory
+                               instead of LDS (0-4; 1 = none)              */
+  uint32_t row_loads;       /* lane-DFS: 0 = tuned, 1 = a leaf pair's rows
+                               are loaded before its expansions, 2 = after */
 } mg_scan_opts;
 
 int mg_abi_version(void);

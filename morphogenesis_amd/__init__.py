@@ -76,7 +76,8 @@ class Timing(ctypes.Structure):
                 ("d2h_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("scan_kernel_ms", ctypes.c_double), ("kernel", ctypes.c_uint32),
                 ("lane_depth", ctypes.c_uint32), ("keys_per_wave", ctypes.c_uint32),
-                ("waves", ctypes.c_uint32)]
+                ("waves", ctypes.c_uint32), ("stack_global", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
@@ -89,7 +90,8 @@ class ScanOpts(ctypes.Structure):
                 ("keys_per_wave", ctypes.c_uint32), ("waves_per_key", ctypes.c_uint32),
                 ("waves_per_block", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
                 ("row_begin", ctypes.c_uint64), ("row_end", ctypes.c_uint64),
-                ("delta", ctypes.c_void_p)]
+                ("delta", ctypes.c_void_p), ("stack_global", ctypes.c_uint32),
+                ("row_loads", ctypes.c_uint32)]
 
 
 class WireOpts(ctypes.Structure):
@@ -271,7 +273,11 @@ class PirResult:
 
 
 def make_opts(kernel=KERNEL_AUTO, lane_depth=0, keys_per_wave=0, waves_per_key=0,
-              waves_per_block=0, use_graph=False, row_begin=0, row_end=0, delta=None):
+              waves_per_block=0, use_graph=False, row_begin=0, row_end=0, delta=None,
+              stack_global=None, late_loads=None):
+    """stack_global: lane-DFS stack levels kept in global memory (None = tuned);
+    late_loads: lane-DFS rows loaded after a leaf pair's expansions (True) or
+    before them (False); None = tuned."""
     o = ScanOpts()
     o.kernel = kernel
     o.lane_depth = lane_depth
@@ -282,6 +288,8 @@ def make_opts(kernel=KERNEL_AUTO, lane_depth=0, keys_per_wave=0, waves_per_key=0
     o.row_begin = row_begin
     o.row_end = row_end
     o.delta = delta._h if delta is not None else None
+    o.stack_global = 0 if stack_global is None else 1 + stack_global
+    o.row_loads = 0 if late_loads is None else (2 if late_loads else 1)
     return o
 
 

@@ -60,6 +60,8 @@ struct Plan {
   uint32_t parts;
   uint64_t tile_begin, tile_end, tiles_per_wave;
   uint64_t steps_per_wave;  // split kernel: leaf-pair steps per DFS/stream pair
+  uint32_t gslots;          // lane-DFS: stack levels kept in global memory
+  uint32_t late_loads;      // lane-DFS: rows loaded after a step's expansions
   uint64_t row_begin, row_end;
   uint64_t n_delta;
   const void* rows;
@@ -71,6 +73,7 @@ struct Plan {
   const void* ws_out;
   const void* ws_partial;
   const void* ws_partial_delta;
+  const void* ws_gstack;
   Plan();
   bool operator==(const Plan& o) const;
 };
@@ -84,6 +87,7 @@ struct ScanBuffers {
   uint4* out = nullptr;
   const uint32_t* out_idx = nullptr;  // launch key i -> out row out_idx[i]
   bool accumulate = false;            // out ^= scan instead of out = scan
+  uint4* gstack = nullptr;            // >= plan.parts * n_keys * plan.gslots * 1 KiB
 };
 
 }  // namespace mg
@@ -96,6 +100,7 @@ struct mg_device {
   hipEvent_t ev[5] = {};
   std::mutex mu;  // serialises every use of `stream` and the workspace
   mg::DevBuf keys, partial, partial_delta, out;
+  mg::DevBuf gstack;  // lane-DFS stack levels kept in global memory
   mg::DevBuf zero_rows;  // 4 KiB of zeros, read in place of rows past the end
   mg::HostBuf h_keys, h_out;
   // HIP graph of the kernel sequence for the last plan.
@@ -147,6 +152,11 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
 
 // Enqueues scan kernel, optional delta kernel and reductions on dev->stream.
 // Events (may be null) bracket the fused scan kernel.
+// Bytes of the lane-DFS global stack levels a plan's launch needs (0 if none).
+inline size_t lane_gstack_bytes(const Plan& pl, uint32_t n_keys) {
+  return (size_t)pl.parts * n_keys * pl.gslots * 64 * 16;
+}
+
 int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_t ev_scan0,
                  hipEvent_t ev_scan1);
 

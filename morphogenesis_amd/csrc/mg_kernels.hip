@@ -129,7 +129,7 @@ __device__ __forceinline__ Node tile_lane_root(const DevKey* __restrict__ k,
 
 // -------------------------------------------------------- lane-DFS scan ----
 
-template <int C>
+template <int C, bool kLateLoads, bool kGlobalStack>
 __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   extern __shared__ uint4 lds[];
   const int lane = threadIdx.x & 63;
@@ -148,12 +148,17 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
 
   // Per-wave DFS stack: the right children waiting at depths 1 .. kl-1 of
-  // the lane subtree (depth 0, the lane root, is never pushed). Seeds sit in
-  // LDS, [kl-1][64]; each child's control byte is kept as the one bit that
-  // matters, t == 1 (dpf.rs:221-239 and 136-141 test nothing else), in bit
-  // `depth` of a per-lane register. At kl = 10 a wave needs 9 KiB of LDS, so
-  // four 4-wave blocks fit per CU.
-  uint4* st_seed = lds + (size_t)wib * (kl - 1) * 64;  // [kl-1][64] seeds
+  // the lane subtree (depth 0, the lane root, is never pushed). Seeds of
+  // depths 1 .. gs sit in global memory (a wave's own [gs][64] region), the
+  // deeper ones in LDS, [kl-1-gs][64]: depth-l slots are used 2^(l-1) times a
+  // tile, so the few top ones cost nothing in global memory while their LDS
+  // would limit the waves per CU. Each child's control byte is kept as the one
+  // bit that matters, t == 1 (dpf.rs:221-239 and 136-141 test nothing else),
+  // in bit `depth` of a per-lane register.
+  // (kGlobalStack false: gslots is 0 and none of the global-slot code exists.)
+  const int gs = kGlobalStack ? (int)p.gslots : 0;
+  uint4* st_seed = lds + (size_t)wib * (kl - 1 - gs) * 64;  // slot s >= gs at s - gs
+  uint4* st_glob = kGlobalStack ? p.gstack + (size_t)(live ? w : 0) * gs * 64 : nullptr;
   uint32_t st_tbits = 0;
 
   uint4 acc[C];
@@ -180,17 +185,21 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
     Node node = tile_lane_root(k, path, lane, tile, tile - 1, tile == t_begin, tb, lb);
     const uint64_t base = (tile << (kl + lb)) | ((uint64_t)lane << kl);
 
-    // One iteration per leaf pair: pop the pending right child (if any), issue
-    // the pair's row loads, walk down to the bottom node pushing right
-    // children, expand it into the two leaves and fold the rows. The loads
-    // are in flight for at least the bottom expansion, and their registers
-    // live within one iteration. Loads are unconditional: a row past n_rows
-    // (or of an idle lane) is read from a zero buffer instead.
+    // One iteration per leaf pair: pop the pending right child (if any), walk
+    // down to the bottom node pushing right children, expand it into the two
+    // leaves and fold the pair's rows into the accumulators. The rows are
+    // loaded at the start of the step (kLateLoads false: in flight during the
+    // expansions) or after the expansions (true: the rows stay out of the
+    // registers the ChaCha rounds need, 69 VGPRs for 32-byte rows against 78,
+    // seven waves per SIMD against six, and the other waves on the SIMD run
+    // while a wave waits for its rows). Loads are unconditional: a row past
+    // n_rows (or of an idle lane) is read from a zero buffer instead.
     int lvl = 0;
     for (uint32_t pair = 0; pair < n_pairs; ++pair) {
       if (pair) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
         lvl = kl - 1 - __builtin_ctz(pair);
-        const uint4 sv = st_seed[(lvl - 1) * 64 + lane];
+        const uint4 sv = (kGlobalStack && lvl - 1 < gs) ? st_glob[(lvl - 1) * 64 + lane]
+                                                        : st_seed[(lvl - 1 - gs) * 64 + lane];
         node.s0 = sv.x; node.s1 = sv.y; node.s2 = sv.z; node.s3 = sv.w;
         node.t = (st_tbits >> lvl) & 1u;
       }
@@ -198,21 +207,34 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
       const uint4* src0 = (lane_active && x < p.n_rows) ? p.rows + x * C : p.zero_rows;
       const uint4* src1 = (lane_active && x + 1 < p.n_rows) ? p.rows + (x + 1) * C : p.zero_rows;
       uint4 r0[C], r1[C];
+      if constexpr (!kLateLoads) {
 #pragma unroll
-      for (int c = 0; c < C; c++) {
-        r0[c] = src0[c];
-        r1[c] = src1[c];
+        for (int c = 0; c < C; c++) {
+          r0[c] = src0[c];
+          r1[c] = src1[c];
+        }
       }
       for (; lvl < kl - 1; ++lvl) {
         Node L, R;
         expand_node(k, leaf_level + lvl, node, L, R);
-        st_seed[lvl * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth lvl + 1
+        const uint4 rv = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth lvl + 1
+        if (kGlobalStack && lvl < gs)
+          st_glob[lvl * 64 + lane] = rv;
+        else
+          st_seed[(lvl - gs) * 64 + lane] = rv;
         const uint32_t bit = 1u << (lvl + 1);
         st_tbits = R.t == 1u ? (st_tbits | bit) : (st_tbits & ~bit);
         node = L;
       }
       Node L, R;
       expand_node(k, leaf_level + kl - 1, node, L, R);
+      if constexpr (kLateLoads) {
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+          r0[c] = src0[c];
+          r1[c] = src1[c];
+        }
+      }
       const uint4 m0 = opaque4(leaf_mask(k, L)), m1 = opaque4(leaf_mask(k, R));
 #pragma unroll
       for (int c = 0; c < C; c++) {
@@ -927,8 +949,8 @@ static int grid_for(uint64_t items, int block) {
   return (int)g;
 }
 
-size_t lane_dfs_lds_bytes(int waves_per_block, uint32_t kl) {
-  return (size_t)waves_per_block * (kl - 1) * 64 * sizeof(uint4);
+size_t lane_dfs_lds_bytes(int waves_per_block, uint32_t kl, uint32_t gslots) {
+  return (size_t)waves_per_block * (kl - 1 - gslots) * 64 * sizeof(uint4);
 }
 
 hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_block,
@@ -936,14 +958,25 @@ hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_blo
   const uint32_t total = p.parts * p.n_keys;
   const dim3 block(64 * waves_per_block);
   const dim3 grid(xcd_grid(total, waves_per_block));
-  const size_t lds = lane_dfs_lds_bytes(waves_per_block, p.kl);
+  if (p.gslots > p.kl - 1 || (p.gslots && !p.gstack)) return hipErrorInvalidValue;
+  const size_t lds = lane_dfs_lds_bytes(waves_per_block, p.kl, p.gslots);
   switch (chunks) {
-#define MG_CASE(C)                                                            \
-  case C:                                                                     \
-    hipLaunchKernelGGL(scan_lane_dfs<C>, grid, block, lds, s, p);             \
+#define MG_LAUNCH(C, L, G)                                                          \
+  hipLaunchKernelGGL((scan_lane_dfs<C, L, G>), grid, block, lds, s, p)
+#define MG_CASE(C)                                                                  \
+  case C:                                                                           \
+    if (p.late_loads && p.gslots)                                                   \
+      MG_LAUNCH(C, true, true);                                                     \
+    else if (p.late_loads)                                                          \
+      MG_LAUNCH(C, true, false);                                                    \
+    else if (p.gslots)                                                              \
+      MG_LAUNCH(C, false, true);                                                    \
+    else                                                                            \
+      MG_LAUNCH(C, false, false);                                                   \
     break;
     MG_CASE(1) MG_CASE(2) MG_CASE(3) MG_CASE(4) MG_CASE(5) MG_CASE(6) MG_CASE(7) MG_CASE(8)
 #undef MG_CASE
+#undef MG_LAUNCH
     default:
       return hipErrorInvalidValue;
   }
@@ -1031,14 +1064,18 @@ hipError_t launch_stream(int S, int G, int PK, const StreamParams& p, int waves_
   return hipErrorInvalidValue;
 }
 
-hipError_t lane_dfs_blocks_per_cu(int chunks, int block_threads, size_t lds, int* blocks) {
+hipError_t lane_dfs_blocks_per_cu(int chunks, bool late_loads, bool global_stack,
+                                  int block_threads, size_t lds, int* blocks) {
   switch (chunks) {
+#define MG_OCC(C, L, G) \
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_lane_dfs<C, L, G>, block_threads, lds)
 #define MG_CASE(C)                                                                          \
   case C:                                                                                   \
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_lane_dfs<C>,           \
-                                                        block_threads, lds);
+    if (late_loads) return global_stack ? MG_OCC(C, true, true) : MG_OCC(C, true, false);   \
+    return global_stack ? MG_OCC(C, false, true) : MG_OCC(C, false, false);
     MG_CASE(1) MG_CASE(2) MG_CASE(3) MG_CASE(4) MG_CASE(5) MG_CASE(6) MG_CASE(7) MG_CASE(8)
 #undef MG_CASE
+#undef MG_OCC
     default:
       return hipErrorInvalidValue;
   }

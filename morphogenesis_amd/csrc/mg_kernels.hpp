@@ -28,6 +28,11 @@ struct LaneDfsParams {
   uint64_t tiles_per_wave;
   uint32_t parts;  // waves per key; wave w serves key w % n_keys
   uint4* partial;  // [parts][n_keys][chunks]
+  // The top gslots levels of each lane's DFS stack (depths 1 .. gslots) live in
+  // global memory, [waves][gslots][64] seeds; the rest in LDS.
+  uint4* gstack;
+  uint32_t gslots;
+  uint32_t late_loads;  // rows loaded after a step's expansions (else before)
 };
 
 struct StreamParams {
@@ -79,8 +84,8 @@ struct SuffixParams {
 hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s);
 hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_block,
                            hipStream_t s);
-// Dynamic LDS of a lane-DFS block: (kl - 1) levels of 64 seeds per wave.
-size_t lane_dfs_lds_bytes(int waves_per_block, uint32_t kl);
+// Dynamic LDS of a lane-DFS block: kl - 1 - gslots levels of 64 seeds per wave.
+size_t lane_dfs_lds_bytes(int waves_per_block, uint32_t kl, uint32_t gslots);
 hipError_t launch_stream(int S, int G, int PK, const StreamParams& p, int waves_per_block,
                          hipStream_t s);
 size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block);
@@ -90,7 +95,8 @@ hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t
 size_t split_lds_bytes(int G, uint32_t kl, int NP);
 hipError_t split_blocks_per_cu(int S, int G, int NP, size_t lds, int* blocks);
 // Resident blocks per CU for the given block size and dynamic LDS.
-hipError_t lane_dfs_blocks_per_cu(int chunks, int block_threads, size_t lds, int* blocks);
+hipError_t lane_dfs_blocks_per_cu(int chunks, bool late_loads, bool global_stack,
+                                  int block_threads, size_t lds, int* blocks);
 hipError_t stream_blocks_per_cu(int S, int G, int PK, int block_threads, size_t lds,
                                 int* blocks);
 hipError_t launch_point(const PointParams& p, hipStream_t s);

@@ -141,9 +141,12 @@ void dedupe_delta(const uint64_t* idx, const uint8_t* diffs, uint64_t n, uint32_
 namespace {
 
 // Resident waves the device holds for a kernel variant (cached per variant).
+// Lane-DFS kinds pass G = late loads (1) + global stack levels (2): the four
+// kernels of one row width.
 int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chunks,
                    int waves_per_block, size_t lds, int* out) {
-  auto key = std::make_pair((kind * 8 + (uint32_t)PK) * 64 + (uint32_t)waves_per_block,
+  auto key = std::make_pair(((kind * 8 + (uint32_t)PK) * 8 + (uint32_t)G) * 64 +
+                                (uint32_t)waves_per_block,
                             (uint32_t)lds);
   auto it = dev->occupancy.find(key);
   int blocks = 0;
@@ -151,7 +154,8 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chun
     blocks = it->second;
   } else {
     if (kind == MG_KERNEL_LANE_DFS * 100 + (uint32_t)chunks)
-      MG_HIP(lane_dfs_blocks_per_cu(chunks, 64 * waves_per_block, lds, &blocks));
+      MG_HIP(lane_dfs_blocks_per_cu(chunks, (G & 1) != 0, (G & 2) != 0, 64 * waves_per_block,
+                                    lds, &blocks));
     else if (kind / 100 == MG_KERNEL_STREAM_SPLIT)
       MG_HIP(split_blocks_per_cu(S, G, waves_per_block / 2, lds, &blocks));
     else
@@ -210,15 +214,21 @@ void split_levels(uint32_t d, uint32_t kl_req, Plan* pl) {
 // 2^27 x 32 B, 3 keys: kl = 9 as 1366 parts x 3 tiles gives 4098 waves, one
 // CU takes a fifth block, and the pass runs 9.43 ms against 7.35 ms for kl = 7
 // as 6144 waves of 8 tiles, which this picks.
+// Lane-DFS stack levels the plan may keep in global memory instead of LDS.
+constexpr uint32_t kLaneMaxGlobalSlots = 4;
+
 struct LaneShape {
   uint32_t kl = 0;
   uint32_t wpb = 0;
+  uint32_t gslots = 0;
+  uint32_t late_loads = 0;
   uint64_t tiles_per_wave = 0;
   double cost = 0;
 };
 
 int choose_lane_shape(mg_device* dev, const Plan& pl0, uint32_t d, uint32_t n_keys, uint32_t kl_fixed,
-                      uint32_t wpb_req, uint64_t parts_fixed, LaneShape* out) {
+                      uint32_t wpb_req, uint64_t parts_fixed, uint32_t gs_fixed,
+                      uint32_t loads_fixed, LaneShape* out) {
   static const double kRate[9] = {0, 0.554, 0.826, 0.892, 0.917, 0.954, 0.971, 0.988, 1.0};
   const uint64_t simds = (uint64_t)dev->cus * 4;
   LaneShape best;
@@ -232,46 +242,61 @@ int choose_lane_shape(mg_device* dev, const Plan& pl0, uint32_t d, uint32_t n_ke
     if (!kl_fixed && pl0.row_begin % tile_rows != 0) continue;
     const uint64_t n_tiles = std::max<uint64_t>(
         1, (pl0.row_end + tile_rows - 1) / tile_rows - pl0.row_begin / tile_rows);
-    uint32_t wpb = wpb_req;
-    while (wpb > 1 && lane_dfs_lds_bytes((int)wpb, kl) > dev->max_lds) wpb--;
-    const size_t lds = lane_dfs_lds_bytes((int)wpb, kl);
-    if (lds > dev->max_lds) continue;
-    int slots = 0;
-    int rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl0.chunks, 0, 0, 1, (int)pl0.chunks,
-                            (int)wpb, lds, &slots);
-    if (rc) return rc;
-    const uint64_t cap = std::max<uint64_t>((uint64_t)slots, 1);
-    // Candidate waves per key: the most that stay resident and a few fewer, the counts that
-    // fill a whole number of waves per SIMD, and those of the smallest tiles per wave.
-    uint64_t cand[32];
-    int nc = 0;
-    if (parts_fixed) {
-      cand[nc++] = parts_fixed;
-    } else {
-      const uint64_t p_hi = std::max<uint64_t>(1, cap / n_keys);
-      for (uint64_t j = 0; j < 9 && j < p_hi; ++j) cand[nc++] = p_hi - j;
-      for (uint64_t m = 1; m <= 8; ++m)
-        if (m * simds >= n_keys) cand[nc++] = m * simds / n_keys;
-      const uint64_t t_lo = std::max<uint64_t>(1, (n_tiles * n_keys + cap - 1) / cap);
-      for (uint64_t t = t_lo; t < t_lo + 8; ++t) cand[nc++] = (n_tiles + t - 1) / t;
-    }
-    for (int ci = 0; ci < nc; ++ci) {
-      const uint64_t p0 = std::max<uint64_t>(1, std::min(cand[ci], n_tiles));
-      const uint64_t tpw = (n_tiles + p0 - 1) / p0;
-      const uint64_t parts = (n_tiles + tpw - 1) / tpw;
-      const uint64_t waves = parts * n_keys;
-      // Waves beyond the resident ones run in later rounds.
-      const uint64_t rounds = (waves + cap - 1) / cap;
-      const uint64_t per_round = (waves + rounds - 1) / rounds;
-      const uint64_t wps = (per_round + simds - 1) / simds;
-      const double rate = kRate[std::min<uint64_t>(wps, 8)];
-      const double cost = (double)rounds * (double)wps * (double)tpw *
-                          (double)((1ull << kl) + 8) / rate;
-      if (best.kl == 0 || cost < best.cost * 0.999) {
-        best.kl = kl;
-        best.wpb = wpb;
-        best.tiles_per_wave = tpw;
-        best.cost = cost;
+    // Stack levels moved to global memory: none, or as many as lift the waves per CU.
+    // (A fixed count beyond this depth's levels is reported by the plan.)
+    const uint32_t gs_lo = gs_fixed ? std::min(gs_fixed - 1, kl - 1) : 0;
+    const uint32_t gs_hi = gs_fixed ? gs_lo : std::min(kLaneMaxGlobalSlots, kl - 1);
+    for (uint32_t gs = gs_lo; gs <= gs_hi; ++gs) {
+      for (uint32_t late = 0; late <= 1; ++late) {
+        if (loads_fixed && late != loads_fixed - 1) continue;
+        uint32_t wpb = wpb_req;
+        while (wpb > 1 && lane_dfs_lds_bytes((int)wpb, kl, gs) > dev->max_lds) wpb--;
+        const size_t lds = lane_dfs_lds_bytes((int)wpb, kl, gs);
+        if (lds > dev->max_lds) continue;
+        int slots = 0;
+        int rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl0.chunks, 0,
+                                (int)late + (gs ? 2 : 0), 1,
+                                (int)pl0.chunks, (int)wpb, lds, &slots);
+        if (rc) return rc;
+        const uint64_t cap = std::max<uint64_t>((uint64_t)slots, 1);
+        // Candidate waves per key: the most that stay resident and a few fewer, the counts that
+        // fill a whole number of waves per SIMD, and those of the smallest tiles per wave.
+        uint64_t cand[32];
+        int nc = 0;
+        if (parts_fixed) {
+          cand[nc++] = parts_fixed;
+        } else {
+          const uint64_t p_hi = std::max<uint64_t>(1, cap / n_keys);
+          for (uint64_t j = 0; j < 9 && j < p_hi; ++j) cand[nc++] = p_hi - j;
+          for (uint64_t m = 1; m <= 8; ++m)
+            if (m * simds >= n_keys) cand[nc++] = m * simds / n_keys;
+          const uint64_t t_lo = std::max<uint64_t>(1, (n_tiles * n_keys + cap - 1) / cap);
+          for (uint64_t t = t_lo; t < t_lo + 8; ++t) cand[nc++] = (n_tiles + t - 1) / t;
+        }
+        for (int ci = 0; ci < nc; ++ci) {
+          const uint64_t p0 = std::max<uint64_t>(1, std::min(cand[ci], n_tiles));
+          const uint64_t tpw = (n_tiles + p0 - 1) / p0;
+          const uint64_t parts = (n_tiles + tpw - 1) / tpw;
+          const uint64_t waves = parts * n_keys;
+          // Waves beyond the resident ones run in later rounds.
+          const uint64_t rounds = (waves + cap - 1) / cap;
+          const uint64_t per_round = (waves + rounds - 1) / rounds;
+          const uint64_t wps = (per_round + simds - 1) / simds;
+          const double rate = kRate[std::min<uint64_t>(wps, 8)];
+          // Rows loaded after a step's expansions wait in the open: 1 % dearer at
+          // equal waves per SIMD (2^27 x 32 B, one key, four waves per SIMD: 2.87
+          // against 2.83 ms), so they win only where they add waves.
+          const double cost = (double)rounds * (double)wps * (double)tpw *
+                              (double)((1ull << kl) + 8) / rate * (late ? 1.01 : 1.0);
+          if (best.kl == 0 || cost < best.cost * 0.999) {
+            best.kl = kl;
+            best.wpb = wpb;
+            best.gslots = gs;
+            best.late_loads = late;
+            best.tiles_per_wave = tpw;
+            best.cost = cost;
+          }
+        }
       }
     }
   }
@@ -366,8 +391,15 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   uint32_t kl_default;
   LaneShape lane_shape;
   if (kernel == MG_KERNEL_LANE_DFS) {
+    if (o->stack_global > kLaneMaxGlobalSlots + 1)
+      return fail(MG_ERR_INVALID_ARG, "stack_global %u: at most %u stack levels in global memory",
+                  o->stack_global, kLaneMaxGlobalSlots);
+    if (o->row_loads > 2)
+      return fail(MG_ERR_INVALID_ARG, "row_loads %u: 0 (tuned), 1 (before) or 2 (after)",
+                  o->row_loads);
     int rc0 = choose_lane_shape(dev, *pl, d, n_keys, o->lane_depth,
-                                (uint32_t)pl->waves_per_block, o->waves_per_key, &lane_shape);
+                                (uint32_t)pl->waves_per_block, o->waves_per_key, o->stack_global,
+                                o->row_loads, &lane_shape);
     if (rc0) return rc0;
     kl_default = lane_shape.kl;
   } else if (kernel == MG_KERNEL_STREAM_SPLIT) {
@@ -400,12 +432,24 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   int rc = MG_OK;
   if (kernel == MG_KERNEL_LANE_DFS) {
     if (!o->waves_per_block) pl->waves_per_block = lane_shape.wpb;
-    const size_t lds = lane_dfs_lds_bytes((int)pl->waves_per_block, pl->kl);
+    if (o->stack_global) {
+      if (o->stack_global - 1 > pl->kl - 1)
+        return fail(MG_ERR_INVALID_ARG, "stack_global %u: a depth-%u lane stack has %u levels",
+                    o->stack_global, pl->kl, pl->kl - 1);
+      pl->gslots = o->stack_global - 1;
+    } else {
+      // The shape's global stack levels apply to its own depth only.
+      pl->gslots = lane_shape.kl == pl->kl ? lane_shape.gslots : 0;
+    }
+    pl->late_loads = o->row_loads ? o->row_loads - 1
+                                  : (lane_shape.kl == pl->kl ? lane_shape.late_loads : 0);
+    const size_t lds = lane_dfs_lds_bytes((int)pl->waves_per_block, pl->kl, pl->gslots);
     if (lds > dev->max_lds)
       return fail(MG_ERR_INVALID_ARG, "lane-DFS plan needs %zu B of LDS per block (max %zu)", lds,
                   dev->max_lds);
-    rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl->chunks, 0, 0, 1, (int)pl->chunks,
-                        (int)pl->waves_per_block, lds, &resident);
+    rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl->chunks, 0,
+                        (int)pl->late_loads + (pl->gslots ? 2 : 0), 1,
+                        (int)pl->chunks, (int)pl->waves_per_block, lds, &resident);
   } else {
     const int S = stream_s(pl->chunks);
     int G = o->keys_per_wave ? (int)o->keys_per_wave : default_keys_per_wave(kernel, S, n_keys);
@@ -513,6 +557,10 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       p.tiles_per_wave = pl.tiles_per_wave;
       p.parts = pl.parts;
       p.partial = b.partial;
+      p.gstack = b.gstack;
+      p.gslots = pl.gslots;
+      p.late_loads = pl.late_loads;
+      if (pl.gslots && !b.gstack) return fail(MG_ERR_INVALID_ARG, "lane-DFS plan needs a stack buffer");
       MG_HIP(launch_lane_dfs((int)pl.chunks, p, (int)pl.waves_per_block, s));
     } else if (pl.kernel == MG_KERNEL_STREAM || pl.kernel == MG_KERNEL_STREAM_SPLIT) {
       StreamParams p{};
@@ -598,6 +646,7 @@ mg::ScanBuffers device_buffers(mg_device* dev) {
   b.partial = (uint4*)dev->partial.p;
   b.partial_delta = (uint4*)dev->partial_delta.p;
   b.out = (uint4*)dev->out.p;
+  b.gstack = (uint4*)dev->gstack.p;
   return b;
 }
 
@@ -654,6 +703,7 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
   if ((rc = dev->keys.reserve(sizeof(DevKey) * n_keys))) return rc;
   if ((rc = dev->out.reserve(out_bytes))) return rc;
   if ((rc = dev->partial.reserve((size_t)pl.parts * n_keys * m->row_bytes))) return rc;
+  if (pl.gslots && (rc = dev->gstack.reserve(lane_gstack_bytes(pl, n_keys)))) return rc;
   if (pl.n_delta &&
       (rc = dev->partial_delta.reserve((size_t)pl.delta_parts * n_keys * m->row_bytes)))
     return rc;
@@ -662,6 +712,7 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
   pl.ws_out = dev->out.p;
   pl.ws_partial = dev->partial.p;
   pl.ws_partial_delta = pl.n_delta ? dev->partial_delta.p : nullptr;
+  pl.ws_gstack = pl.gslots ? dev->gstack.p : nullptr;
 
   hipStream_t s = dev->stream;
   MG_HIP(hipEventRecord(dev->ev[0], s));
@@ -692,6 +743,7 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
     t->total_ms = mg::now_ms() - t0;
     t->kernel = pl.kernel;
     t->lane_depth = pl.kl;
+    t->stack_global = pl.gslots;
     const bool stream = pl.kernel == MG_KERNEL_STREAM || pl.kernel == MG_KERNEL_STREAM_SPLIT;
     t->keys_per_wave = stream ? pl.keys_per_wave : 1;
     t->waves = (pl.kernel == MG_KERNEL_POINT || pl.kernel == MG_KERNEL_SUFFIX)
@@ -773,6 +825,7 @@ int mg_device_close(mg_device* dev) {
   dev->partial.release();
   dev->partial_delta.release();
   dev->out.release();
+  dev->gstack.release();
   dev->zero_rows.release();
   dev->h_keys.release();
   dev->h_out.release();

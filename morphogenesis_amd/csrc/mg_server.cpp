@@ -114,6 +114,7 @@ struct mg_server {
   uint32_t* d_idx = nullptr;  // slot lists: [0, cap) scan list, [cap, 2cap) admission list
   DevKey* d_stage_keys = nullptr;
   mg::DevBuf partial, partial_delta;
+  mg::DevBuf gstack;  // lane-DFS stack levels in global memory
   mg::HostBuf h_idx, h_keys, h_acc;
   uint32_t step = 0;
 
@@ -143,6 +144,7 @@ void free_server_memory(mg_server* s) {
   if (s->base_rows[1]) (void)hipFree(s->base_rows[1]);
   s->partial.release();
   s->partial_delta.release();
+  s->gstack.release();
   s->h_idx.release();
   s->h_keys.release();
   s->h_acc.release();
@@ -283,11 +285,13 @@ int sweep_step(mg_server* s, std::vector<uint32_t>* finished) {
     int rc = mg::make_plan(s->dev, s->base_rows[b], s->n_rows, s->row_bytes, n, s->d, &o, &pl);
     if (rc) return rc;
     if ((rc = s->partial.reserve((size_t)pl.parts * n * s->row_bytes))) return rc;
+    if (pl.gslots && (rc = s->gstack.reserve(mg::lane_gstack_bytes(pl, n)))) return rc;
     MG_HIP(hipMemcpyAsync(s->d_idx + start, h_list + start, n * 4, hipMemcpyHostToDevice, st));
     mg::ScanBuffers bufs;
     bufs.keys = s->d_keys;
     bufs.key_idx = s->d_idx + start;
     bufs.partial = (uint4*)s->partial.p;
+    bufs.gstack = (uint4*)s->gstack.p;
     bufs.out = s->d_acc;
     bufs.out_idx = s->d_idx + start;
     bufs.accumulate = true;

@@ -78,6 +78,43 @@ def test_lane_depths_agree(mg, scanner, kl):
         got = scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_LANE_DFS, lane_depth=kl,
                                                      waves_per_key=wpk))
         assert np.array_equal(got, ref), (kl, wpk)
+    # The top 0-4 levels of the lanes' DFS stacks in global memory, the rest in
+    # LDS (default and seven waves per key, so waves share blocks and sit
+    # idle), with the rows loaded before and after each step's expansions.
+    for gs in range(0, min(4, kl - 1) + 1):
+        for wpk in (0, 7):
+            for late in (False, True):
+                got, t = scanner.scan_keys(m, keys, mg.make_opts(
+                    kernel=mg.KERNEL_LANE_DFS, lane_depth=kl, waves_per_key=wpk,
+                    stack_global=gs, late_loads=late), with_timing=True)
+                assert t["stack_global"] == gs
+                assert np.array_equal(got, ref), (kl, gs, wpk, late)
+    for bad in (kl, 5):
+        if bad <= kl - 1 and bad <= 4:
+            continue
+        with pytest.raises(mg.MgError):
+            scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_LANE_DFS, lane_depth=kl,
+                                                    stack_global=bad))
+    m.close()
+
+
+@pytest.mark.parametrize("w", [16, 48, 128])
+def test_global_stack_levels_by_row_width(mg, scanner, w):
+    # 2^17 rows at kl = 9: every stack split, and the planner's own choice.
+    n, d = 1 << 17, 17
+    rows = oracle.fill_rows(seed=w, first_row=0, n_rows=n, row_bytes=w)
+    m = upload(mg, scanner, rows)
+    keys = [p[i % 2] for i, p in enumerate(pairs_for(d, [5, 70_000, n - 1], seed=w))]
+    ref = oracle.scan_tiled(keys, rows, THREADS)
+    for gs in (None, 0, 1, 3, 4):
+        for late in (None, False, True):
+            got = scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_LANE_DFS, lane_depth=9,
+                                                         stack_global=gs, late_loads=late))
+            assert np.array_equal(got, ref), (w, gs, late)
+    bad = mg.make_opts(kernel=mg.KERNEL_LANE_DFS)
+    bad.row_loads = 3
+    with pytest.raises(mg.MgError):
+        scanner.scan_keys(m, keys, bad)
     m.close()
 
 
