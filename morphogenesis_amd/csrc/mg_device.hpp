@@ -202,7 +202,23 @@ __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uin
       uint32_t* b[8] = {&x4, &x5, &x6, &x7, &y4, &y5, &y6, &y7};
       uint32_t* c[8] = {&x8, &x9, &x10, &x11, &y8, &y9, &y10, &y11};
       uint32_t* d[8] = {&x12, &x13, &x14, &x15, &y12, &y13, &y14, &y15};
-      qr_stages<8, 12, R16>(a, b, c, d);
+      if (rd < 7) {
+        qr_stages<8, 12, R16>(a, b, c, d);
+      } else {
+        // Round 7 without block 1's dead tail (as MG_A_ROUND7N): its column 0
+        // stops after stage 9, its column 3 after stage 10.
+        qr_stages<8, 9, R16>(a, b, c, d);
+        uint32_t* c7[7] = {&x8, &x9, &x10, &x11, &y9, &y10, &y11};
+        uint32_t* d7[7] = {&x12, &x13, &x14, &x15, &y13, &y14, &y15};
+#pragma unroll
+        for (int q = 0; q < 7; q++) *c7[q] = op_add(*c7[q], *d7[q]);
+        uint32_t* b6[6] = {&x4, &x5, &x6, &x7, &y5, &y6};
+        uint32_t* c6[6] = {&x8, &x9, &x10, &x11, &y9, &y10};
+#pragma unroll
+        for (int q = 0; q < 6; q++) *b6[q] = op_xor(*b6[q], *c6[q]);
+#pragma unroll
+        for (int q = 0; q < 6; q++) *b6[q] = op_rotr25(*b6[q]);
+      }
     }
   }
   // Round 8 (diagonal): block 0 in full; block 1's quarter-rounds through
@@ -263,6 +279,21 @@ __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uin
   A QRS(MG_A_S1) X QRS(MG_A_S2) R QRS(MG_A_S3) A QRS(MG_A_S4) X QRS(MG_A_S5) R          \
   QRS(MG_A_S6) A QRS(MG_A_S1) X QRS(MG_A_S2) R QRS(MG_A_S9) A QRS(MG_A_S4) X            \
   QRS(MG_A_S5) R QRS(MG_A_S12)
+// Round 7 (columns) without block 1's dead tail: after round 8 block 1 only
+// needs its words 0 and 1, whose round-8 quarter-rounds read words 0, 5, 10,
+// 15, 1, 6, 11 and 12 of round 7. Its column 0 (0, 4, 8, 12) then only needs
+// words 0 and 12 (stages 1-9), column 3 (3, 7, 11, 15) words 11 and 15
+// (stages 1-10): 5 ops fewer.
+#define MG_A_COLS_7(S)                                                             \
+  S(0, 4, 8, 12) S(1, 5, 9, 13) S(2, 6, 10, 14) S(3, 7, 11, 15) S(17, 21, 25, 29) \
+  S(18, 22, 26, 30) S(19, 23, 27, 31)
+#define MG_A_COLS_6(S) \
+  S(0, 4, 8, 12) S(1, 5, 9, 13) S(2, 6, 10, 14) S(3, 7, 11, 15) S(17, 21, 25, 29) S(18, 22, 26, 30)
+#define MG_A_ROUND7N(A, X, R)                                                              \
+  A MG_A_COLS(MG_A_S1) X MG_A_COLS(MG_A_S2) R MG_A_COLS(MG_A_S3) A MG_A_COLS(MG_A_S4) X    \
+  MG_A_COLS(MG_A_S5) R MG_A_COLS(MG_A_S6) A MG_A_COLS(MG_A_S1) X MG_A_COLS(MG_A_S2) R      \
+  MG_A_COLS(MG_A_S9) A MG_A_COLS_7(MG_A_S4) X MG_A_COLS_6(MG_A_S5) R MG_A_COLS_6(MG_A_S12)
+
 // Round 8: block 0's diagonals in full; block 1's quarter-rounds through its
 // words 0 and 1, (0, 5, 10, 15) and (1, 6, 11, 12), as %16-%19 and %20-%23,
 // up to stage 7, which sets them.
@@ -328,7 +359,7 @@ __device__ __forceinline__ void prg_expand_rounds(uint32_t s0, uint32_t s1, uint
     asm volatile(MG_A_ROUNDN(MG_A_DIAGS, "", "", "") : MG_A_X32(x, y));  // round 4
     asm volatile(MG_A_ROUNDN(MG_A_COLS, "", "", "") : MG_A_X32(x, y));   // round 5
     asm volatile(MG_A_ROUNDN(MG_A_DIAGS, "", "", "") : MG_A_X32(x, y));  // round 6
-    asm volatile(MG_A_ROUNDN(MG_A_COLS, "", "", "") : MG_A_X32(x, y));   // round 7
+    asm volatile(MG_A_ROUND7N("", "", "") : MG_A_X32(x, y));             // round 7
   } else {
     // The s_nop lengths are operands %32-%34 (immediates).
 #define MG_A_R(QRS)                                                                     \
@@ -339,8 +370,10 @@ __device__ __forceinline__ void prg_expand_rounds(uint32_t s0, uint32_t s1, uint
     MG_A_R(MG_A_DIAGS);  // round 4
     MG_A_R(MG_A_COLS);   // round 5
     MG_A_R(MG_A_DIAGS);  // round 6
-    MG_A_R(MG_A_COLS);   // round 7
 #undef MG_A_R
+    asm volatile(MG_A_ROUND7N("s_nop %32\n\t", "s_nop %33\n\t", "s_nop %34\n\t")  // round 7
+                 : MG_A_X32(x, y)
+                 : "i"(NA - 1), "i"(NX - 1), "i"(NR - 1));
   }
   z[0] = y[0]; z[1] = y[5]; z[2] = y[10]; z[3] = y[15];
   z[4] = y[1]; z[5] = y[6]; z[6] = y[11]; z[7] = y[12];
