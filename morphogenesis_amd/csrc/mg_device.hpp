@@ -514,6 +514,9 @@ constexpr int kPrgRoundNops3[][3] = {{2, 2, 5}, {1, 2, 5}, {1, 3, 5}, {2, 3, 5},
 // n.t == 1 (dpf.rs:221-239). `k` is wave-uniform, so the correction word is
 // read with scalar loads.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// 16-byte words in LDS / global memory, typed by address space.
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+typedef __attribute__((address_space(1))) u32x4 glob_u32x4;
 
 // Key reads go through the constant address space: no kernel writes a key,
 // and a wave-uniform read then becomes a scalar load into SGPRs (a divergent

@@ -156,9 +156,13 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   // bit that matters, t == 1 (dpf.rs:221-239 and 136-141 test nothing else),
   // in bit `depth` of a per-lane register.
   // (kGlobalStack false: gslots is 0 and none of the global-slot code exists.)
+  // The two parts are typed by address space, so a pop stays a ds_read or a
+  // global load: through one generic pointer it would be a flat load, slower
+  // for the LDS slots that nearly every pop reads.
   const int gs = kGlobalStack ? (int)p.gslots : 0;
-  uint4* st_seed = lds + (size_t)wib * (kl - 1 - gs) * 64;  // slot s >= gs at s - gs
-  uint4* st_glob = kGlobalStack ? p.gstack + (size_t)(live ? w : 0) * gs * 64 : nullptr;
+  lds_u32x4* st_seed = (lds_u32x4*)(lds + (size_t)wib * (kl - 1 - gs) * 64);  // slot s >= gs
+  glob_u32x4* st_glob =
+      kGlobalStack ? (glob_u32x4*)(p.gstack + (size_t)(live ? w : 0) * gs * 64) : nullptr;
   uint32_t st_tbits = 0;
 
   uint4 acc[C];
@@ -198,8 +202,11 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
     for (uint32_t pair = 0; pair < n_pairs; ++pair) {
       if (pair) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
         lvl = kl - 1 - __builtin_ctz(pair);
-        const uint4 sv = (kGlobalStack && lvl - 1 < gs) ? st_glob[(lvl - 1) * 64 + lane]
-                                                        : st_seed[(lvl - 1 - gs) * 64 + lane];
+        u32x4 sv;
+        if (kGlobalStack && lvl - 1 < gs)
+          sv = st_glob[(lvl - 1) * 64 + lane];
+        else
+          sv = st_seed[(lvl - 1 - gs) * 64 + lane];
         node.s0 = sv.x; node.s1 = sv.y; node.s2 = sv.z; node.s3 = sv.w;
         node.t = (st_tbits >> lvl) & 1u;
       }
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
       for (; lvl < kl - 1; ++lvl) {
         Node L, R;
         expand_node(k, leaf_level + lvl, node, L, R);
-        const uint4 rv = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth lvl + 1
+        const u32x4 rv = {R.s0, R.s1, R.s2, R.s3};  // depth lvl + 1
         if (kGlobalStack && lvl < gs)
           st_glob[lvl * 64 + lane] = rv;
         else
