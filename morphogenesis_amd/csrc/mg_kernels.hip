@@ -609,6 +609,26 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
     uint32_t st_tbits[G];
 #pragma unroll
     for (int g = 0; g < G; g++) st_tbits[g] = 0;
+    // The next tile's lane roots, walked one tree level per step while the
+    // current tile runs (levels pre_lvl .. tb + lb - 1 still to go), so that a
+    // tile's first step only descends its kl levels. Walking the changed
+    // tile-prefix levels and the lb lane levels (about 8 node expansions per
+    // key) all in that step held the streaming wave up once per tile.
+    Node nroot[G];
+    int pre_lvl = -1;  // -1: no walk under way
+    const int root_lvl = tb + lb;
+    auto root_level = [&](uint64_t ntile) {  // one level of the walk, every key
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const uint32_t bit = pre_lvl < tb ? (uint32_t)(ntile >> (tb - 1 - pre_lvl)) & 1u
+                                          : ((uint32_t)lane >> (root_lvl - 1 - pre_lvl)) & 1u;
+        Node a, b;
+        expand_node<kSplitPrgVariant, false>(kp[g], pre_lvl, nroot[g], a, b);
+        nroot[g] = bit ? b : a;
+        if (pre_lvl < tb) path[g].put(lane, pre_lvl + 1, nroot[g]);
+      }
+      ++pre_lvl;
+    };
     for (uint64_t s = 0; s < block_steps; ++s) {
       if (s < n_steps) {
         const uint64_t gs = gs0 + s;
@@ -616,9 +636,16 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
         const uint32_t pair = (uint32_t)(gs & (n_pairs - 1));
         const bool from_root = pair == 0 || s == 0;  // a new tile, or the run's first step
         if (from_root) {
+          if (s != 0 && pre_lvl >= 0) {
+            while (pre_lvl < root_lvl) root_level(tile);  // finish the walk
 #pragma unroll
-          for (int g = 0; g < G; g++)
-            node[g] = tile_lane_root(kp[g], path[g], lane, tile, tile - 1, s == 0, tb, lb);
+            for (int g = 0; g < G; g++) node[g] = nroot[g];
+          } else {
+#pragma unroll
+            for (int g = 0; g < G; g++)
+              node[g] = tile_lane_root(kp[g], path[g], lane, tile, tile - 1, s == 0, tb, lb);
+          }
+          pre_lvl = -1;
         }
         const bool seek = s == 0 && pair != 0;
         if (seek) {
@@ -663,6 +690,16 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
           expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + kl - 1, node[g], L, R);
           mbuf[g * 128 + 2 * lane] = leaf_mask<false>(kp[g], L);
           mbuf[g * 128 + 2 * lane + 1] = leaf_mask<false>(kp[g], R);
+        }
+        // One level of the next tile's lane-root walk, if this run reaches that tile.
+        if (((tile + 1 - p.tile_begin) << (kl - 1)) < gs0 + n_steps) {
+          if (pre_lvl < 0) {  // start at the first level where the two tiles' paths part
+            const uint64_t diff = (tile + 1) ^ tile;
+            pre_lvl = tb - (64 - __builtin_clzll(diff));
+#pragma unroll
+            for (int g = 0; g < G; g++) nroot[g] = path[g].get(pre_lvl);
+          }
+          if (pre_lvl < root_lvl) root_level(tile + 1);
         }
       }
       __syncthreads();  // step s's masks are published
