@@ -104,7 +104,7 @@ def test_wire_length_rule_and_roundtrip():
         r = oracle.key_from_bytes(b1)
         for i in (0, 1, (1 << d) - 1):
             assert np.array_equal(oracle.eval_point(k1, i), oracle.eval_point(r, i))
-    # Reference sizes: C1 395 B, C2/C3 521 B, C4 593 B.
+    # Wire sizes at the benchmark domains: 395 B at d = 20, 521 B at d = 27, 593 B at d = 31.
     assert [oracle.key_wire_len(d) for d in (20, 27, 31)] == [395, 521, 593]
 
 
