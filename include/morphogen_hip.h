@@ -100,7 +100,8 @@ typedef struct {
   uint32_t lane_depth;      /* kl: leaves per lane = 2^kl; 0 = tuned       */
   uint32_t keys_per_wave;   /* stream kernel only, 1-4; 0 = tuned          */
   uint32_t waves_per_key;   /* 0 = tuned                                   */
-  uint32_t waves_per_block; /* 1-4; 0 = tuned                              */
+  uint32_t waves_per_block; /* lane-DFS, stream 1-4; split 2, 4, 8;
+                               0 = tuned                                   */
   uint32_t use_graph;       /* replay the launch sequence as a HIP graph   */
   uint64_t row_begin;       /* scan rows [row_begin, row_end) only; the    */
   uint64_t row_end;         /* range must be tile-aligned unless it is the */
