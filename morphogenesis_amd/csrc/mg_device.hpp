@@ -489,10 +489,12 @@ __device__ __forceinline__ uint32_t t_mask(uint32_t t) {
 // op-major with the rotate by 16 as a packed add (3) or SDWA xors (4), 5 the
 // op-major order with rounds 3-8 as one asm statement each, 6-21 variant 5
 // with the s_nops of kPrgRoundNops before its stages, 22 and on rounds 2-8
-// spaced as kPrgRoundNops3 says. The kernels use 33 (two wait states before
-// add and xor stages, six before rotates): 4.5 % more node expansions per
-// second than 1 at 4 waves per SIMD, 7 % at 5, 9.5 % at 8.
-constexpr int kPrgVariant = 33;
+// spaced as kPrgRoundNops3 says. The kernels use 44 (two wait states before
+// add and xor stages, ten before rotates): 4.6 % more node expansions per
+// second than 1 at 4 waves per SIMD, 6.5 % at 5, 9.7 % at 8, and in the
+// 2^31 x 32 B, 24-key lane-DFS pass 800.9-802.5 ms against 817.3-818.2 ms
+// for six before rotates (12: 804.9-805.6, 14: 831.1-832.0 ms).
+constexpr int kPrgVariant = 44;
 // The node routine of the stream and split kernels' DFS waves: the unspaced
 // op-major one. They run one to three DFS waves per SIMD, where the spacing
 // s_nops cost more than they save (78,643,200 x 1 KiB, 4 keys: 14.40-14.61
@@ -508,7 +510,9 @@ constexpr int kPrgRoundNops3[][3] = {{2, 2, 5}, {1, 2, 5}, {1, 3, 5}, {2, 3, 5},
                                      {1, 3, 6}, {2, 3, 6}, {1, 2, 4}, {2, 3, 4}, {1, 4, 5},
                                      {3, 2, 5}, {2, 2, 6}, {2, 4, 5}, {2, 2, 7}, {2, 2, 8},
                                      {3, 3, 7}, {2, 1, 6}, {1, 1, 6}, {3, 2, 6}, {2, 3, 7},
-                                     {1, 2, 7}, {3, 3, 8}, {2, 2, 10}};
+                                     {1, 2, 7}, {3, 3, 8}, {2, 2, 10}, {2, 2, 12}, {2, 2, 14},
+                                     {2, 2, 16}, {3, 2, 10}, {3, 3, 10}, {2, 3, 10}, {3, 2, 12},
+                                     {4, 2, 12}};
 
 // Both children of `n` at tree level `level`, correction word applied when
 // n.t == 1 (dpf.rs:221-239). `k` is wave-uniform, so the correction word is

@@ -206,8 +206,8 @@ void split_levels(uint32_t d, uint32_t kl_req, Plan* pl) {
 // waves of `tiles_per_wave` tiles. A tile costs each lane 2^kl leaf
 // expansions plus about 8 more (its lb-level path to the lane root and the
 // tile prefix). The SIMD's rate depends on how many waves share it (the
-// ChaCha node rate at 1 .. 8 waves per SIMD, measured: 0.554, 0.826, 0.892,
-// 0.917, 0.954, 0.971, 0.988, 1.0 of the 8-wave rate; the stage-spaced s_nops
+// ChaCha node rate at 1 .. 8 waves per SIMD, measured: 0.488, 0.834, 0.878,
+// 0.908, 0.946, 0.966, 0.986, 1.0 of the 8-wave rate; the stage-spaced s_nops
 // of the node routine leave a lone wave idle, more waves fill them). The shape minimising
 // waves x tiles x (2^kl + 8) / rate on the most loaded SIMD wins; kl is
 // searched when `kl_fixed` is 0, tiles per wave when `parts_fixed` is 0.
@@ -229,7 +229,7 @@ struct LaneShape {
 int choose_lane_shape(mg_device* dev, const Plan& pl0, uint32_t d, uint32_t n_keys, uint32_t kl_fixed,
                       uint32_t wpb_req, uint64_t parts_fixed, uint32_t gs_fixed,
                       uint32_t loads_fixed, LaneShape* out) {
-  static const double kRate[9] = {0, 0.554, 0.826, 0.892, 0.917, 0.954, 0.971, 0.988, 1.0};
+  static const double kRate[9] = {0, 0.488, 0.834, 0.878, 0.908, 0.946, 0.966, 0.986, 1.0};
   const uint64_t simds = (uint64_t)dev->cus * 4;
   LaneShape best;
   const uint32_t kl_lo = kl_fixed ? std::min(kl_fixed, d) : 1;

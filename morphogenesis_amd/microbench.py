@@ -67,7 +67,9 @@ PRG_VARIANTS = ["compiler_order", "ordered", "skewed", "ordered_rot16_pk_add",
                         (2, 2, 5), (1, 2, 5), (1, 3, 5), (2, 3, 5), (1, 2, 6), (1, 3, 6),
                         (2, 3, 6), (1, 2, 4), (2, 3, 4), (1, 4, 5), (3, 2, 5), (2, 2, 6),
                         (2, 4, 5), (2, 2, 7), (2, 2, 8), (3, 3, 7), (2, 1, 6), (1, 1, 6),
-                        (3, 2, 6), (2, 3, 7), (1, 2, 7), (3, 3, 8), (2, 2, 10))]
+                        (3, 2, 6), (2, 3, 7), (1, 2, 7), (3, 3, 8), (2, 2, 10), (2, 2, 12),
+                        (2, 2, 14), (2, 2, 16), (3, 2, 10), (3, 3, 10), (2, 3, 10), (3, 2, 12),
+                        (4, 2, 12))]
 
 # int32 VALU ops per GGM node expansion, the constant every roofline here uses:
 # two ChaCha8 blocks after dead-code elimination plus the correction word.
