@@ -512,7 +512,8 @@ constexpr int kPrgRoundNops3[][3] = {{2, 2, 5}, {1, 2, 5}, {1, 3, 5}, {2, 3, 5},
                                      {3, 3, 7}, {2, 1, 6}, {1, 1, 6}, {3, 2, 6}, {2, 3, 7},
                                      {1, 2, 7}, {3, 3, 8}, {2, 2, 10}, {2, 2, 12}, {2, 2, 14},
                                      {2, 2, 16}, {3, 2, 10}, {3, 3, 10}, {2, 3, 10}, {3, 2, 12},
-                                     {4, 2, 12}};
+                                     {4, 2, 12}, {2, 2, 9}, {2, 2, 11}, {1, 2, 10}, {2, 1, 10},
+                                     {1, 1, 10}, {1, 2, 11}};
 
 // Both children of `n` at tree level `level`, correction word applied when
 // n.t == 1 (dpf.rs:221-239). `k` is wave-uniform, so the correction word is

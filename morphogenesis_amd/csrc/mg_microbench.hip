@@ -15,7 +15,7 @@
 //   int mgb_prg(int device, uint32_t blocks, uint32_t iters, double* ms,
 //               uint32_t* checksum)           (the kernels' expansion)
 //   int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters,
-//                double* ms, uint32_t* checksum)  (expansion variant 0-52)
+//                double* ms, uint32_t* checksum)  (expansion variant 0-58)
 //   int mgb_valu(int device, uint32_t blocks, uint32_t iters, double* ms,
 //                uint32_t* checksum)
 // Both launch `blocks` blocks of 256 threads, time the launch with HIP events
@@ -416,11 +416,11 @@ extern "C" {
 // 2 = op-major with block 1 one stage behind block 0, 3 / 4 = op-major with
 // the rotate by 16 as a packed 16-bit add / two SDWA xors, 5 = op-major with
 // rounds 3-8 as one asm statement each, 6-21 = 5 with the s_nops of
-// kPrgRoundNops (mg_device.hpp) before its stages, 22-52 = rounds 2-8 spaced
+// kPrgRoundNops (mg_device.hpp) before its stages, 22-58 = rounds 2-8 spaced
 // as kPrgRoundNops3 says (add / xor / rotate stages).
 int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters, double* ms,
              uint32_t* checksum) {
-  if (!ms || !checksum || !blocks || ordered > 52) return (int)hipErrorInvalidValue;
+  if (!ms || !checksum || !blocks || ordered > 58) return (int)hipErrorInvalidValue;
   // A key with every correction word set, so half the nodes (t == 1) apply it.
   mg::DevKey hk{};
   for (int l = 0; l < mg::kMaxDomainBits; l++) {
@@ -450,7 +450,8 @@ int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters, doub
             MGB_PRG(31) MGB_PRG(32) MGB_PRG(33) MGB_PRG(34) MGB_PRG(35) MGB_PRG(36)
             MGB_PRG(37) MGB_PRG(38) MGB_PRG(39) MGB_PRG(40) MGB_PRG(41) MGB_PRG(42)
             MGB_PRG(43) MGB_PRG(44) MGB_PRG(45) MGB_PRG(46) MGB_PRG(47) MGB_PRG(48)
-            MGB_PRG(49) MGB_PRG(50) MGB_PRG(51) MGB_PRG(52)
+            MGB_PRG(49) MGB_PRG(50) MGB_PRG(51) MGB_PRG(52) MGB_PRG(53) MGB_PRG(54)
+            MGB_PRG(55) MGB_PRG(56) MGB_PRG(57) MGB_PRG(58)
 #undef MGB_PRG
           }
         } else if (ordered == 5)

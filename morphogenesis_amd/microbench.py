@@ -69,7 +69,8 @@ PRG_VARIANTS = ["compiler_order", "ordered", "skewed", "ordered_rot16_pk_add",
                         (2, 4, 5), (2, 2, 7), (2, 2, 8), (3, 3, 7), (2, 1, 6), (1, 1, 6),
                         (3, 2, 6), (2, 3, 7), (1, 2, 7), (3, 3, 8), (2, 2, 10), (2, 2, 12),
                         (2, 2, 14), (2, 2, 16), (3, 2, 10), (3, 3, 10), (2, 3, 10), (3, 2, 12),
-                        (4, 2, 12))]
+                        (4, 2, 12), (2, 2, 9), (2, 2, 11), (1, 2, 10), (2, 1, 10), (1, 1, 10),
+                        (1, 2, 11))]
 
 # int32 VALU ops per GGM node expansion, the constant every roofline here uses:
 # two ChaCha8 blocks after dead-code elimination plus the correction word.
