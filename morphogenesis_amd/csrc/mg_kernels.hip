@@ -210,17 +210,19 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
         node.s0 = sv.x; node.s1 = sv.y; node.s2 = sv.z; node.s3 = sv.w;
         node.t = (st_tbits >> lvl) & 1u;
       }
-      const uint64_t x = base + 2 * (uint64_t)pair;
-      const uint4* src0 = (lane_active && x < p.n_rows) ? p.rows + x * C : p.zero_rows;
-      const uint4* src1 = (lane_active && x + 1 < p.n_rows) ? p.rows + (x + 1) * C : p.zero_rows;
       uint4 r0[C], r1[C];
-      if constexpr (!kLateLoads) {
+      auto load_rows = [&]() {
+        const uint64_t x = base + 2 * (uint64_t)pair;
+        const uint4* src0 = (lane_active && x < p.n_rows) ? p.rows + x * C : p.zero_rows;
+        const uint4* src1 =
+            (lane_active && x + 1 < p.n_rows) ? p.rows + (x + 1) * C : p.zero_rows;
 #pragma unroll
         for (int c = 0; c < C; c++) {
           r0[c] = src0[c];
           r1[c] = src1[c];
         }
-      }
+      };
+      if constexpr (!kLateLoads) load_rows();
       for (; lvl < kl - 1; ++lvl) {
         Node L, R;
         expand_node(k, leaf_level + lvl, node, L, R);
@@ -235,13 +237,7 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
       }
       Node L, R;
       expand_node(k, leaf_level + kl - 1, node, L, R);
-      if constexpr (kLateLoads) {
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-          r0[c] = src0[c];
-          r1[c] = src1[c];
-        }
-      }
+      if constexpr (kLateLoads) load_rows();
       const uint4 m0 = opaque4(leaf_mask(k, L)), m1 = opaque4(leaf_mask(k, R));
 #pragma unroll
       for (int c = 0; c < C; c++) {
