@@ -98,7 +98,9 @@ enum {
 typedef struct {
   uint32_t kernel;          /* MG_KERNEL_*                                 */
   uint32_t lane_depth;      /* kl: leaves per lane = 2^kl; 0 = tuned       */
-  uint32_t keys_per_wave;   /* stream kernel only, 1-4; 0 = tuned          */
+  uint32_t keys_per_wave;   /* stream and split kernels, 1-4, clamped to   */
+                            /* the width's cap (stream: 3 at 2 KiB rows,   */
+                            /* 2 at 4 KiB; split: 3 at 4 KiB); 0 = tuned   */
   uint32_t waves_per_key;   /* 0 = tuned                                   */
   uint32_t waves_per_block; /* lane-DFS, stream 1-4; split 2, 4, 8;
                                0 = tuned                                   */

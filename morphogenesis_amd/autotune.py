@@ -81,8 +81,10 @@ def candidates(row_bytes: int, n_keys: int):
                 continue
             out.append(dict(kernel=KERNEL_STREAM, lane_depth=kl, keys_per_wave=g,
                             waves_per_block=wpb))
-        # Split blocks of 1 or 4 producer/consumer pairs (2 or 8 waves).
-        for kl, g, wpb in itertools.product((5, 6, 7, 8), range(1, gmax + 1), (2, 8)):
+        # Split blocks of 1 or 4 producer/consumer pairs (2 or 8 waves); their
+        # streaming waves hold up to 4 keys' accumulators at 2 KiB rows, 3 at 4 KiB.
+        gmax_split = 4 if chunks <= 128 else 3
+        for kl, g, wpb in itertools.product((5, 6, 7, 8), range(1, gmax_split + 1), (2, 8)):
             if g > n_keys:
                 continue
             out.append(dict(kernel=KERNEL_STREAM_SPLIT, lane_depth=kl, keys_per_wave=g,

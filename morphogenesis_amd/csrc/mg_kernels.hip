@@ -1041,8 +1041,8 @@ hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s) {
 // (S, G) instances of scan_split, each with NP = 1, 2 and 4 pairs per block.
 #define MG_SPLIT_SHAPES(NN)                                                   \
   MG_SG(1, 1, NN) MG_SG(1, 2, NN) MG_SG(1, 3, NN) MG_SG(1, 4, NN)             \
-  MG_SG(2, 1, NN) MG_SG(2, 2, NN) MG_SG(2, 3, NN)                             \
-  MG_SG(4, 1, NN) MG_SG(4, 2, NN)
+  MG_SG(2, 1, NN) MG_SG(2, 2, NN) MG_SG(2, 3, NN) MG_SG(2, 4, NN)             \
+  MG_SG(4, 1, NN) MG_SG(4, 2, NN) MG_SG(4, 3, NN) MG_SG(4, 4, NN)
 
 hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t s) {
   const uint32_t n_groups = (p.n_keys + G - 1) / G;

@@ -166,7 +166,13 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chun
   return MG_OK;
 }
 
-int max_keys_per_wave(int S) { return S == 1 ? 4 : (S == 2 ? 3 : 2); }
+// Keys per wave (stream) or per DFS/streaming pair (split) that a kernel instantiation takes
+// at S chunks of 64 lanes per row: the split kernel's streaming wave holds 4 row batches and
+// G x S accumulators (246 VGPRs for 4 KiB rows and 3 keys; 4 keys would spill).
+int max_keys_per_wave(uint32_t kernel, int S) {
+  if (kernel == MG_KERNEL_STREAM_SPLIT) return S <= 2 ? 4 : 3;
+  return S == 1 ? 4 : (S == 2 ? 3 : 2);
+}
 
 // Split kernel: the most producer/consumer pairs per CU to keep resident, indexed by pairs
 // per block (1, 2, 4).
@@ -185,8 +191,8 @@ int stream_pk(uint32_t chunks) { return chunks <= 16 ? 4 : (chunks <= 32 ? 2 : 1
 // kernel's cost per key falls up to its widest group (1 KiB rows: 11.7 ms for one key, 14.9 ms
 // for four), the stream kernel's up to 3.
 int default_keys_per_wave(uint32_t kernel, int S, uint32_t n_keys) {
-  const int gmax = kernel == MG_KERNEL_STREAM_SPLIT ? max_keys_per_wave(S)
-                                                    : std::min(3, max_keys_per_wave(S));
+  const int gmax = kernel == MG_KERNEL_STREAM_SPLIT ? max_keys_per_wave(kernel, S)
+                                                    : std::min(3, max_keys_per_wave(kernel, S));
   const int nk = std::max(1, (int)n_keys);
   const int ng = (nk + gmax - 1) / gmax;
   return (nk + ng - 1) / ng;
@@ -340,12 +346,15 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     // key count at 1 KiB rows (1 / 3 / 8 keys: 5.08 / 5.79 / 11.77 ms against 5.32 / 6.33 /
     // 18.74 ms) and three or more keys at 528-1008-byte rows (768 B, 3 keys: 7.30 against
     // 8.24 ms), but loses one or two keys there (768 B, 1 key: 6.04 against 5.26 ms) and
-    // everywhere below 528 B, where rows pack several to a load. It needs d >= kl + 6; its
+    // everywhere below 528 B, where rows pack several to a load. Over 80.5 GB of 2 KiB and
+    // 4 KiB rows it wins every key count, most where its groups hold more keys than a stream
+    // wave (4 KiB, 1 / 3 / 8 keys: 11.74 / 11.79 / 26.2 ms against 12.56 / 23.4 / 46.4 ms;
+    // 2 KiB, 1 / 8 keys: 12.75 / 19.0 ms against 13.64 / 38.0 ms). It needs d >= kl + 6; its
     // default kl (5 to 7) shrinks to d - 6 on smaller domains.
     const uint32_t kl_split = o->lane_depth ? o->lane_depth : 5;
     if (d == 0) kernel = MG_KERNEL_POINT;
     else if (pl->chunks <= 8) kernel = MG_KERNEL_LANE_DFS;
-    else if (pl->chunks > 32 && pl->chunks <= 64 && (n_keys >= 3 || pl->chunks == 64) &&
+    else if (pl->chunks > 32 && pl->chunks <= 256 && (n_keys >= 3 || pl->chunks >= 64) &&
              d >= kl_split + 6)
       kernel = MG_KERNEL_STREAM_SPLIT;
     else if (pl->chunks <= 256) kernel = MG_KERNEL_STREAM;
@@ -453,7 +462,7 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   } else {
     const int S = stream_s(pl->chunks);
     int G = o->keys_per_wave ? (int)o->keys_per_wave : default_keys_per_wave(kernel, S, n_keys);
-    G = std::max(1, std::min(G, max_keys_per_wave(S)));
+    G = std::max(1, std::min(G, max_keys_per_wave(kernel, S)));
     pl->keys_per_wave = (uint32_t)G;
     groups = (n_keys + G - 1) / G;
     if (kernel == MG_KERNEL_STREAM) {

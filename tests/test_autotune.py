@@ -41,7 +41,10 @@ def test_candidates_cover_each_kernel_for_its_widths(at):
     assert any(c["kernel"] == mg.KERNEL_STREAM_SPLIT and c["keys_per_wave"] == 3 for c in wide)
     # Never more keys per wave than keys, nor than the width allows.
     assert all(c.get("keys_per_wave", 1) <= 3 for c in wide)
-    assert all(c.get("keys_per_wave", 1) <= 2 for c in at.candidates(4096, 8))
+    assert all(c.get("keys_per_wave", 1) <= (3 if c["kernel"] == mg.KERNEL_STREAM_SPLIT else 2)
+               for c in at.candidates(4096, 8))
+    assert any(c.get("keys_per_wave") == 3 for c in at.candidates(4096, 8))
+    assert any(c.get("keys_per_wave") == 4 for c in at.candidates(2048, 8))
     # Every stored field is one make_opts accepts.
     for c in narrow + wide:
         assert set(c) <= set(at.SETTING_KEYS)
