@@ -27,22 +27,39 @@ def arrivals(rate, duration, k_min=1, k_max=32, keys_per_lookup=3, seed=0):
         out.append((t, keys_per_lookup * int(rng.integers(k_min, k_max + 1))))
 
 
-def simulate(reqs, cap, key_s, steps=4, step_overhead_s=0.0):
-    """Latencies (seconds, in completion order) of `reqs` under the loop's
-    policy. A request larger than `cap` is refused by the server; here it
-    raises ValueError."""
+POLICIES = ("fifo", "skip", "smallest")
+
+
+def simulate(reqs, cap, key_s, steps=4, step_overhead_s=0.0, policy="fifo"):
+    """Latencies (seconds, in completion order) of `reqs` under an admission
+    policy: "fifo" is the loop's (the head of the queue blocks those behind
+    it), "skip" admits later requests that fit past a blocked head, "smallest"
+    admits the smallest waiting requests first. A request larger than `cap` is
+    refused by the server; here it raises ValueError."""
     if any(k > cap for _, k in reqs):
         raise ValueError("a request exceeds the key cap")
-    now, i, active, lat = 0.0, 0, [], []
-    while i < len(reqs) or active:
-        if not active and reqs[i][0] > now:
+    if policy not in POLICIES:
+        raise ValueError(f"unknown policy {policy!r}")
+    now, i, active, waiting, lat = 0.0, 0, [], [], []
+    while i < len(reqs) or active or waiting:
+        if not active and not waiting and reqs[i][0] > now:
             now = reqs[i][0]  # idle until the next arrival
-        n_act = sum(a[1] for a in active)
-        # FIFO admission: the head of the queue blocks those behind it.
-        while i < len(reqs) and reqs[i][0] <= now and n_act + reqs[i][1] <= cap:
-            active.append([steps, reqs[i][1], reqs[i][0]])
-            n_act += reqs[i][1]
+        while i < len(reqs) and reqs[i][0] <= now:
+            waiting.append(reqs[i])
             i += 1
+        n_act = sum(a[1] for a in active)
+        if policy == "smallest":
+            waiting.sort(key=lambda r: r[1])
+        j = 0
+        while j < len(waiting):
+            if n_act + waiting[j][1] <= cap:
+                t, k = waiting.pop(j)
+                active.append([steps, k, t])
+                n_act += k
+            elif policy == "fifo":
+                break
+            else:
+                j += 1
         now += n_act * key_s / steps + step_overhead_s
         for a in active:
             a[0] -= 1
@@ -66,12 +83,14 @@ def main():
     ap.add_argument("--duration", type=float, default=2000.0)
     ap.add_argument("--k-max", type=int, default=32)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--policy", choices=POLICIES, default="fifo")
     args = ap.parse_args()
     reqs = arrivals(args.rate, args.duration, k_max=args.k_max, seed=args.seed)
     for key_ms in args.key_ms or [2.1]:
         for cap in args.cap or [96]:
-            lat = simulate(reqs, cap, key_ms * 1e-3, args.steps)
+            lat = simulate(reqs, cap, key_ms * 1e-3, args.steps, policy=args.policy)
             print(json.dumps({"key_ms": key_ms, "cap": cap, "steps": args.steps,
+                              "policy": args.policy,
                               "requests": len(reqs), **percentiles(lat)}))
 
 

@@ -44,3 +44,24 @@ def test_latency_grows_with_key_cost_and_load():
     assert all(3 <= k <= 96 and k % 3 == 0 for _, k in reqs)
     p = [sm.percentiles(sm.simulate(reqs, 96, c))["p99_ms"] for c in (1e-3, 2e-3, 3e-3)]
     assert p[0] < p[1] < p[2]
+
+
+def test_policies_differ_only_when_the_head_is_blocked():
+    # 80 keys hold the cap; a 30-key request waits; a 3-key one behind it fits.
+    reqs = [(0.0, 80), (0.0, 30), (0.0, 3)]
+    fifo = sm.simulate(reqs, 96, C, policy="fifo")
+    skip = sm.simulate(reqs, 96, C, policy="skip")
+    small = sm.simulate(reqs, 96, C, policy="smallest")
+    # FIFO: 80 alone, then 30 + 3 together.
+    assert fifo.tolist() == pytest.approx([80 * C, 80 * C + 33 * C, 80 * C + 33 * C])
+    # Skip runs the 3-key request beside the 80 keys, then the 30.
+    assert skip.tolist() == pytest.approx([83 * C, 83 * C, 83 * C + 30 * C])
+    # Smallest first takes 3 + 30, then the 80 keys alone.
+    assert small.tolist() == pytest.approx([33 * C, 33 * C, 33 * C + 80 * C])
+    # Without a blocked head all three agree.
+    free = [(0.0, 10), (0.001, 20), (0.5, 5)]
+    ref = sm.simulate(free, 96, C)
+    for policy in sm.POLICIES:
+        assert sm.simulate(free, 96, C, policy=policy).tolist() == pytest.approx(ref.tolist())
+    with pytest.raises(ValueError):
+        sm.simulate(free, 96, C, policy="lifo")
