@@ -490,13 +490,16 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     // block's eight waves then put one DFS wave on each SIMD; single-pair blocks, three to a
     // CU, can stack two DFS waves on one SIMD (78,643,200 x 1 KiB, 3 keys: 12.90 against
     // 13.06 ms; 4 keys: 14.65 against 15.15 ms). Two pairs per block measured slower than
-    // either. One or two keys per group: single-pair blocks (1 key: 11.39 against 11.51 ms).
+    // either. One or two keys per group: single-pair blocks (1 key: 11.39 against 11.51 ms),
+    // unless the pass has several groups: a 4-pair block then holds one part's pairs of four
+    // groups, which read the same rows at about the same time and share them in L2
+    // (19,660,800 x 4 KiB, 4 keys as 2 + 2: 16.53 against 20.80 ms).
     uint32_t wpb = o->waves_per_block;
     if (wpb != 2 && wpb != 4 && wpb != 8) {
       // 0, or a block size meant for the kernel AUTO would otherwise have picked.
       if (wpb && o->kernel != MG_KERNEL_AUTO)
         return fail(MG_ERR_INVALID_ARG, "split kernel blocks hold 2, 4 or 8 waves (got %u)", wpb);
-      wpb = G >= 3 && split_lds_bytes(G, pl->kl, 4) <= dev->max_lds ? 8 : 2;
+      wpb = (G >= 3 || groups >= 2) && split_lds_bytes(G, pl->kl, 4) <= dev->max_lds ? 8 : 2;
     }
     pl->waves_per_block = wpb;
     const int NP = (int)wpb / 2;
