@@ -100,7 +100,8 @@ typedef struct {
   uint32_t lane_depth;      /* kl: leaves per lane = 2^kl; 0 = tuned       */
   uint32_t keys_per_wave;   /* stream and split kernels, 1-4, clamped to   */
                             /* the width's cap (stream: 3 at 2 KiB rows,   */
-                            /* 2 at 4 KiB; split: 3 at 4 KiB); 0 = tuned   */
+                            /* 2 at 4 KiB; split: 3 at 4 KiB, 4 in column  */
+                            /* halves); 0 = tuned                          */
   uint32_t waves_per_key;   /* 0 = tuned                                   */
   uint32_t waves_per_block; /* lane-DFS, stream 1-4; split 2, 4, 8;
                                0 = tuned                                   */
@@ -115,6 +116,9 @@ typedef struct {
                                instead of LDS (0-4; 1 = none)              */
   uint32_t row_loads;       /* lane-DFS: 0 = tuned, 1 = a leaf pair's rows
                                are loaded before its expansions, 2 = after */
+  uint32_t row_halves;      /* split kernel, rows over 2 KiB: 0 = tuned,
+                               1 = one pair per row, 2 = two pairs, one
+                               per 128-chunk column half (4 keys each)     */
 } mg_scan_opts;
 
 int mg_abi_version(void);

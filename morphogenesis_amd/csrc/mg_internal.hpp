@@ -62,6 +62,7 @@ struct Plan {
   uint64_t steps_per_wave;  // split kernel: leaf-pair steps per DFS/stream pair
   uint32_t gslots;          // lane-DFS: stack levels kept in global memory
   uint32_t late_loads;      // lane-DFS: rows loaded after a step's expansions
+  uint32_t col_splits;      // split kernel: 1, or 2 pairs per row, one per 128-chunk half
   uint64_t row_begin, row_end;
   uint64_t n_delta;
   const void* rows;

@@ -558,17 +558,22 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
   const bool producer = wib < NP;
   const int pi = wib % NP;  // this wave's pair in the block
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
-  const uint32_t total = p.parts * n_groups;  // pairs with work
+  const uint32_t halves = p.col_splits;
+  const uint32_t total = p.parts * n_groups * halves;  // pairs with work
   const uint32_t per = (gridDim.x + kXcds - 1) / kXcds;
   const uint32_t blk = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;  // XCD-aware
   if ((uint64_t)blk * NP >= total) return;  // the whole block
+  // Column half fastest, then group, then part: the pairs of a block read the same rows.
   const uint32_t pid = blk * NP + (uint32_t)pi;
   const bool live = pid < total;
-  const uint32_t grp = live ? pid % n_groups : 0;
-  const uint32_t part = live ? pid / n_groups : 0;
+  const uint32_t half = live ? pid % halves : 0;
+  const uint32_t grp = live ? (pid / halves) % n_groups : 0;
+  const uint32_t part = live ? pid / (halves * n_groups) : 0;
   const uint32_t key0 = grp * G;
   const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
-  const uint32_t C = p.chunks;
+  const uint32_t C = p.chunks;  // row stride in chunks
+  const uint32_t col0 = half * 128;  // this pair's chunks: [col0, col_end)
+  const uint32_t col_end = halves == 1 ? C : (col0 + 128 < C ? col0 + 128 : C);
   // This pair's leaf-pair steps: global steps gs0 .. gs0 + n_steps - 1, where global step
   // gs is pair gs mod 2^(kl-1) of tile tile_begin + gs / 2^(kl-1).
   const uint32_t n_pairs = 1u << (kl - 1);
@@ -711,8 +716,8 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
   uint32_t lane_off[S];
 #pragma unroll
   for (int j = 0; j < S; j++) {
-    const uint32_t c = lane + 64 * j;
-    lane_off[j] = c < C ? c : 0;
+    const uint32_t c = col0 + lane + 64 * j;
+    lane_off[j] = c < col_end ? c : 0;
   }
   uint32_t row_off[U][S];
 #pragma unroll
@@ -791,8 +796,8 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
     if (key0 + g >= p.n_keys) continue;
 #pragma unroll
     for (int j = 0; j < S; j++) {
-      const uint32_t c = lane + 64 * j;
-      if (c < C) p.partial[((size_t)part * p.n_keys + key0 + g) * C + c] = acc[g][j];
+      const uint32_t c = col0 + lane + 64 * j;
+      if (c < col_end) p.partial[((size_t)part * p.n_keys + key0 + g) * C + c] = acc[g][j];
     }
   }
 }
@@ -1049,8 +1054,10 @@ hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s) {
 #define MG_SPLIT_ALL MG_SPLIT_SHAPES(1) MG_SPLIT_SHAPES(2) MG_SPLIT_SHAPES(4)
 
 hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t s) {
+  if (p.col_splits != 1 && !(p.col_splits == 2 && S == 2 && p.chunks > 128 && p.chunks <= 256))
+    return hipErrorInvalidValue;
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
-  const uint32_t total = p.parts * n_groups;
+  const uint32_t total = p.parts * n_groups * p.col_splits;
   const uint32_t blocks = (total + NP - 1) / NP;
   const dim3 block(128 * NP);
   const dim3 grid((blocks + kXcds - 1) / kXcds * kXcds);

@@ -182,6 +182,22 @@ constexpr int kSplitPairsPerCu[5] = {0, 3, 4, 0, 4};
 
 int stream_s(uint32_t chunks) { return chunks <= 64 ? 1 : (chunks <= 128 ? 2 : 4); }
 
+// The split kernel's chunks per lane group: a pair that takes one 128-chunk column half of
+// each row works as on 2 KiB rows.
+int split_s(const Plan* pl) { return pl->col_splits == 2 ? 2 : stream_s(pl->chunks); }
+
+// Keys per pass from which rows of 129-256 chunks go in column halves by default. Over
+// 19,660,800 x 4 KiB rows: 4 keys 11.55 against 16.44 ms (one group of 4 per half against
+// 2 + 2), 8 keys 18.06 against 26.46 ms; 12 keys 29.8 ms both ways; 1-3 keys 11.27-11.30
+// against 11.05-11.25 ms.
+constexpr uint32_t kSplitHalvesFromKeys = 4;
+
+// Rows of 129-256 chunks in column halves by default: two pairs per row, each holding up to
+// 4 keys' accumulators of its half, in place of one pair with at most 3 keys.
+bool split_halves_by_default(uint32_t chunks, uint32_t n_keys) {
+  return chunks > 128 && n_keys >= kSplitHalvesFromKeys;
+}
+
 // Rows per load instruction of the stream kernel: rows of at most 16 (32)
 // chunks go 4 (2) to a load. A 144-byte row then fills 36 lanes of a load
 // instead of 9.
@@ -372,6 +388,17 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     return fail(MG_ERR_INVALID_ARG, "suffix-walk kernel takes rows of at most 128 bytes, d >= 1");
   pl->kernel = kernel;
   pl->waves_per_block = o->waves_per_block ? o->waves_per_block : 4;
+  pl->col_splits = 1;
+  if (o->row_halves > 2)
+    return fail(MG_ERR_INVALID_ARG, "row_halves %u: 0 (tuned), 1 (whole rows) or 2 (halves)",
+                o->row_halves);
+  if (kernel == MG_KERNEL_STREAM_SPLIT) {
+    if (o->row_halves == 2 && pl->chunks <= 128)
+      return fail(MG_ERR_INVALID_ARG, "column halves need rows of more than 2 KiB");
+    pl->col_splits = (o->row_halves == 2 ||
+                      (o->row_halves == 0 && split_halves_by_default(pl->chunks, n_keys)))
+                         ? 2 : 1;
+  }
 
   if (kernel == MG_KERNEL_POINT) {
     pl->parts = (uint32_t)((pl->row_end - pl->row_begin + 63) / 64);
@@ -418,7 +445,7 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     // per block (1 key: 11.39 ms; 11.69 ms at kl = 6), three and four at kl = 6 with four pairs
     // per block (12.33 and 13.66 ms; four keys at kl = 5: 14.15 ms).
     const int g = o->keys_per_wave ? (int)o->keys_per_wave
-                                   : default_keys_per_wave(kernel, stream_s(pl->chunks), n_keys);
+                                   : default_keys_per_wave(kernel, split_s(pl), n_keys);
     kl_default = g <= 2 ? 7 : 6;
     if (d > 6) kl_default = std::min(kl_default, d - 6);
   } else {
@@ -462,7 +489,7 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
                         (int)pl->late_loads + (pl->gslots ? 2 : 0), 1,
                         (int)pl->chunks, (int)pl->waves_per_block, lds, &resident);
   } else {
-    const int S = stream_s(pl->chunks);
+    const int S = kernel == MG_KERNEL_STREAM_SPLIT ? split_s(pl) : stream_s(pl->chunks);
     int G = o->keys_per_wave ? (int)o->keys_per_wave : default_keys_per_wave(kernel, S, n_keys);
     G = std::max(1, std::min(G, max_keys_per_wave(kernel, S)));
     pl->keys_per_wave = (uint32_t)G;
@@ -484,7 +511,7 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   }
   if (kernel == MG_KERNEL_STREAM_SPLIT) {
     // A block holds NP pairs of a DFS wave and a streaming wave; parts count pairs.
-    const int S = stream_s(pl->chunks);
+    const int S = split_s(pl);
     const int G = (int)pl->keys_per_wave;
     // Default: four pairs per block from three keys per group on, when their LDS fits. A
     // block's eight waves then put one DFS wave on each SIMD; single-pair blocks, three to a
@@ -499,7 +526,10 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
       // 0, or a block size meant for the kernel AUTO would otherwise have picked.
       if (wpb && o->kernel != MG_KERNEL_AUTO)
         return fail(MG_ERR_INVALID_ARG, "split kernel blocks hold 2, 4 or 8 waves (got %u)", wpb);
-      wpb = (G >= 3 || groups >= 2) && split_lds_bytes(G, pl->kl, 4) <= dev->max_lds ? 8 : 2;
+      wpb = (G >= 3 || groups * pl->col_splits >= 2) &&
+                    split_lds_bytes(G, pl->kl, 4) <= dev->max_lds
+                ? 8
+                : 2;
     }
     pl->waves_per_block = wpb;
     const int NP = (int)wpb / 2;
@@ -519,7 +549,7 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   uint64_t parts = o->waves_per_key;
   if (!parts && kernel == MG_KERNEL_LANE_DFS && lane_shape.tiles_per_wave)
     parts = (n_tiles + lane_shape.tiles_per_wave - 1) / lane_shape.tiles_per_wave;
-  if (!parts) parts = std::max<uint64_t>(1, (uint64_t)resident / groups);
+  if (!parts) parts = std::max<uint64_t>(1, (uint64_t)resident / (groups * pl->col_splits));
   if (kernel == MG_KERNEL_STREAM_SPLIT && pl->waves_per_block == 2) {
     // Single-pair blocks take equal runs of leaf-pair steps, which may start and end inside a
     // tile, so every resident pair gets work (78,643,200 x 1 KiB at kl 7, 1 key: 9600 tiles are
@@ -592,12 +622,14 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       p.parts = pl.parts;
       p.partial = b.partial;
       p.zero_rows = (const uint4*)dev->zero_rows.p;
-      if (pl.kernel == MG_KERNEL_STREAM_SPLIT)
-        MG_HIP(launch_split(stream_s(pl.chunks), (int)pl.keys_per_wave,
+      if (pl.kernel == MG_KERNEL_STREAM_SPLIT) {
+        p.col_splits = pl.col_splits;
+        MG_HIP(launch_split(split_s(&pl), (int)pl.keys_per_wave,
                             (int)pl.waves_per_block / 2, p, s));
-      else
+      } else {
         MG_HIP(launch_stream(stream_s(pl.chunks), (int)pl.keys_per_wave, stream_pk(pl.chunks), p,
                              (int)pl.waves_per_block, s));
+      }
     } else if (pl.kernel == MG_KERNEL_SUFFIX) {
       SuffixParams p{};
       p.rows = (const uint4*)pl.rows;
@@ -764,7 +796,7 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
                    ? pl.parts * n_keys
                    : pl.parts * (stream ? (n_keys + pl.keys_per_wave - 1) / pl.keys_per_wave
                                         : n_keys) *
-                         (pl.kernel == MG_KERNEL_STREAM_SPLIT ? 2 : 1);
+                         (pl.kernel == MG_KERNEL_STREAM_SPLIT ? 2 * pl.col_splits : 1);
   }
   return MG_OK;
 }

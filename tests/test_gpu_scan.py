@@ -148,7 +148,8 @@ def test_stream_kernel_matches_oracle(mg, scanner, n, w, d):
 
 
 @pytest.mark.parametrize("n,w,d", [(3000, 1024, 12), (5000, 2048, 13), (777, 4096, 10),
-                                   (2500, 272, 12), (600, 32, 10), (70_001, 1024, 17)])
+                                   (2500, 272, 12), (600, 32, 10), (70_001, 1024, 17),
+                                   (3001, 3088, 12), (9000, 4096, 14)])
 def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
     # Producer/consumer blocks: masks from a DFS wave, rows from a streaming wave.
     print(f"case split n={n} w={w} d={d}", flush=True)
@@ -192,6 +193,26 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
                                         keys_per_wave=g, waves_per_key=wpk, waves_per_block=wpb)
                     got = scanner.scan_keys(m, keys, opts)
                     assert np.array_equal(got, ref), (g, kl, wpk, wpb)
+    # Rows over 2 KiB in two column halves, one pair each, up to 4 keys per pair.
+    for g in range(1, 5):
+        for kl in sorted({3, d - 6}):
+            for wpb in (0, 2, 4, 8):
+                for wpk in (0, 7):
+                    opts = mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=kl,
+                                        keys_per_wave=g, waves_per_block=wpb, waves_per_key=wpk,
+                                        row_halves=True)
+                    if w <= 2048:
+                        with pytest.raises(mg.MgError):
+                            scanner.scan_keys(m, keys, opts)
+                        continue
+                    if (wpb or 2) // 2 * (g * (kl - 1) * 64 + 256 * g) * 16 > 160 * 1024:
+                        continue
+                    got, t = scanner.scan_keys(m, keys, opts, with_timing=True)
+                    assert np.array_equal(got, ref), (g, kl, wpb, wpk)
+    bad_halves = mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT)
+    bad_halves.row_halves = 3
+    with pytest.raises(mg.MgError):
+        scanner.scan_keys(m, keys, bad_halves)
     for bad in (1, 3, 6):
         with pytest.raises(mg.MgError):
             scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT,
