@@ -224,15 +224,16 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
 
 
 # Kernel AUTO picks: lane-DFS up to 128-byte rows; the split kernel for
-# 528-1008-byte rows with >= 3 keys and 1024-4096-byte rows, when d >= 11;
-# the stream kernel otherwise. Small domains shrink the split kernel's lane depth.
+# 528-1008-byte rows with >= 3 keys and 1024-4096-byte rows, when d >= 11
+# (rows over 2 KiB in column halves from 4 keys); the stream kernel otherwise. Small domains shrink the split kernel's lane depth.
 @pytest.mark.parametrize("n,w,d,q,kernel", [
     (3000, 1024, 12, 1, "stream_split"), (2000, 1024, 11, 1, "stream_split"),
     (1000, 1024, 10, 1, "stream"), (3000, 768, 12, 3, "stream_split"),
     (3000, 768, 12, 2, "stream"), (3000, 512, 12, 3, "stream"), (3000, 128, 12, 3, "lane_dfs"),
     (3000, 2048, 12, 3, "stream_split"), (3000, 2048, 12, 2, "stream_split"),
     (3000, 4096, 12, 3, "stream_split"), (3000, 4096, 12, 1, "stream_split"),
-    (1000, 4096, 10, 3, "stream"), (9000, 1024, 14, 3, "stream_split"),
+    (1000, 4096, 10, 3, "stream"), (9000, 4096, 14, 4, "stream_split"),
+    (3000, 3088, 12, 5, "stream_split"), (9000, 1024, 14, 3, "stream_split"),
     (9000, 1024, 14, 4, "stream_split"), (9000, 1024, 14, 8, "stream_split")])
 def test_auto_kernel_choice(mg, scanner, n, w, d, q, kernel):
     print(f"case auto n={n} w={w} d={d} q={q}", flush=True)
