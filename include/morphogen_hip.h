@@ -116,9 +116,10 @@ typedef struct {
                                instead of LDS (0-4; 1 = none)              */
   uint32_t row_loads;       /* lane-DFS: 0 = tuned, 1 = a leaf pair's rows
                                are loaded before its expansions, 2 = after */
-  uint32_t row_halves;      /* split kernel, rows over 2 KiB: 0 = tuned,
-                               1 = one pair per row, 2 = two pairs, one
-                               per 128-chunk column half (4 keys each)     */
+  uint32_t row_halves;      /* split kernel, rows over 1 KiB: 0 = tuned
+                               (halves over 2 KiB from 4 keys), 1 = one
+                               pair per row, 2 = two pairs, one per column
+                               half of 64 (to 2 KiB) or 128 chunks          */
 } mg_scan_opts;
 
 int mg_abi_version(void);

@@ -278,8 +278,9 @@ def make_opts(kernel=KERNEL_AUTO, lane_depth=0, keys_per_wave=0, waves_per_key=0
     """stack_global: lane-DFS stack levels kept in global memory (None = tuned);
     late_loads: lane-DFS rows loaded after a leaf pair's expansions (True) or
     before them (False); None = tuned. row_halves: split kernel on rows over
-    2 KiB, two pairs per row, one per 128-chunk column half (True), or one pair
-    per row (False); None = tuned."""
+    1 KiB, two pairs per row, one per column half of 64 (rows up to 2 KiB) or
+    128 chunks (True), or one pair per row (False); None = tuned (halves on
+    rows over 2 KiB from 4 keys)."""
     o = ScanOpts()
     o.kernel = kernel
     o.lane_depth = lane_depth

@@ -572,8 +572,8 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
   const uint32_t key0 = grp * G;
   const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
   const uint32_t C = p.chunks;  // row stride in chunks
-  const uint32_t col0 = half * 128;  // this pair's chunks: [col0, col_end)
-  const uint32_t col_end = halves == 1 ? C : (col0 + 128 < C ? col0 + 128 : C);
+  const uint32_t col0 = half * 64 * S;  // this pair's chunks: [col0, col_end)
+  const uint32_t col_end = halves == 1 ? C : (col0 + 64 * S < C ? col0 + 64 * S : C);
   // This pair's leaf-pair steps: global steps gs0 .. gs0 + n_steps - 1, where global step
   // gs is pair gs mod 2^(kl-1) of tile tile_begin + gs / 2^(kl-1).
   const uint32_t n_pairs = 1u << (kl - 1);
@@ -1054,7 +1054,8 @@ hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s) {
 #define MG_SPLIT_ALL MG_SPLIT_SHAPES(1) MG_SPLIT_SHAPES(2) MG_SPLIT_SHAPES(4)
 
 hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t s) {
-  if (p.col_splits != 1 && !(p.col_splits == 2 && S == 2 && p.chunks > 128 && p.chunks <= 256))
+  if (p.col_splits != 1 &&
+      !(p.col_splits == 2 && p.chunks > 64u * S && p.chunks <= 128u * S && S <= 2))
     return hipErrorInvalidValue;
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
   const uint32_t total = p.parts * n_groups * p.col_splits;

@@ -53,8 +53,9 @@ struct StreamParams {
   uint32_t parts;  // waves per key group
   uint4* partial;  // [parts][n_keys][chunks]
   const uint4* zero_rows;  // >= 256 zero chunks, read in place of absent rows
-  // scan_split: 1, or 2 when each pair takes one 128-chunk column half of every row (rows of
-  // 129-256 chunks with S = 2); the halves write disjoint chunks of the same partial slot.
+  // scan_split: 1, or 2 when each pair takes one 64 x S-chunk column half of every row (rows
+  // of 65-128 chunks with S = 1, 129-256 with S = 2); the halves write disjoint chunks of the
+  // same partial slot.
   uint32_t col_splits;
 };
 

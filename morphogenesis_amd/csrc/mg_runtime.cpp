@@ -184,12 +184,16 @@ int stream_s(uint32_t chunks) { return chunks <= 64 ? 1 : (chunks <= 128 ? 2 : 4
 
 // The split kernel's chunks per lane group: a pair that takes one 128-chunk column half of
 // each row works as on 2 KiB rows.
-int split_s(const Plan* pl) { return pl->col_splits == 2 ? 2 : stream_s(pl->chunks); }
+int split_s(const Plan* pl) {
+  return pl->col_splits == 2 ? (pl->chunks > 128 ? 2 : 1) : stream_s(pl->chunks);
+}
 
 // Keys per pass from which rows of 129-256 chunks go in column halves by default. Over
 // 19,660,800 x 4 KiB rows: 4 keys 11.55 against 16.44 ms (one group of 4 per half against
 // 2 + 2), 8 keys 18.06 against 26.46 ms; 12 keys 29.8 ms both ways; 1-3 keys 11.27-11.30
-// against 11.05-11.25 ms.
+// against 11.05-11.25 ms. 2 KiB rows keep whole rows: there a pair holds 4 keys either way,
+// and the halves only double the DFS work (39,321,600 x 2 KiB, 4 / 8 / 12 keys: 13.68 / 23.17
+// / 37.37 against 11.83 / 18.72 / 27.99 ms).
 constexpr uint32_t kSplitHalvesFromKeys = 4;
 
 // Rows of 129-256 chunks in column halves by default: two pairs per row, each holding up to
@@ -393,8 +397,8 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     return fail(MG_ERR_INVALID_ARG, "row_halves %u: 0 (tuned), 1 (whole rows) or 2 (halves)",
                 o->row_halves);
   if (kernel == MG_KERNEL_STREAM_SPLIT) {
-    if (o->row_halves == 2 && pl->chunks <= 128)
-      return fail(MG_ERR_INVALID_ARG, "column halves need rows of more than 2 KiB");
+    if (o->row_halves == 2 && pl->chunks <= 64)
+      return fail(MG_ERR_INVALID_ARG, "column halves need rows of more than 1 KiB");
     pl->col_splits = (o->row_halves == 2 ||
                       (o->row_halves == 0 && split_halves_by_default(pl->chunks, n_keys)))
                          ? 2 : 1;

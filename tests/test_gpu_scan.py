@@ -193,7 +193,7 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
                                         keys_per_wave=g, waves_per_key=wpk, waves_per_block=wpb)
                     got = scanner.scan_keys(m, keys, opts)
                     assert np.array_equal(got, ref), (g, kl, wpk, wpb)
-    # Rows over 2 KiB in two column halves, one pair each, up to 4 keys per pair.
+    # Rows over 1 KiB in two column halves, one pair each, up to 4 keys per pair.
     for g in range(1, 5):
         for kl in sorted({3, d - 6}):
             for wpb in (0, 2, 4, 8):
@@ -201,7 +201,7 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
                     opts = mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=kl,
                                         keys_per_wave=g, waves_per_block=wpb, waves_per_key=wpk,
                                         row_halves=True)
-                    if w <= 2048:
+                    if w <= 1024:
                         with pytest.raises(mg.MgError):
                             scanner.scan_keys(m, keys, opts)
                         continue
