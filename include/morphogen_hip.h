@@ -242,6 +242,24 @@ int mg_scan(mg_device *dev, mg_matrix *m, const uint8_t *keys,
             const size_t *key_lens, uint32_t n_keys, uint8_t *out,
             const mg_scan_opts *opts, mg_timing *timing);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mg_scan with the same keys, options and answers, but
+ * every device buffer the scan needs beyond the matrix and the delta (keys,
+ * per-wave partials, lane-DFS stack levels, answers) lives in one fixed
+ * array of MG_NOALLOC_SCRATCH_BYTES that the library holds on each GPU until
+ * the process ends; keys are staged in host memory reserved at
+ * mg_device_open. The call allocates nothing. A scan that needs more than
+ * the array, or more than MG_NOALLOC_MAX_KEYS keys, is refused with
+ * MG_ERR_INVALID_ARG; use_graph is refused too (instantiating a graph
+ * allocates). Calls through one mg_device serialise on its lock; calls
+ * through two handles opened on the same GPU share the array and race. */
+#define MG_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define MG_NOALLOC_MAX_KEYS 1024u
+int mg_scan_noalloc(mg_device *dev, mg_matrix *m, const uint8_t *keys,
+                    const size_t *key_lens, uint32_t n_keys, uint8_t *out,
+                    const mg_scan_opts *opts, mg_timing *timing);
+
 /* scan_batch_single_query_multistream_optimized(_with_graph)
  * (kernel.rs:955-1097): one pass per query of keys_per_query keys, the
  * reference's "plain" serving mode; same answers as mg_scan. */

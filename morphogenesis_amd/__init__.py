@@ -163,6 +163,8 @@ def _load():
         "mg_key_full_eval": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, u8p, ctypes.c_uint64]),
         "mg_scan": (ctypes.c_int, [vp, vp, u8p, szp, ctypes.c_uint32, u8p,
                                    ctypes.POINTER(ScanOpts), ctypes.POINTER(Timing)]),
+        "mg_scan_noalloc": (ctypes.c_int, [vp, vp, u8p, szp, ctypes.c_uint32, u8p,
+                                           ctypes.POINTER(ScanOpts), ctypes.POINTER(Timing)]),
         "mg_scan_per_query": (ctypes.c_int, [vp, vp, u8p, szp, ctypes.c_uint32, ctypes.c_uint32,
                                              u8p, ctypes.POINTER(ScanOpts),
                                              ctypes.POINTER(Timing)]),
@@ -339,6 +341,21 @@ class GpuScanner:
         t = Timing()
         _check(lib.mg_scan(self._h, db._h, _u8(blob), lens, n, _u8(out),
                            ctypes.byref(opts) if opts is not None else None, ctypes.byref(t)))
+        out = out[:n]
+        return (out, t.as_dict()) if with_timing else out
+
+    def scan_keys_noalloc(self, db: "GpuPageMatrix", keys: Sequence[bytes],
+                          opts: ScanOpts = None, with_timing=False):
+        """scan_keys through mg_scan_noalloc: the same answers, with every per-call
+        device buffer inside the library's fixed scratch array (MG_NOALLOC_SCRATCH_BYTES
+        per GPU, at most MG_NOALLOC_MAX_KEYS keys)."""
+        n = len(keys)
+        out = np.zeros((max(n, 1), db.row_bytes), dtype=np.uint8)
+        blob, lens = _pack_keys(keys)
+        t = Timing()
+        _check(lib.mg_scan_noalloc(self._h, db._h, _u8(blob), lens, n, _u8(out),
+                                   ctypes.byref(opts) if opts is not None else None,
+                                   ctypes.byref(t)))
         out = out[:n]
         return (out, t.as_dict()) if with_timing else out
 

@@ -109,6 +109,10 @@ struct mg_device {
   mg::Plan graph_plan;
   hipGraphExec_t graph_exec = nullptr;
   std::map<std::pair<uint32_t, uint32_t>, int> occupancy;  // (variant, lds) -> blocks/CU
+  // mg_scan_noalloc: this device's address of the library's fixed scratch
+  // array (set at open) and host room for MG_NOALLOC_MAX_KEYS parsed keys.
+  void* noalloc_scratch = nullptr;
+  std::vector<mg::DevKey> noalloc_keys;
   // Matrices, deltas and request loops still open on this device: closing
   // the device first is refused.
   std::atomic<int> children{0};
@@ -160,6 +164,17 @@ inline size_t lane_gstack_bytes(const Plan& pl, uint32_t n_keys) {
 
 int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_t ev_scan0,
                  hipEvent_t ev_scan1);
+
+// Sets the plan's delta fields from opts->delta after checking it against
+// the matrix and device.
+int attach_delta(mg_device* dev, const mg_matrix* m, const mg_scan_opts* opts, Plan* pl);
+
+// Fills *t from dev->ev[0..4] (H2D, scan kernel, reductions, D2H) and the plan.
+int fill_timing(mg_device* dev, const Plan& pl, double t0, mg_timing* t);
+
+// Resolves this device's address of the no-alloc scratch array (mg_noalloc.hip);
+// called by mg_device_open with the device current.
+int noalloc_setup(mg_device* dev);
 
 // Collapses repeated row indices by XOR-ing their diffs (for the merge kernel).
 void dedupe_delta(const uint64_t* idx, const uint8_t* diffs, uint64_t n, uint32_t row_bytes,

@@ -683,6 +683,48 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
   return MG_OK;
 }
 
+int attach_delta(mg_device* dev, const mg_matrix* m, const mg_scan_opts* opts, Plan* pl) {
+  const mg_delta* dl = opts ? opts->delta : nullptr;
+  if (!dl) return MG_OK;
+  if (dl->row_bytes != m->row_bytes)
+    return fail(MG_ERR_INVALID_ARG, "delta rows are %u B, matrix rows %u B", dl->row_bytes,
+                m->row_bytes);
+  if (dl->dev != dev) return fail(MG_ERR_INVALID_ARG, "delta belongs to another device");
+  if (dl->n && dl->max_idx >= m->n_rows)
+    return fail(MG_ERR_ROWS, "delta row %llu outside the %llu-row matrix",
+                (unsigned long long)dl->max_idx, (unsigned long long)m->n_rows);
+  pl->n_delta = dl->n;
+  pl->delta_idx = dl->d_idx;
+  pl->delta_rows = dl->d_diffs;
+  pl->delta_parts = (uint32_t)((dl->n + 63) / 64);
+  return MG_OK;
+}
+
+int fill_timing(mg_device* dev, const Plan& pl, double t0, mg_timing* t) {
+  float h2d = 0, scan = 0, kern = 0, d2h = 0;
+  MG_HIP(hipEventElapsedTime(&h2d, dev->ev[0], dev->ev[1]));
+  MG_HIP(hipEventElapsedTime(&scan, dev->ev[1], dev->ev[2]));
+  MG_HIP(hipEventElapsedTime(&kern, dev->ev[1], dev->ev[3]));
+  MG_HIP(hipEventElapsedTime(&d2h, dev->ev[3], dev->ev[4]));
+  t->h2d_ms = h2d;
+  t->scan_kernel_ms = scan;
+  t->kernel_ms = kern;
+  t->d2h_ms = d2h;
+  t->total_ms = now_ms() - t0;
+  t->kernel = pl.kernel;
+  t->lane_depth = pl.kl;
+  t->stack_global = pl.gslots;
+  const uint32_t n_keys = pl.n_keys;
+  const bool stream = pl.kernel == MG_KERNEL_STREAM || pl.kernel == MG_KERNEL_STREAM_SPLIT;
+  t->keys_per_wave = stream ? pl.keys_per_wave : 1;
+  t->waves = (pl.kernel == MG_KERNEL_POINT || pl.kernel == MG_KERNEL_SUFFIX)
+                 ? pl.parts * n_keys
+                 : pl.parts * (stream ? (n_keys + pl.keys_per_wave - 1) / pl.keys_per_wave
+                                      : n_keys) *
+                       (pl.kernel == MG_KERNEL_STREAM_SPLIT ? 2 * pl.col_splits : 1);
+  return MG_OK;
+}
+
 }  // namespace mg
 
 using mg::DevKey;
@@ -735,20 +777,7 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
   mg::Plan pl;
   int rc = mg::make_plan(dev, m->rows, m->n_rows, m->row_bytes, n_keys, d, opts, &pl);
   if (rc) return rc;
-  const mg_delta* dl = opts ? opts->delta : nullptr;
-  if (dl) {
-    if (dl->row_bytes != m->row_bytes)
-      return fail(MG_ERR_INVALID_ARG, "delta rows are %u B, matrix rows %u B", dl->row_bytes,
-                  m->row_bytes);
-    if (dl->dev != dev) return fail(MG_ERR_INVALID_ARG, "delta belongs to another device");
-    if (dl->n && dl->max_idx >= m->n_rows)
-      return fail(MG_ERR_ROWS, "delta row %llu outside the %llu-row matrix",
-                  (unsigned long long)dl->max_idx, (unsigned long long)m->n_rows);
-    pl.n_delta = dl->n;
-    pl.delta_idx = dl->d_idx;
-    pl.delta_rows = dl->d_diffs;
-    pl.delta_parts = (uint32_t)((dl->n + 63) / 64);
-  }
+  if ((rc = mg::attach_delta(dev, m, opts, &pl))) return rc;
   const size_t out_bytes = (size_t)n_keys * m->row_bytes;
   if ((rc = dev->keys.reserve(sizeof(DevKey) * n_keys))) return rc;
   if ((rc = dev->out.reserve(out_bytes))) return rc;
@@ -780,28 +809,7 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
   MG_HIP(hipEventRecord(dev->ev[4], s));
   MG_HIP(hipStreamSynchronize(s));
   memcpy(out, dev->h_out.p, out_bytes);
-  if (t) {
-    float h2d = 0, scan = 0, kern = 0, d2h = 0;
-    MG_HIP(hipEventElapsedTime(&h2d, dev->ev[0], dev->ev[1]));
-    MG_HIP(hipEventElapsedTime(&scan, dev->ev[1], dev->ev[2]));
-    MG_HIP(hipEventElapsedTime(&kern, dev->ev[1], dev->ev[3]));
-    MG_HIP(hipEventElapsedTime(&d2h, dev->ev[3], dev->ev[4]));
-    t->h2d_ms = h2d;
-    t->scan_kernel_ms = scan;
-    t->kernel_ms = kern;
-    t->d2h_ms = d2h;
-    t->total_ms = mg::now_ms() - t0;
-    t->kernel = pl.kernel;
-    t->lane_depth = pl.kl;
-    t->stack_global = pl.gslots;
-    const bool stream = pl.kernel == MG_KERNEL_STREAM || pl.kernel == MG_KERNEL_STREAM_SPLIT;
-    t->keys_per_wave = stream ? pl.keys_per_wave : 1;
-    t->waves = (pl.kernel == MG_KERNEL_POINT || pl.kernel == MG_KERNEL_SUFFIX)
-                   ? pl.parts * n_keys
-                   : pl.parts * (stream ? (n_keys + pl.keys_per_wave - 1) / pl.keys_per_wave
-                                        : n_keys) *
-                         (pl.kernel == MG_KERNEL_STREAM_SPLIT ? 2 * pl.col_splits : 1);
-  }
+  if (t) return mg::fill_timing(dev, pl, t0, t);
   return MG_OK;
 }
 
@@ -856,6 +864,7 @@ int mg_device_open(int ordinal, mg_device** out) {
   if (!rc && (mg::launch_zero((uint4*)dev->zero_rows.p, 4096 / 16, dev->stream) != hipSuccess ||
               hipStreamSynchronize(dev->stream) != hipSuccess))
     rc = fail(MG_ERR_DEVICE, "zeroing the zero-row buffer failed");
+  if (!rc) rc = mg::noalloc_setup(dev);
   if (rc) {
     mg_device_close(dev);
     return rc;
