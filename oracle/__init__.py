@@ -65,6 +65,17 @@ def lib():
                                               ctypes.c_uint32, ctypes.c_int, u8p]),
             "orc_scan_delta": (ctypes.c_int, [kp, ctypes.c_int, u64p, u8p, ctypes.c_uint64,
                                               ctypes.c_uint32, u8p]),
+            "orc_path_node": (None, [kp, ctypes.c_uint64, ctypes.c_int, u32p, u8p]),
+            "orc_eval_subtree_fast": (ctypes.c_int, [kp, ctypes.c_uint64, ctypes.c_uint64,
+                                                     u32p]),
+            "orc_scan_fast": (ctypes.c_int, [kp, ctypes.c_int, u8p, ctypes.c_uint64,
+                                             ctypes.c_uint32, ctypes.c_int, u8p]),
+            "orc_scan_rows8": (ctypes.c_int, [kp, ctypes.c_int, u8p, ctypes.c_uint64,
+                                              ctypes.c_uint32, ctypes.c_int, u8p]),
+            "orc_scan_seeded": (ctypes.c_int, [kp, ctypes.c_int, ctypes.c_uint64,
+                                               ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                               ctypes.c_uint32, ctypes.c_int, u8p]),
+            "orc_simd_isa": (ctypes.c_char_p, []),
             "orc_splitmix64_word": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64]),
             "orc_fill_rows": (None, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                      ctypes.c_uint32, u8p]),
@@ -190,6 +201,55 @@ def scan_tiled(keys, rows: np.ndarray, n_threads=1) -> np.ndarray:
     _check(lib().orc_scan_tiled(arr, len(keys), _ptr(rows), n, w, n_threads, _ptr(out)),
            "scan_tiled")
     return out
+
+
+def _scan_rows_fn(fn, name, keys, rows, n_threads):
+    arr = _keys_array(keys)
+    rows = np.ascontiguousarray(rows, dtype=np.uint8)
+    n, w = rows.shape
+    out = np.zeros((len(keys), w), dtype=np.uint8)
+    _check(fn(arr, len(keys), _ptr(rows), n, w, n_threads, _ptr(out)), name)
+    return out
+
+
+def scan_fast(keys, rows: np.ndarray, n_threads=1) -> np.ndarray:
+    """scan_tiled's answers from the 16-lane vector ChaCha8 path (keys share one domain)."""
+    return _scan_rows_fn(lib().orc_scan_fast, "scan_fast", keys, rows, n_threads)
+
+
+def scan_rows8(keys, rows: np.ndarray, n_threads=1) -> np.ndarray:
+    """scan_tiled's answers in the reference server scan's eight-rows-per-step shape."""
+    return _scan_rows_fn(lib().orc_scan_rows8, "scan_rows8", keys, rows, n_threads)
+
+
+def scan_seeded(keys, seed, n_rows, row_bytes, row_begin=0, row_end=None, n_threads=1):
+    """Answers over rows [row_begin, row_end) of the n_rows x row_bytes matrix that
+    fill_rows(seed, ...) generates, regenerated tile by tile (no host copy)."""
+    arr = _keys_array(keys)
+    row_end = n_rows if row_end is None else row_end
+    out = np.zeros((len(keys), row_bytes), dtype=np.uint8)
+    _check(lib().orc_scan_seeded(arr, len(keys), seed, n_rows, row_begin, row_end, row_bytes,
+                                 n_threads, _ptr(out)), "scan_seeded")
+    return out
+
+
+def eval_subtree_fast(key: OrcKey, start: int, size: int) -> np.ndarray:
+    out = np.zeros((size, 4), dtype=np.uint32)
+    _check(lib().orc_eval_subtree_fast(ctypes.byref(key), start, size,
+                                       _ptr(out, ctypes.c_uint32)), "eval_subtree_fast")
+    return out
+
+
+def path_node(key: OrcKey, index: int, levels: int):
+    seed = np.zeros(4, dtype=np.uint32)
+    t = ctypes.c_uint8()
+    lib().orc_path_node(ctypes.byref(key), index, levels, _ptr(seed, ctypes.c_uint32),
+                        ctypes.byref(t))
+    return seed, t.value
+
+
+def simd_isa() -> str:
+    return lib().orc_simd_isa().decode()
 
 
 def scan_delta(keys, row_idx, diffs: np.ndarray) -> np.ndarray:

@@ -173,6 +173,19 @@ void orc_eval(const orc_key *key, uint64_t index, uint32_t out[4]) {
   leaf_output(key, seed, t, out);
 }
 
+/* The node on index's path at tree level `levels` (0 <= levels <= d): the
+ * first `levels` steps of orc_eval (dpf.rs:107-134). */
+void orc_path_node(const orc_key *key, uint64_t index, int levels,
+                   uint32_t seed[4], uint8_t *t) {
+  memcpy(seed, key->root_seed, 16);
+  *t = key->root_t;
+  int d = key->domain_bits;
+  for (int level = 0; level < levels; level++) {
+    int bit = (int)((index >> (d - 1 - level)) & 1u);
+    ggm_step(key, level, bit, seed, t);
+  }
+}
+
 /* dpf.rs:210-246: both children of every node; the left child fills the
  * first half of the output. Only element 0 of each leaf slice is written. */
 static void expand_recursive(const orc_key *key, int level,

@@ -34,6 +34,10 @@
  *   orc_scan_*         brute force (per-row eval) == tiled form; two-party
  *                      XOR == target row; delta linearity; closed forms for
  *                      all-zero / all-ones rows.
+ *   oracle_fast.c      orc_eval_subtree_fast equals orc_eval_subtree on
+ *                      random and fuzzed keys; orc_scan_fast, orc_scan_rows8
+ *                      and orc_scan_seeded equal orc_scan_tiled (and brute
+ *                      force) on the same keys and rows, whole and in ranges.
  *   orc_lcg_fill       jump-ahead equals the sequential byte loop.
  *   orc_siphash        (c, d) = (1, 3) and (2, 4) equal OpenSSL 3's SIPHASH MAC
  *                      with c-rounds/d-rounds set (committed fixture
@@ -104,6 +108,30 @@ int orc_scan_tiled(const orc_key *keys, int n_keys, const uint8_t *rows,
 int orc_scan_delta(const orc_key *keys, int n_keys, const uint64_t *row_idx,
                    const uint8_t *diffs, uint64_t n_entries,
                    uint32_t row_bytes, uint8_t *out);
+
+/* The node on index's path at tree level `levels` (first steps of orc_eval). */
+void orc_path_node(const orc_key *key, uint64_t index, int levels,
+                   uint32_t seed[4], uint8_t *t);
+
+/* oracle_fast.c: the same results computed with 16-lane vector ChaCha8 and
+ * wide masked-XOR folds (pinned against the scalar functions above).
+ * orc_eval_subtree_fast: size <= 2^d and <= 2^20, else ORC_ERR_SUBTREE.
+ * orc_scan_fast / orc_scan_seeded: all keys share one domain (ORC_ERR_DOMAIN).
+ * orc_scan_seeded: answers over rows [row_begin, row_end) of the n_rows-row
+ * matrix that orc_fill_rows(seed) generates, without holding it.
+ * orc_scan_rows8: the reference server scan's eight-rows-per-step shape. */
+int orc_eval_subtree_fast(const orc_key *key, uint64_t start, uint64_t size,
+                          uint32_t *out);
+int orc_scan_fast(const orc_key *keys, int n_keys, const uint8_t *rows,
+                  uint64_t n_rows, uint32_t row_bytes, int n_threads,
+                  uint8_t *out);
+int orc_scan_seeded(const orc_key *keys, int n_keys, uint64_t seed,
+                    uint64_t n_rows, uint64_t row_begin, uint64_t row_end,
+                    uint32_t row_bytes, int n_threads, uint8_t *out);
+int orc_scan_rows8(const orc_key *keys, int n_keys, const uint8_t *rows,
+                   uint64_t n_rows, uint32_t row_bytes, int n_threads,
+                   uint8_t *out);
+const char *orc_simd_isa(void);
 
 uint64_t orc_splitmix64_word(uint64_t seed, uint64_t j);
 void orc_fill_rows(uint64_t seed, uint64_t first_row, uint64_t n_rows,

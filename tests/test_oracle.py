@@ -350,3 +350,115 @@ def test_lcg_jump_ahead_matches_sequential():
     assert oracle.lcg_fill(12345, 1234, 100).tolist() == seq[1234:1334]
     assert oracle.lcg_state_at(7, 0) == 7
     assert oracle.lcg_state_at(0, 1) == 1
+
+
+# ------------------------------------------------------ fast oracle forms --
+# oracle_fast.c restates the same scan with 16-lane vector ChaCha8 and wide
+# folds; each form is pinned against the scalar functions above.
+
+def fuzzed_key(d, target, seed, rng):
+    """A keygen key whose control bytes are replaced by arbitrary bytes."""
+    b = bytearray(oracle.keygen_bytes(d, target, seed=seed)[seed % 2])
+    b[16] = int(rng.choice([0, 1, 2, 0xFF]))
+    for lvl in range(d):
+        off = 19 + 18 * lvl
+        b[off + 16] = int(rng.choice([0, 1, 3, 0x81]))
+        b[off + 17] = int(rng.choice([0, 1, 2, 0xFE]))
+    return oracle.key_from_bytes(bytes(b))
+
+
+@pytest.mark.parametrize("d", [1, 4, 5, 12, 17, 20])
+def test_eval_subtree_fast_equals_scalar(d):
+    rng = np.random.default_rng(d)
+    keys = [oracle.keygen(d, int(rng.integers(0, 1 << d)), seed=s)[s % 2] for s in range(3)]
+    keys.append(fuzzed_key(d, int(rng.integers(0, 1 << d)), 9, rng))
+    for k in keys:
+        for levels in sorted({0, 1, 3, 4, 5, min(d, 12), d}):
+            if levels > d:
+                continue
+            size = 1 << levels
+            for start in {0, ((1 << d) - size), int(rng.integers(0, 1 << (d - levels))) * size}:
+                assert np.array_equal(oracle.eval_subtree_fast(k, start, size),
+                                      oracle.eval_subtree(k, start, size)), (d, levels, start)
+
+
+def test_eval_subtree_fast_refusals():
+    k = oracle.keygen(8, 3, seed=1)[0]
+    for start, size in [(0, 0), (0, 3), (2, 4), (0, 512)]:
+        with pytest.raises(oracle.OracleError) as e:
+            oracle.eval_subtree_fast(k, start, size)
+        assert e.value.code == oracle.ERR_SUBTREE
+
+
+def test_path_node_is_the_walk_of_eval():
+    d = 14
+    k = oracle.keygen(d, 9000, seed=4)[1]
+    seed, t = oracle.path_node(k, 9000, d)
+    leaf = oracle.eval_point(k, 9000)
+    fcw = np.array(list(k.final_cw), dtype=np.uint32)
+    assert np.array_equal(leaf, seed ^ fcw if t == 1 else seed)
+
+
+@pytest.mark.parametrize("n,w,d", [(1, 16, 0), (7, 16, 3), (255, 32, 8), (4096, 32, 12),
+                                   (5000, 48, 13), (9001, 64, 14), (3000, 144, 12),
+                                   (20000, 1024, 15), (1 << 16, 32, 16)])
+def test_fast_scans_equal_tiled(n, w, d):
+    rng = np.random.default_rng(n + w)
+    rows = oracle.fill_rows(seed=n, first_row=0, n_rows=n, row_bytes=w)
+    targets = [int(rng.integers(0, max(n, 1))) for _ in range(3)]
+    if d:
+        keys = [oracle.keygen(d, t, seed=i)[i % 2] for i, t in enumerate(targets)]
+        keys.append(fuzzed_key(d, targets[0], 5, rng))
+    else:
+        # d = 0 keys by hand: root t = 1 and 2 with a final correction word.
+        z = bytearray(35)
+        z[:16] = bytes(range(16))
+        z[19:35] = bytes([0xA5] * 16)
+        keys = []
+        for t in (1, 2):
+            z[16] = t
+            keys.append(bytes(z))
+    ref = oracle.scan_tiled(keys, rows, 3)
+    assert np.array_equal(oracle.scan_fast(keys, rows, 3), ref)
+    assert np.array_equal(oracle.scan_rows8(keys, rows, 3), ref)
+    assert np.array_equal(oracle.scan_seeded(keys, n, n, w, n_threads=3), ref)
+    # One thread and many threads agree (the per-thread folds XOR together).
+    assert np.array_equal(oracle.scan_fast(keys, rows, 1), ref)
+    if d:
+        # Two-party reconstruction through the fast form alone.
+        k0, k1 = oracle.keygen(d, targets[1], seed=77)
+        a = oracle.scan_seeded([k0, k1], n, n, w, n_threads=2)
+        assert np.array_equal(a[0] ^ a[1], rows[targets[1]])
+
+
+def test_scan_seeded_row_ranges():
+    # Answers over [a, b) equal the tiled scan of the matrix with every row
+    # outside [a, b) zeroed, and the ranges XOR to the whole (ragged tile edges).
+    n, w, d = 30000, 32, 15
+    rows = oracle.fill_rows(seed=3, first_row=0, n_rows=n, row_bytes=w)
+    keys = [oracle.keygen(d, t, seed=t)[t % 2] for t in (5, 17000, 29999)]
+    cuts = [0, 1, 4095, 4096, 12289, 29999, 30000]
+    parts = []
+    for a, b in zip(cuts, cuts[1:]):
+        got = oracle.scan_seeded(keys, 3, n, w, row_begin=a, row_end=b, n_threads=2)
+        masked = np.zeros_like(rows)
+        masked[a:b] = rows[a:b]
+        assert np.array_equal(got, oracle.scan_tiled(keys, masked, 2)), (a, b)
+        parts.append(got)
+    whole = np.bitwise_xor.reduce(np.stack(parts), axis=0)
+    assert np.array_equal(whole, oracle.scan_tiled(keys, rows, 2))
+
+
+def test_fast_scan_refusals():
+    k10 = oracle.keygen(10, 1, seed=1)[0]
+    k11 = oracle.keygen(11, 1, seed=1)[0]
+    rows = np.zeros((16, 32), dtype=np.uint8)
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.scan_fast([k10, k11], rows)
+    assert e.value.code == oracle.ERR_DOMAIN
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.scan_seeded([k10], 1, 2048, 32)  # more rows than 2^d
+    assert e.value.code == oracle.ERR_ROWS
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.scan_seeded([k10], 1, 1024, 32, row_begin=10, row_end=5)
+    assert e.value.code == oracle.ERR_ROWS
