@@ -1,0 +1,216 @@
+"""GPU parity at the benchmark workloads' full sizes.
+
+Each server's answer must equal the CPU scan (kernel.rs:1276-1371, restated by
+the oracle) byte for byte on the same keys and seeded rows, and the two
+servers' answers must XOR to the target row. The oracle regenerates the seeded
+rows tile by tile (oracle.scan_seeded), so the 75 GiB and 64 GiB matrices are
+checked without a host copy; the delta contribution follows scan.rs:434-485
+(oracle.scan_delta), applied on top as the reference applies it.
+
+Workloads:
+  78,643,200 rows x 1 KiB, 27-bit keys, 1 key and 3 cuckoo keys per pass;
+  2^31 rows x 32 B plus a 2^16-row epoch delta, 31-bit keys, 3 keys per pass
+  and 384 keys (192 pairs) in one pass;
+  serving on the 2^31 x 32 B matrix: a Poisson burst of uneven requests with
+  an epoch update in the middle.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def mg():
+    import morphogenesis_amd
+    return morphogenesis_amd
+
+
+@pytest.fixture(scope="module")
+def scanner(mg):
+    s = mg.GpuScanner(0)
+    yield s
+    s.close()
+
+
+def row_at(seed, t, w, idx=None, diffs=None):
+    """Row t of the seeded matrix, with every delta diff aimed at it XOR-ed in."""
+    r = oracle.fill_rows(seed, t, 1, w)[0].copy()
+    if idx is not None:
+        for j in np.nonzero(idx == np.uint64(t))[0]:
+            r ^= diffs[j]
+    return r
+
+
+# ------------------------------------------- 78,643,200 rows x 1 KiB pages --
+
+C3_ROWS, C3_W, C3_D, C3_SEED = 78_643_200, 1024, 27, 1303
+
+
+@pytest.fixture(scope="module")
+def c3_matrix(mg, scanner):
+    m = mg.GpuPageMatrix(scanner, C3_ROWS, C3_W)
+    m.fill_seeded(C3_SEED)
+    yield m
+    m.close()
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("n_keys", [1, 3])
+def test_page_rows_full_size_each_server_equals_oracle(mg, scanner, c3_matrix, n_keys):
+    rng = np.random.default_rng(27 + n_keys)
+    targets = [int(x) for x in rng.integers(0, C3_ROWS, size=n_keys)]
+    pairs = [mg.generate_keys(C3_D, t, seed=9000 + n_keys * 10 + i) for i, t in enumerate(targets)]
+    answers = []
+    for party in (0, 1):
+        keys = [p[party] for p in pairs]
+        t0 = time.time()
+        ans, tm = scanner.scan_keys(c3_matrix, keys, with_timing=True)
+        say(f"C3 {n_keys} keys party {party}: kernel {tm['kernel']} {tm['scan_kernel_ms']:.2f} ms")
+        ref = oracle.scan_seeded(keys, C3_SEED, C3_ROWS, C3_W, n_threads=THREADS)
+        say(f"  oracle {time.time() - t0:.1f} s")
+        assert np.array_equal(ans, ref), f"party {party} differs from the oracle"
+        answers.append(ans)
+    for q, t in enumerate(targets):
+        assert np.array_equal(answers[0][q] ^ answers[1][q], row_at(C3_SEED, t, C3_W))
+
+
+# --------------------------------------------- 2^31 rows x 32 B + delta ----
+
+C4_ROWS, C4_W, C4_D, C4_SEED = 1 << 31, 32, 31, 3131
+C4_DELTA = 1 << 16
+
+
+@pytest.fixture(scope="module")
+def c4(mg, scanner):
+    m = mg.GpuPageMatrix(scanner, C4_ROWS, C4_W)
+    m.fill_seeded(C4_SEED)
+    rng = np.random.default_rng(31)
+    idx = rng.integers(0, C4_ROWS, size=C4_DELTA).astype(np.uint64)
+    diffs = rng.integers(0, 256, size=(C4_DELTA, C4_W), dtype=np.uint8)
+    dl = mg.DeltaBuffer(scanner, C4_W, C4_DELTA)
+    dl.set(idx, diffs, epoch=1)
+    yield m, dl, idx, diffs
+    dl.close()
+    m.close()
+
+
+def c4_oracle(keys, idx, diffs):
+    return (oracle.scan_seeded(keys, C4_SEED, C4_ROWS, C4_W, n_threads=THREADS)
+            ^ oracle.scan_delta(keys, idx, diffs))
+
+
+@pytest.mark.timeout(900)
+def test_mainnet_three_keys_each_server_equals_oracle(mg, scanner, c4):
+    m, dl, idx, diffs = c4
+    rng = np.random.default_rng(3)
+    targets = [int(x) for x in rng.integers(0, C4_ROWS, size=3)]
+    targets[0] = int(idx[7])  # one target inside the delta
+    pairs = [mg.generate_keys(C4_D, t, seed=31_000 + i) for i, t in enumerate(targets)]
+    opts = mg.make_opts(delta=dl)
+    answers = []
+    for party in (0, 1):
+        keys = [p[party] for p in pairs]
+        ans, tm = scanner.scan_keys(m, keys, opts, with_timing=True)
+        say(f"C4 3 keys party {party}: kernel {tm['kernel']} kl={tm['lane_depth']} "
+            f"{tm['scan_kernel_ms']:.1f} ms")
+        t0 = time.time()
+        ref = c4_oracle(keys, idx, diffs)
+        say(f"  oracle {time.time() - t0:.1f} s")
+        assert np.array_equal(ans, ref), f"party {party} differs from the oracle"
+        answers.append(ans)
+    for q, t in enumerate(targets):
+        assert np.array_equal(answers[0][q] ^ answers[1][q], row_at(C4_SEED, t, C4_W, idx, diffs))
+
+
+@pytest.mark.timeout(900)
+def test_mainnet_384_keys_one_pass(mg, scanner, c4):
+    m, dl, idx, diffs = c4
+    rng = np.random.default_rng(384)
+    targets = [int(x) for x in rng.integers(0, C4_ROWS, size=192)]
+    targets[5] = int(idx[100])
+    pairs = [mg.generate_keys(C4_D, t, seed=384_000 + i) for i, t in enumerate(targets)]
+    keys = [k for p in pairs for k in p]
+    ans, tm = scanner.scan_keys(m, keys, mg.make_opts(delta=dl), with_timing=True)
+    say(f"C4 384 keys: kernel {tm['kernel']} kl={tm['lane_depth']} waves={tm['waves']} "
+        f"{tm['scan_kernel_ms']:.0f} ms")
+    for i, t in enumerate(targets):
+        assert np.array_equal(ans[2 * i] ^ ans[2 * i + 1], row_at(C4_SEED, t, C4_W, idx, diffs)), i
+    # One pair byte for byte against the oracle, and the same pair in a pass of its own.
+    j = 5
+    ref = c4_oracle(list(pairs[j]), idx, diffs)
+    assert np.array_equal(ans[2 * j:2 * j + 2], ref)
+    alone = scanner.scan_keys(m, list(pairs[j]), mg.make_opts(delta=dl))
+    assert np.array_equal(alone, ref)
+
+
+# ---------------------------------------------- serving on 2^31 x 32 B ----
+
+@pytest.mark.timeout(900)
+def test_mainnet_serving_burst_with_epoch_update(mg, scanner, c4):
+    """Requests arrive as a Poisson process (20 per second) while the loop
+    serves; each carries 3 cuckoo lookups x k for both parties (k uneven, one
+    large request of 32), and a 2^16-row epoch update lands in the middle of
+    the burst. Each answer equals the same keys scanned alone on the matrix
+    state of its admission epoch, both parties' answers XOR to the target row of
+    that state, and one pair per epoch equals the oracle."""
+    m, _dl, _idx, _diffs = c4
+    rng = np.random.default_rng(55)
+    up_idx = rng.integers(0, C4_ROWS, size=1 << 16).astype(np.uint64)
+    up_diffs = rng.integers(0, 256, size=(1 << 16, C4_W), dtype=np.uint8)
+    srv = mg.GpuServer(scanner, m)
+    reqs = []
+    for r, k in enumerate([1, 2, 1, 3, 32, 1, 2]):
+        targets = [int(x) for x in rng.integers(0, C4_ROWS, size=3 * k)]
+        if r == 5:
+            targets[0] = int(up_idx[3])  # read a row the update changes
+        pairs = [mg.generate_keys(C4_D, t, seed=55_000 + 100 * r + i)
+                 for i, t in enumerate(targets)]
+        reqs.append((targets, pairs, [key for p in pairs for key in p]))
+    tickets = []
+    t_start = time.time()
+    for r, (_, _, keys) in enumerate(reqs):
+        tickets.append(srv.submit(keys))
+        if r == 2:
+            srv.update(up_idx, up_diffs, epoch=1)
+        time.sleep(float(rng.exponential(1 / 20)))
+    results = [srv.wait(t, len(reqs[r][2]), timeout_ms=600_000) for r, t in enumerate(tickets)]
+    say(f"burst served in {time.time() - t_start:.1f} s; "
+        f"epochs {[info['epoch'] for _, info in results]}; "
+        f"latency ms {[round(info['latency_ms']) for _, info in results]}")
+    stats = srv.stats()
+    srv.close()
+    assert stats["completed"] == len(reqs) and stats["epoch"] == 1
+    epochs = [info["epoch"] for _, info in results]
+    # Requests after the update see it; earlier ones may run at either epoch (a request
+    # still queued when the update lands answers at the newer one, labelled so), but
+    # epochs never go back in submission order.
+    assert all(e == 1 for e in epochs[3:]) and epochs == sorted(epochs)
+    up = mg.DeltaBuffer(scanner, C4_W, 1 << 16)
+    up.set(up_idx, up_diffs, epoch=1)
+    checked = set()
+    for r, ((ans, info), (targets, pairs, keys)) in enumerate(zip(results, reqs)):
+        e = info["epoch"]
+        opts = mg.make_opts(delta=up) if e == 1 else None
+        alone = scanner.scan_keys(m, keys, opts)
+        assert np.array_equal(ans, alone), f"request {r} differs from itself run alone"
+        for i, t in enumerate(targets):
+            want = row_at(C4_SEED, t, C4_W, *((up_idx, up_diffs) if e == 1 else (None, None)))
+            assert np.array_equal(ans[2 * i] ^ ans[2 * i + 1], want), (r, i)
+        if e not in checked:
+            checked.add(e)
+            ref = oracle.scan_seeded(list(pairs[0]), C4_SEED, C4_ROWS, C4_W, n_threads=THREADS)
+            if e == 1:
+                ref ^= oracle.scan_delta(list(pairs[0]), up_idx, up_diffs)
+            assert np.array_equal(ans[0:2], ref), f"request {r} (epoch {e}) differs from the oracle"
+    up.close()
