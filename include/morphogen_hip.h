@@ -120,6 +120,9 @@ typedef struct {
                                (halves over 2 KiB from 4 keys), 1 = one
                                pair per row, 2 = two pairs, one per column
                                half of 64 (to 2 KiB) or 128 chunks          */
+  uint32_t mask_depth;      /* split kernel: leaf-pair steps of masks a DFS
+                               wave may run ahead of its streaming wave,
+                               2-8; 0 = tuned (as many as LDS holds, <= 4)  */
 } mg_scan_opts;
 
 int mg_abi_version(void);

@@ -91,7 +91,8 @@ class ScanOpts(ctypes.Structure):
                 ("waves_per_block", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
                 ("row_begin", ctypes.c_uint64), ("row_end", ctypes.c_uint64),
                 ("delta", ctypes.c_void_p), ("stack_global", ctypes.c_uint32),
-                ("row_loads", ctypes.c_uint32), ("row_halves", ctypes.c_uint32)]
+                ("row_loads", ctypes.c_uint32), ("row_halves", ctypes.c_uint32),
+                ("mask_depth", ctypes.c_uint32)]
 
 
 class WireOpts(ctypes.Structure):
@@ -276,13 +277,14 @@ class PirResult:
 
 def make_opts(kernel=KERNEL_AUTO, lane_depth=0, keys_per_wave=0, waves_per_key=0,
               waves_per_block=0, use_graph=False, row_begin=0, row_end=0, delta=None,
-              stack_global=None, late_loads=None, row_halves=None):
+              stack_global=None, late_loads=None, row_halves=None, mask_depth=0):
     """stack_global: lane-DFS stack levels kept in global memory (None = tuned);
     late_loads: lane-DFS rows loaded after a leaf pair's expansions (True) or
     before them (False); None = tuned. row_halves: split kernel on rows over
     1 KiB, two pairs per row, one per column half of 64 (rows up to 2 KiB) or
     128 chunks (True), or one pair per row (False); None = tuned (halves on
-    rows over 2 KiB from 4 keys)."""
+    rows over 2 KiB from 4 keys). mask_depth: split kernel, leaf-pair steps of masks a
+    DFS wave may run ahead of its streaming wave (2-8; 0 = tuned)."""
     o = ScanOpts()
     o.kernel = kernel
     o.lane_depth = lane_depth
@@ -296,6 +298,7 @@ def make_opts(kernel=KERNEL_AUTO, lane_depth=0, keys_per_wave=0, waves_per_key=0
     o.stack_global = 0 if stack_global is None else 1 + stack_global
     o.row_loads = 0 if late_loads is None else (2 if late_loads else 1)
     o.row_halves = 0 if row_halves is None else (2 if row_halves else 1)
+    o.mask_depth = mask_depth
     return o
 
 

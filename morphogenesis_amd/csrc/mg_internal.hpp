@@ -63,6 +63,7 @@ struct Plan {
   uint32_t gslots;          // lane-DFS: stack levels kept in global memory
   uint32_t late_loads;      // lane-DFS: rows loaded after a step's expansions
   uint32_t col_splits;      // split kernel: 1, or 2 pairs per row, one per 128-chunk half
+  uint32_t mask_depth;      // split kernel: steps of masks a DFS wave may run ahead
   uint64_t row_begin, row_end;
   uint64_t n_delta;
   const void* rows;
@@ -103,6 +104,7 @@ struct mg_device {
   mg::DevBuf keys, partial, partial_delta, out;
   mg::DevBuf gstack;  // lane-DFS stack levels kept in global memory
   mg::DevBuf zero_rows;  // 4 KiB of zeros, read in place of rows past the end
+  mg::DevBuf fault;      // one word the split kernel sets if a hand-over wait runs out
   mg::HostBuf h_keys, h_out;
   // HIP graph of the kernel sequence for the last plan.
   bool graph_valid = false;
@@ -171,6 +173,10 @@ int attach_delta(mg_device* dev, const mg_matrix* m, const mg_scan_opts* opts, P
 
 // Fills *t from dev->ev[0..4] (H2D, scan kernel, reductions, D2H) and the plan.
 int fill_timing(mg_device* dev, const Plan& pl, double t0, mg_timing* t);
+
+// After a stream sync: MG_ERR_DEVICE (and the word cleared) if a split-kernel launch
+// since the last check reported a hand-over wait that ran out; a no-op for other plans.
+int check_fault(mg_device* dev, const Plan& pl);
 
 // Resolves this device's address of the no-alloc scratch array (mg_noalloc.hip);
 // called by mg_device_open with the device current.

@@ -537,18 +537,55 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
 // ------------------------------------------- split producer/consumer scan --
 // Wide rows, several keys per pass. A block holds NP producer/consumer pairs
 // of waves: waves 0 .. NP-1 run the keys' lane DFS exactly as scan_stream
-// does and publish each leaf-pair step's 2 x 64 masks per key into one of
-// two LDS mask buffers of their pair; waves NP .. 2NP-1 only stream their
-// pair's rows (NB batches of U rows in flight, including the next step's
-// first batches while they wait) and fold them. A block barrier per step
-// hands the buffers over, so a producer runs at most one step ahead and
-// never overwrites a buffer its consumer still reads. Every wave of the
-// block takes the same number of barriers: pairs with fewer steps (the last
-// part, or pairs past the grid's work) run empty steps. With NP = 4 the
-// waves of one block land one producer and one consumer on each SIMD, where
-// single-pair blocks can stack two DFS waves on one SIMD.
+// does and publish each leaf-pair step's 2 x 64 masks per key into a ring of
+// D mask buffers of their pair; waves NP .. 2NP-1 only stream their pair's
+// rows (NB batches of U rows in flight, including the next step's first
+// batches while they wait) and fold them. The two waves of a pair hand steps
+// over through two counters in LDS, steps published and steps folded: a DFS
+// wave runs up to D - 1 steps ahead and only waits when the ring is full, so
+// the steps that descend a whole subtree (kl expansions per key, against one
+// or two for most steps) no longer stall the streaming wave as a per-step
+// block barrier did. Pairs never wait for each other. Every wait is bounded
+// (kSplitSpinLimit polls); a wait that runs out sets *p.fault, which the host
+// reports as an error, and the wave stops, so the kernel always ends.
+// With NP = 4 the waves of one block land one producer and one consumer on
+// each SIMD, where single-pair blocks can stack two DFS waves on one SIMD.
 // Needs all 64 lanes live (lb = 6), so a step is always 128 rows.
 constexpr int kSplitNB = 4;
+constexpr uint32_t kSplitSpinLimit = 1u << 22;  // ~0.3 s of polls; a step takes microseconds
+
+__device__ __forceinline__ uint32_t lds_load_acquire(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void lds_store_release(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Waits until the counter has reached `want` (both count steps, so the
+// difference stays small and the signed compare is wrap-safe). False when the
+// wait ran out.
+__device__ __forceinline__ bool wait_count(const uint32_t* ctr, uint32_t want) {
+  for (uint32_t spin = 0; spin < kSplitSpinLimit; ++spin) {
+    if ((int32_t)(lds_load_acquire(ctr) - want) >= 0) return true;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return false;
+}
+
+// Stack levels of a depth-kl lane DFS kept in LDS by the split kernel: the
+// kl - 1 levels minus the top one (depth 1, used once per tile) and the two
+// bottom ones (depths kl - 1 and kl - 2, pushed and popped within one to
+// three steps), which the DFS wave holds in registers.
+__host__ __device__ __forceinline__ uint32_t split_lds_levels(uint32_t kl) {
+  return kl > 4 ? kl - 4 : 0;
+}
+
+// LDS of one pair, in uint4 units: G stacks of split_lds_levels x 64 seeds,
+// D mask buffers of G x 128 masks, and one uint4 holding the two counters.
+__host__ __device__ __forceinline__ size_t split_pair_u4(int G, uint32_t kl, uint32_t D) {
+  return (size_t)G * split_lds_levels(kl) * 64 + (size_t)D * G * 128 + 1;
+}
 
 template <int S, int G, int NP>
 __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
@@ -582,14 +619,25 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
   uint64_t n_steps = 0;
   if (live && gs0 < total_steps)
     n_steps = total_steps - gs0 < p.steps_per_wave ? total_steps - gs0 : p.steps_per_wave;
-  const uint64_t block_steps = p.steps_per_wave;  // barriers every wave takes
+  const uint32_t D = p.mask_depth;
   // Right children wait at depths 1 .. kl-1 (depth 0 is the lane root,
-  // never pushed), so a key's stack has kl - 1 slots of 64 seeds; their
-  // control bytes are kept as the bit t == 1 in a register per key.
+  // never pushed): a key's stack has kl - 1 slots, slot l holding depth l + 1.
+  // Slots 0, kl - 3 and kl - 2 sit in the DFS wave's registers, the others in
+  // LDS ([G][kl - 4][64] seeds, slot l at level l - 1); the control bytes are
+  // kept as the bit t == 1 in a register per key.
   const int slots = kl - 1;
-  uint4* pair_lds = lds + (size_t)pi * ((size_t)G * slots * 64 + (size_t)2 * G * 128);
-  uint4* stacks = pair_lds;                              // [G][slots][64]
-  uint4* masks = pair_lds + (size_t)G * slots * 64;      // [2][G][128]
+  const int lds_slots = (int)split_lds_levels(p.kl);
+  uint4* pair_lds = lds + (size_t)pi * split_pair_u4(G, p.kl, D);
+  uint4* stacks = pair_lds;                                  // [G][lds_slots][64]
+  uint4* masks = pair_lds + (size_t)G * lds_slots * 64;      // [D][G][128]
+  uint32_t* ctr = reinterpret_cast<uint32_t*>(masks + (size_t)D * G * 128);
+  uint32_t* published = ctr;  // steps whose masks are in the ring
+  uint32_t* folded = ctr + 1;  // steps whose masks the streaming wave is done with
+  if (producer && lane == 0) {
+    *published = 0;
+    *folded = 0;
+  }
+  __syncthreads();  // the only block barrier: counters are set before any wave reads them
 
   if (producer) {
     const DevKey* kp[G];
@@ -608,8 +656,44 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
     const int leaf_level = d - kl;
     Node node[G];
     uint32_t st_tbits[G];
+    // Register slots as plain words: [0] slot 0, [1] slot kl - 2, [2] slot kl - 3 (for
+    // small kl they may name the same slot; each holds what was last pushed there).
+    uint32_t st_reg[G][3][4];
 #pragma unroll
     for (int g = 0; g < G; g++) st_tbits[g] = 0;
+    // Seed of stack slot `slot` (uniform) for key g. The register slots are
+    // written and read as values (selects), never through a pointer, so they
+    // stay in VGPRs.
+    auto put_seed = [&](int g, int slot, const Node& n) __attribute__((always_inline)) {
+      const uint32_t v[4] = {n.s0, n.s1, n.s2, n.s3};
+      const bool in_reg[3] = {slot == 0, slot == slots - 1, slot == slots - 2};
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) st_reg[g][r][i] = in_reg[r] ? v[i] : st_reg[g][r][i];
+      if (!in_reg[0] && !in_reg[1] && !in_reg[2])
+        stacks[((size_t)g * lds_slots + (slot - 1)) * 64 + lane] = make_uint4(v[0], v[1], v[2], v[3]);
+    };
+    auto get_seed = [&](int g, int slot) __attribute__((always_inline)) -> uint4 {
+      uint32_t a[3][4];
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          a[r][i] = st_reg[g][r][i];
+          // Every word is read as a value: folding the selects below into a load
+          // through a selected address would keep the slots in scratch memory.
+          asm("" : "+v"(a[r][i]));
+        }
+      const int r = slot == 0 ? 0 : (slot == slots - 1 ? 1 : 2);
+      uint4 v = make_uint4(r == 0 ? a[0][0] : (r == 1 ? a[1][0] : a[2][0]),
+                           r == 0 ? a[0][1] : (r == 1 ? a[1][1] : a[2][1]),
+                           r == 0 ? a[0][2] : (r == 1 ? a[1][2] : a[2][2]),
+                           r == 0 ? a[0][3] : (r == 1 ? a[1][3] : a[2][3]));
+      if (slot != 0 && slot != slots - 1 && slot != slots - 2)
+        v = stacks[((size_t)g * lds_slots + (slot - 1)) * 64 + lane];
+      return v;
+    };
     // The next tile's lane roots, walked one tree level per step while the
     // current tile runs (levels pre_lvl .. tb + lb - 1 still to go), so that a
     // tile's first step only descends its kl levels. Walking the changed
@@ -630,80 +714,81 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
       }
       ++pre_lvl;
     };
-    for (uint64_t s = 0; s < block_steps; ++s) {
-      if (s < n_steps) {
-        const uint64_t gs = gs0 + s;
-        const uint64_t tile = p.tile_begin + (gs >> (kl - 1));
-        const uint32_t pair = (uint32_t)(gs & (n_pairs - 1));
-        const bool from_root = pair == 0 || s == 0;  // a new tile, or the run's first step
-        if (from_root) {
-          if (s != 0 && pre_lvl >= 0) {
-            while (pre_lvl < root_lvl) root_level(tile);  // finish the walk
+    for (uint64_t s = 0; s < n_steps; ++s) {
+      // Buffer s % D is free once the streaming wave has folded step s - D.
+      if (s >= D && !wait_count(folded, (uint32_t)(s - D + 1))) {
+        atomicOr(p.fault, 1u);
+        return;
+      }
+      const uint64_t gs = gs0 + s;
+      const uint64_t tile = p.tile_begin + (gs >> (kl - 1));
+      const uint32_t pair = (uint32_t)(gs & (n_pairs - 1));
+      const bool from_root = pair == 0 || s == 0;  // a new tile, or the run's first step
+      if (from_root) {
+        if (s != 0 && pre_lvl >= 0) {
+          while (pre_lvl < root_lvl) root_level(tile);  // finish the walk
 #pragma unroll
-            for (int g = 0; g < G; g++) node[g] = nroot[g];
-          } else {
+          for (int g = 0; g < G; g++) node[g] = nroot[g];
+        } else {
 #pragma unroll
-            for (int g = 0; g < G; g++)
-              node[g] = tile_lane_root(kp[g], path[g], lane, tile, tile - 1, s == 0, tb, lb);
-          }
-          pre_lvl = -1;
+          for (int g = 0; g < G; g++)
+            node[g] = tile_lane_root(kp[g], path[g], lane, tile, tile - 1, s == 0, tb, lb);
         }
-        const bool seek = s == 0 && pair != 0;
-        if (seek) {
-          // The run starts inside a tile: walk each key from the lane root down to leaf
-          // pair `pair`, keeping the right children that lie ahead of it, as the
-          // depth-first walk from pair 0 would have left them.
+        pre_lvl = -1;
+      }
+      const bool seek = s == 0 && pair != 0;
+      if (seek) {
+        // The run starts inside a tile: walk each key from the lane root down to leaf
+        // pair `pair`, keeping the right children that lie ahead of it, as the
+        // depth-first walk from pair 0 would have left them.
 #pragma unroll
-          for (int g = 0; g < G; g++) {
-            uint4* st_seed = stacks + (size_t)g * slots * 64;
-            for (int l = 0; l < kl - 1; ++l) {
-              Node L, R;
-              expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
-              if ((pair >> (kl - 2 - l)) & 1u) {
-                node[g] = R;
-              } else {
-                st_seed[l * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);
-                const uint32_t bit = 1u << (l + 1);
-                st_tbits[g] = R.t == 1u ? (st_tbits[g] | bit) : (st_tbits[g] & ~bit);
-                node[g] = L;
-              }
+        for (int g = 0; g < G; g++) {
+          for (int l = 0; l < kl - 1; ++l) {
+            Node L, R;
+            expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
+            if ((pair >> (kl - 2 - l)) & 1u) {
+              node[g] = R;
+            } else {
+              put_seed(g, l, R);
+              const uint32_t bit = 1u << (l + 1);
+              st_tbits[g] = R.t == 1u ? (st_tbits[g] | bit) : (st_tbits[g] & ~bit);
+              node[g] = L;
             }
           }
         }
-        uint4* mbuf = masks + (size_t)(s & 1) * G * 128;
-        const int lvl = seek ? kl - 1 : from_root ? 0 : kl - 1 - __builtin_ctz(pair);
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-          uint4* st_seed = stacks + (size_t)g * slots * 64;
-          if (!from_root) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
-            const uint4 sv = st_seed[(lvl - 1) * 64 + lane];
-            node[g].s0 = sv.x; node[g].s1 = sv.y; node[g].s2 = sv.z; node[g].s3 = sv.w;
-            node[g].t = (st_tbits[g] >> lvl) & 1u;
-          }
-          Node L, R;
-          for (int l = lvl; l < kl - 1; ++l) {
-            expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
-            st_seed[l * 64 + lane] = make_uint4(R.s0, R.s1, R.s2, R.s3);  // depth l + 1
-            const uint32_t bit = 1u << (l + 1);
-            st_tbits[g] = R.t == 1u ? (st_tbits[g] | bit) : (st_tbits[g] & ~bit);
-            node[g] = L;
-          }
-          expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + kl - 1, node[g], L, R);
-          mbuf[g * 128 + 2 * lane] = leaf_mask<false>(kp[g], L);
-          mbuf[g * 128 + 2 * lane + 1] = leaf_mask<false>(kp[g], R);
-        }
-        // One level of the next tile's lane-root walk, if this run reaches that tile.
-        if (((tile + 1 - p.tile_begin) << (kl - 1)) < gs0 + n_steps) {
-          if (pre_lvl < 0) {  // start at the first level where the two tiles' paths part
-            const uint64_t diff = (tile + 1) ^ tile;
-            pre_lvl = tb - (64 - __builtin_clzll(diff));
-#pragma unroll
-            for (int g = 0; g < G; g++) nroot[g] = path[g].get(pre_lvl);
-          }
-          if (pre_lvl < root_lvl) root_level(tile + 1);
-        }
       }
-      __syncthreads();  // step s's masks are published
+      uint4* mbuf = masks + (size_t)(s % D) * G * 128;
+      const int lvl = seek ? kl - 1 : from_root ? 0 : kl - 1 - __builtin_ctz(pair);
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        if (!from_root) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
+          const uint4 sv = get_seed(g, lvl - 1);
+          node[g].s0 = sv.x; node[g].s1 = sv.y; node[g].s2 = sv.z; node[g].s3 = sv.w;
+          node[g].t = (st_tbits[g] >> lvl) & 1u;
+        }
+        Node L, R;
+        for (int l = lvl; l < kl - 1; ++l) {
+          expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
+          put_seed(g, l, R);  // depth l + 1
+          const uint32_t bit = 1u << (l + 1);
+          st_tbits[g] = R.t == 1u ? (st_tbits[g] | bit) : (st_tbits[g] & ~bit);
+          node[g] = L;
+        }
+        expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + kl - 1, node[g], L, R);
+        mbuf[g * 128 + 2 * lane] = leaf_mask<false>(kp[g], L);
+        mbuf[g * 128 + 2 * lane + 1] = leaf_mask<false>(kp[g], R);
+      }
+      lds_store_release(published, (uint32_t)(s + 1));  // after this wave's mask stores
+      // One level of the next tile's lane-root walk, if this run reaches that tile.
+      if (((tile + 1 - p.tile_begin) << (kl - 1)) < gs0 + n_steps) {
+        if (pre_lvl < 0) {  // start at the first level where the two tiles' paths part
+          const uint64_t diff = (tile + 1) ^ tile;
+          pre_lvl = tb - (64 - __builtin_clzll(diff));
+#pragma unroll
+          for (int g = 0; g < G; g++) nroot[g] = path[g].get(pre_lvl);
+        }
+        if (pre_lvl < root_lvl) root_level(tile + 1);
+      }
     }
     return;
   }
@@ -761,10 +846,12 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
 #pragma unroll
     for (int k = 0; k < NB; k++) load(0, (uint32_t)k, ring[k]);
   }
-  for (uint64_t s = 0; s < block_steps; ++s) {
-    __syncthreads();  // step s's masks are in buffer s & 1
-    if (s >= n_steps) continue;
-    const uint4* mbuf = masks + (size_t)(s & 1) * G * 128;
+  for (uint64_t s = 0; s < n_steps; ++s) {
+    if (!wait_count(published, (uint32_t)(s + 1))) {  // step s's masks are in buffer s % D
+      atomicOr(p.fault, 1u);
+      return;
+    }
+    const uint4* mbuf = masks + (size_t)(s % D) * G * 128;
     for (uint32_t b0 = 0; b0 < n_batches; b0 += NB) {
 #pragma unroll
       for (int k = 0; k < NB; k++) {
@@ -788,6 +875,7 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
         if (ns < n_steps) load(ns, nb, ring[k]);
       }
     }
+    lds_store_release(folded, (uint32_t)(s + 1));  // after this step's mask reads
   }
 
   if (!live) return;
@@ -1062,7 +1150,8 @@ hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t
   const uint32_t blocks = (total + NP - 1) / NP;
   const dim3 block(128 * NP);
   const dim3 grid((blocks + kXcds - 1) / kXcds * kXcds);
-  const size_t lds = split_lds_bytes(G, p.kl, NP);
+  if (p.mask_depth < 2 || !p.fault) return hipErrorInvalidValue;
+  const size_t lds = split_lds_bytes(G, p.kl, NP, p.mask_depth);
 #define MG_SG(SS, GG, NN)                                                     \
   if (S == SS && G == GG && NP == NN) {                                       \
     hipLaunchKernelGGL((scan_split<SS, GG, NN>), grid, block, lds, s, p);     \
@@ -1073,8 +1162,8 @@ hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t
   return hipErrorInvalidValue;
 }
 
-size_t split_lds_bytes(int G, uint32_t kl, int NP) {
-  return (size_t)NP * ((size_t)G * (kl - 1) * 64 + (size_t)2 * G * 128) * sizeof(uint4);
+size_t split_lds_bytes(int G, uint32_t kl, int NP, uint32_t mask_depth) {
+  return (size_t)NP * split_pair_u4(G, kl, mask_depth) * sizeof(uint4);
 }
 
 hipError_t split_blocks_per_cu(int S, int G, int NP, size_t lds, int* blocks) {

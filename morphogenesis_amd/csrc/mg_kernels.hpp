@@ -57,6 +57,11 @@ struct StreamParams {
   // of 65-128 chunks with S = 1, 129-256 with S = 2); the halves write disjoint chunks of the
   // same partial slot.
   uint32_t col_splits;
+  // scan_split: leaf-pair steps of masks a DFS wave may run ahead of its streaming wave
+  // (the mask ring's depth, >= 2), and a device word the waves set when a hand-over wait
+  // runs out (a bug guard: every wait is bounded, so the kernel always ends).
+  uint32_t mask_depth;
+  uint32_t* fault;
 };
 
 struct PointParams {
@@ -96,7 +101,7 @@ size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block);
 // The producer/consumer variant of the stream kernel: blocks of NP (1, 2, 4)
 // pairs of a DFS wave and a row-streaming wave; needs lb = 6.
 hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t s);
-size_t split_lds_bytes(int G, uint32_t kl, int NP);
+size_t split_lds_bytes(int G, uint32_t kl, int NP, uint32_t mask_depth);
 hipError_t split_blocks_per_cu(int S, int G, int NP, size_t lds, int* blocks);
 // Resident blocks per CU for the given block size and dynamic LDS.
 hipError_t lane_dfs_blocks_per_cu(int chunks, bool late_loads, bool global_stack,

@@ -122,6 +122,7 @@ extern "C" int mg_scan_noalloc(mg_device* dev, mg_matrix* m, const uint8_t* keys
   MG_HIP(hipMemcpyAsync(out, base + l.out, out_bytes, hipMemcpyDeviceToHost, s));
   MG_HIP(hipEventRecord(dev->ev[4], s));
   MG_HIP(hipStreamSynchronize(s));
+  if ((rc = mg::check_fault(dev, pl))) return rc;
   if (timing) return mg::fill_timing(dev, pl, t0, timing);
   return MG_OK;
 }

@@ -174,6 +174,10 @@ int max_keys_per_wave(uint32_t kernel, int S) {
   return S == 1 ? 4 : (S == 2 ? 3 : 2);
 }
 
+// Split kernel: leaf-pair steps of masks a DFS wave may run ahead when the caller leaves it
+// open (fewer when the block's LDS does not hold them; at least 2).
+constexpr uint32_t kSplitDefaultDepth = 4;
+
 // Split kernel: the most producer/consumer pairs per CU to keep resident, indexed by pairs
 // per block (1, 2, 4). A fourth single-pair block fits the registers but does not pay at one
 // key either: 78,643,200 x 1 KiB 11.62 against 11.28-11.32 ms, 19,660,800 x 4 KiB 12.14-12.17
@@ -359,6 +363,8 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     return fail(MG_ERR_INVALID_ARG, "waves_per_block must be 0-4 (split kernel: 2, 4 or 8)");
   if (o->keys_per_wave > 4) return fail(MG_ERR_INVALID_ARG, "keys_per_wave must be 0-4");
   if (o->lane_depth > 16) return fail(MG_ERR_INVALID_ARG, "lane_depth must be 0-16");
+  if (o->mask_depth == 1 || o->mask_depth > 8)
+    return fail(MG_ERR_INVALID_ARG, "mask_depth must be 0 (tuned) or 2-8");
 
   uint32_t kernel = o->kernel;
   if (kernel > MG_KERNEL_STREAM_SPLIT)
@@ -445,12 +451,16 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     if (rc0) return rc0;
     kl_default = lane_shape.kl;
   } else if (kernel == MG_KERNEL_STREAM_SPLIT) {
-    // Split kernel, same matrix: one or two keys per group run best at kl = 7 with one pair
-    // per block (1 key: 11.39 ms; 11.69 ms at kl = 6), three and four at kl = 6 with four pairs
-    // per block (12.33 and 13.66 ms; four keys at kl = 5: 14.15 ms).
-    const int g = o->keys_per_wave ? (int)o->keys_per_wave
-                                   : default_keys_per_wave(kernel, split_s(pl), n_keys);
-    kl_default = g <= 2 ? 7 : 6;
+    // Split kernel (kernel time, one box, the default mask ring): rows of up to 64 chunks
+    // want deep lane subtrees, fewer as a group's keys add stack LDS: 78,643,200 x 1 KiB,
+    // 1 key 11.07 / 11.15 / 11.27 ms at kl = 9 / 8 / 7; 2 keys 11.29 / 11.43 ms at kl = 8 / 7;
+    // 3 keys 11.46 / 11.54 / 11.88 ms at kl = 8 / 9 / 6; 4 keys 13.06 / 13.09 ms at kl = 7 / 6.
+    // Wider rows stream best at kl = 7 (19,660,800 x 4 KiB, 1 key 12.28 against 12.38 ms at
+    // kl = 8; 3 keys 12.27 against 12.30 / 12.37 ms at kl = 6 / 8; 39,321,600 x 2 KiB, 3 keys
+    // 12.44 against 12.56 / 12.58 ms at kl = 6 / 8).
+    const int S = split_s(pl);
+    const int g = o->keys_per_wave ? (int)o->keys_per_wave : default_keys_per_wave(kernel, S, n_keys);
+    kl_default = S > 1 ? 7 : (g <= 2 ? 9 : (g == 3 ? 8 : 7));
     if (d > 6) kl_default = std::min(kl_default, d - 6);
   } else {
     const int g = o->keys_per_wave ? (int)o->keys_per_wave
@@ -531,13 +541,22 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
       if (wpb && o->kernel != MG_KERNEL_AUTO)
         return fail(MG_ERR_INVALID_ARG, "split kernel blocks hold 2, 4 or 8 waves (got %u)", wpb);
       wpb = (G >= 3 || groups * pl->col_splits >= 2) &&
-                    split_lds_bytes(G, pl->kl, 4) <= dev->max_lds
+                    split_lds_bytes(G, pl->kl, 4, 2) <= dev->max_lds
                 ? 8
                 : 2;
     }
     pl->waves_per_block = wpb;
     const int NP = (int)wpb / 2;
-    const size_t lds = split_lds_bytes(G, pl->kl, NP);
+    // Mask ring: as many steps as the block's LDS holds, up to kSplitDefaultDepth, so a DFS
+    // wave can run ahead through the cheap steps of a tile and cover the steps that descend
+    // a whole subtree.
+    uint32_t depth = o->mask_depth;
+    if (!depth) {
+      depth = kSplitDefaultDepth;
+      while (depth > 2 && split_lds_bytes(G, pl->kl, NP, depth) > dev->max_lds) depth--;
+    }
+    pl->mask_depth = depth;
+    const size_t lds = split_lds_bytes(G, pl->kl, NP, depth);
     if (lds > dev->max_lds)
       return fail(MG_ERR_INVALID_ARG, "split plan needs %zu B of LDS per block (max %zu)", lds,
                   dev->max_lds);
@@ -554,13 +573,11 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   if (!parts && kernel == MG_KERNEL_LANE_DFS && lane_shape.tiles_per_wave)
     parts = (n_tiles + lane_shape.tiles_per_wave - 1) / lane_shape.tiles_per_wave;
   if (!parts) parts = std::max<uint64_t>(1, (uint64_t)resident / (groups * pl->col_splits));
-  if (kernel == MG_KERNEL_STREAM_SPLIT && pl->waves_per_block == 2) {
-    // Single-pair blocks take equal runs of leaf-pair steps, which may start and end inside a
-    // tile, so every resident pair gets work (78,643,200 x 1 KiB at kl 7, 1 key: 9600 tiles are
-    // 13 per pair on 739 of 768 pairs, 800 steps on all 768; 11.4-11.7 against 11.7-12.0 ms).
-    // The pairs of a larger block meet at a barrier every step and a step's work depends on
-    // its place in the tile, so they keep whole tiles, in phase (3 keys: 13.4-13.5 ms against
-    // 14.4 ms with runs out of phase).
+  if (kernel == MG_KERNEL_STREAM_SPLIT) {
+    // Pairs take equal runs of leaf-pair steps, which may start and end inside a tile, so
+    // every resident pair gets work (78,643,200 x 1 KiB at kl 7, 1 key: 9600 tiles are 13 per
+    // pair on 739 of 768 pairs, 800 steps on all 768; 11.4-11.7 against 11.7-12.0 ms). The
+    // pairs of a block never wait for each other, so runs out of phase cost nothing.
     const uint64_t n_pairs = 1ull << (pl->kl - 1);
     const uint64_t total_steps = n_tiles * n_pairs;
     parts = std::min<uint64_t>(parts, total_steps ? total_steps : 1);
@@ -576,7 +593,6 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     pl->tiles_per_wave = (n_tiles + parts - 1) / parts;
     if (pl->tiles_per_wave == 0) pl->tiles_per_wave = 1;
     pl->parts = (uint32_t)((n_tiles + pl->tiles_per_wave - 1) / pl->tiles_per_wave);
-    if (kernel == MG_KERNEL_STREAM_SPLIT) pl->steps_per_wave = pl->tiles_per_wave << (pl->kl - 1);
   }
   if (pl->parts == 0) pl->parts = 1;
   if ((uint64_t)pl->parts * groups > (1ull << 30))
@@ -628,6 +644,8 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       p.zero_rows = (const uint4*)dev->zero_rows.p;
       if (pl.kernel == MG_KERNEL_STREAM_SPLIT) {
         p.col_splits = pl.col_splits;
+        p.mask_depth = pl.mask_depth;
+        p.fault = (uint32_t*)dev->fault.p;
         MG_HIP(launch_split(split_s(&pl), (int)pl.keys_per_wave,
                             (int)pl.waves_per_block / 2, p, s));
       } else {
@@ -698,6 +716,17 @@ int attach_delta(mg_device* dev, const mg_matrix* m, const mg_scan_opts* opts, P
   pl->delta_rows = dl->d_diffs;
   pl->delta_parts = (uint32_t)((dl->n + 63) / 64);
   return MG_OK;
+}
+
+int check_fault(mg_device* dev, const Plan& pl) {
+  if (pl.kernel != MG_KERNEL_STREAM_SPLIT) return MG_OK;
+  uint32_t word = 0;
+  MG_HIP(hipMemcpy(&word, dev->fault.p, sizeof word, hipMemcpyDeviceToHost));
+  if (!word) return MG_OK;
+  const uint32_t zero = 0;
+  MG_HIP(hipMemcpy(dev->fault.p, &zero, sizeof zero, hipMemcpyHostToDevice));
+  return fail(MG_ERR_DEVICE, "split kernel: a DFS/streaming hand-over wait ran out; answers "
+                             "of this call are void");
 }
 
 int fill_timing(mg_device* dev, const Plan& pl, double t0, mg_timing* t) {
@@ -808,6 +837,7 @@ int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8
   MG_HIP(hipMemcpyAsync(dev->h_out.p, dev->out.p, out_bytes, hipMemcpyDeviceToHost, s));
   MG_HIP(hipEventRecord(dev->ev[4], s));
   MG_HIP(hipStreamSynchronize(s));
+  if ((rc = mg::check_fault(dev, pl))) return rc;
   memcpy(out, dev->h_out.p, out_bytes);
   if (t) return mg::fill_timing(dev, pl, t0, t);
   return MG_OK;
@@ -861,7 +891,9 @@ int mg_device_open(int ordinal, mg_device** out) {
     }
   }
   int rc = dev->zero_rows.reserve(4096);
+  if (!rc) rc = dev->fault.reserve(16);
   if (!rc && (mg::launch_zero((uint4*)dev->zero_rows.p, 4096 / 16, dev->stream) != hipSuccess ||
+              mg::launch_zero((uint4*)dev->fault.p, 1, dev->stream) != hipSuccess ||
               hipStreamSynchronize(dev->stream) != hipSuccess))
     rc = fail(MG_ERR_DEVICE, "zeroing the zero-row buffer failed");
   if (!rc) rc = mg::noalloc_setup(dev);
@@ -886,6 +918,7 @@ int mg_device_close(mg_device* dev) {
   dev->out.release();
   dev->gstack.release();
   dev->zero_rows.release();
+  dev->fault.release();
   dev->h_keys.release();
   dev->h_out.release();
   for (auto& ev : dev->ev)

@@ -299,6 +299,7 @@ int sweep_step(mg_server* s, std::vector<uint32_t>* finished) {
     // The slot list region is reused by the next base's launch only after
     // this one's kernels have read it.
     MG_HIP(hipStreamSynchronize(st));
+    if ((rc = mg::check_fault(s->dev, pl))) return rc;
   }
   for (uint32_t i = 0; i < s->cap_keys; i++) {
     Slot& sl = s->slots[i];

@@ -147,6 +147,13 @@ def test_stream_kernel_matches_oracle(mg, scanner, n, w, d):
     m.close()
 
 
+def split_pair_bytes(g, kl):
+    """LDS of one split-kernel pair at the smallest mask ring: g stacks of the kl - 1
+    DFS levels less the three held in registers (64 x 16 B each), two 128-mask buffers
+    per key and 16 B of hand-over counters. A block gets 160 KiB of LDS."""
+    return (g * max(0, kl - 4) * 64 + 256 * g + 1) * 16
+
+
 @pytest.mark.parametrize("n,w,d", [(3000, 1024, 12), (5000, 2048, 13), (777, 4096, 10),
                                    (2500, 272, 12), (600, 32, 10), (70_001, 1024, 17),
                                    (3001, 3088, 12), (9000, 4096, 14)])
@@ -161,20 +168,31 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
     max_g = 4 if w <= 2048 else 3
     # 2, 4, 8 waves per block = 1, 2, 4 producer/consumer pairs; pairs past the
     # work and pairs with fewer tiles than the block's others run empty steps.
+    # Lane depths 1-5 cover DFS stacks of no slot, one to three register slots, and
+    # three register slots around one LDS level.
     for g in range(1, max_g + 1):
-        for kl in sorted({1, 3, d - 6}):
+        for kl in sorted(k for k in {1, 2, 3, 4, 5, d - 6} if k <= d - 6):
             for wpb in (0, 4, 8):
                 opts = mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=kl,
                                     keys_per_wave=g, waves_per_block=wpb)
-                # Per pair: g stacks of kl - 1 levels (64 x 16 B each) and two
-                # 128-mask buffers per key; a block gets 160 KiB of LDS.
-                if (wpb or 2) // 2 * (g * (kl - 1) * 64 + 256 * g) * 16 > 160 * 1024:
+                if (wpb or 2) // 2 * split_pair_bytes(g, kl) > 160 * 1024:
                     with pytest.raises(mg.MgError):
                         scanner.scan_keys(m, keys, opts)
                     continue
                 got, t = scanner.scan_keys(m, keys, opts, with_timing=True)
                 assert t["kernel"] == "stream_split"
                 assert np.array_equal(got, ref), (g, kl, wpb)
+    # Mask rings of 2 to 8 leaf-pair steps (a DFS wave runs up to depth - 1 steps ahead).
+    for depth in (2, 3, 5, 8):
+        opts = mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=d - 6, mask_depth=depth)
+        try:
+            got = scanner.scan_keys(m, keys, opts)
+        except mg.MgError as e:
+            assert "LDS" in str(e), e  # a deep ring of many keys can exceed a block's LDS
+            continue
+        assert np.array_equal(got, ref), depth
+    with pytest.raises(mg.MgError):
+        scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, mask_depth=1))
     # Uneven parts: a waves-per-key count that leaves the last part short.
     for wpb in (2, 8):
         got = scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=1,
@@ -187,7 +205,7 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
         for kl in sorted({3, d - 6}):
             for wpk in (3, 7, 1000):
                 for wpb in (2, 8):
-                    if wpb // 2 * (g * (kl - 1) * 64 + 256 * g) * 16 > 160 * 1024:
+                    if wpb // 2 * split_pair_bytes(g, kl) > 160 * 1024:
                         continue
                     opts = mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=kl,
                                         keys_per_wave=g, waves_per_key=wpk, waves_per_block=wpb)
@@ -205,7 +223,7 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
                         with pytest.raises(mg.MgError):
                             scanner.scan_keys(m, keys, opts)
                         continue
-                    if (wpb or 2) // 2 * (g * (kl - 1) * 64 + 256 * g) * 16 > 160 * 1024:
+                    if (wpb or 2) // 2 * split_pair_bytes(g, kl) > 160 * 1024:
                         continue
                     got, t = scanner.scan_keys(m, keys, opts, with_timing=True)
                     assert np.array_equal(got, ref), (g, kl, wpb, wpk)
