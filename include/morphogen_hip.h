@@ -305,13 +305,25 @@ typedef struct {
   uint32_t segments;          /* sweep steps per full pass; 0 = one per 2^25
                                  rows, between 1 and 64                     */
   uint32_t lane_depth;        /* kl of the fused kernel; 0 = 10 for rows of
-                                 at most 128 B, else 6                      */
+                                 at most 128 B, else 7                      */
   uint32_t max_active_keys;   /* 0 = bounded by free HBM (at most 32768)    */
   uint32_t domain_bits;       /* key domain served; 0 = ceil(log2 n_rows)   */
   uint64_t hbm_reserve_bytes; /* HBM left free for other allocations       */
   uint64_t max_delta_entries; /* update rows kept before a rebase; 0 = 2^20 */
   uint32_t double_buffer;     /* keep a second matrix copy so a rebase never
                                  waits for in-flight keys                   */
+  uint32_t deadline_ms;       /* admission deadline: mg_server_submit refuses
+                                 (MG_ERR_BUSY) a request that, by the loop's
+                                 step-time model (fixed + per key, fitted to
+                                 the steps run) and the requests queued and in
+                                 flight, would finish later than this after
+                                 its submit, or would make an earlier request
+                                 miss its own; 0 = none (the reference answers
+                                 such requests with 503, network/api/mod.rs:
+                                 638, 828)                                  */
+  uint32_t use_graph;         /* replay each sweep step's kernels (scan and
+                                 reductions into the accumulators) from a HIP
+                                 graph cached per segment                   */
 } mg_server_opts;
 
 typedef struct {
@@ -330,6 +342,13 @@ typedef struct {
   double busy_ms;             /* time spent in sweep steps                 */
   uint64_t snapshots;         /* matrices swapped in by mg_server_snapshot */
   uint64_t n_rows;            /* rows of the matrix served now             */
+  uint64_t rejected;          /* submits refused by the admission deadline */
+  uint64_t abandoned;         /* requests given up before completing: while
+                                 queued (never run) or in flight (their keys
+                                 leave the sweep at the next step)          */
+  double key_step_ms;         /* fitted sweep-step time per key, and ...   */
+  double step_fixed_ms;       /* ... per step: the admission model         */
+  uint64_t graph_launches;    /* sweep-step launches replayed from a graph */
 } mg_server_stats;
 
 int mg_server_create(mg_device *dev, mg_matrix *m, const mg_server_opts *opts,
@@ -342,11 +361,16 @@ int mg_server_submit(mg_server *s, const uint8_t *keys, const size_t *key_lens,
  * MG_ERR_BUSY on timeout (the request stays queued). */
 int mg_server_wait(mg_server *s, uint64_t ticket, uint8_t *out,
                    uint32_t timeout_ms, mg_request_info *info);
-/* Gives up a ticket nobody will wait for (e.g. after a timed-out wait): the
- * request still runs, and its answers are dropped when it completes. */
+/* Gives up a ticket nobody will wait for (e.g. after a timed-out wait). A
+ * request still queued is removed and never runs; one in flight leaves the
+ * sweep at the next step, freeing its key slots; a finished one is dropped. */
 int mg_server_abandon(mg_server *s, uint64_t ticket);
 /* Epoch update: rows[row_idx[i]] ^= diffs[i] for every request admitted
- * after this call (EpochManager::try_advance, epoch.rs:465-549). */
+ * after this call (EpochManager::try_advance, epoch.rs:465-549). Updates are
+ * applied at the next sweep step, ahead of requests still waiting for key
+ * slots (as the reference's merge worker advances epochs whatever queries
+ * wait, epoch.rs:465-549, 685); those then answer at the newer epoch, which
+ * mg_request_info reports. Keys already admitted keep their epoch. */
 int mg_server_update(mg_server *s, const uint64_t *row_idx,
                      const uint8_t *diffs, uint64_t n_entries, uint64_t epoch);
 /* Snapshot hot-swap (EpochManager::submit_snapshot, crates/morphogen-server/
@@ -359,6 +383,11 @@ int mg_server_update(mg_server *s, const uint64_t *row_idx,
  * loop owns the matrix it serves: merges write into it. */
 int mg_server_snapshot(mg_server *s, mg_matrix *m, uint64_t epoch);
 int mg_server_stats_get(mg_server *s, mg_server_stats *out);
+/* Fault injection for tests of the failure path (no reference counterpart):
+ * the loop's next turn fails with `code` (an MG_ERR_* value) as a device
+ * error would: waiting requests and queued snapshots get the error, later
+ * calls get MG_ERR_* "server stopped", and mg_server_destroy still works. */
+int mg_server_inject_failure(mg_server *s, int code);
 int mg_server_destroy(mg_server *s);
 
 /* ---------------------------------------------------------------------------

@@ -104,7 +104,8 @@ class ServerOpts(ctypes.Structure):
     _fields_ = [("segments", ctypes.c_uint32), ("lane_depth", ctypes.c_uint32),
                 ("max_active_keys", ctypes.c_uint32), ("domain_bits", ctypes.c_uint32),
                 ("hbm_reserve_bytes", ctypes.c_uint64), ("max_delta_entries", ctypes.c_uint64),
-                ("double_buffer", ctypes.c_uint32)]
+                ("double_buffer", ctypes.c_uint32), ("deadline_ms", ctypes.c_uint32),
+                ("use_graph", ctypes.c_uint32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -123,7 +124,10 @@ class ServerStats(ctypes.Structure):
                 ("rebases", ctypes.c_uint64), ("active_keys", ctypes.c_uint32),
                 ("queued_requests", ctypes.c_uint32), ("max_active_keys", ctypes.c_uint32),
                 ("segments", ctypes.c_uint32), ("busy_ms", ctypes.c_double),
-                ("snapshots", ctypes.c_uint64), ("n_rows", ctypes.c_uint64)]
+                ("snapshots", ctypes.c_uint64), ("n_rows", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("abandoned", ctypes.c_uint64),
+                ("key_step_ms", ctypes.c_double), ("step_fixed_ms", ctypes.c_double),
+                ("graph_launches", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -185,6 +189,7 @@ def _load():
         "mg_server_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
         "mg_server_snapshot": (ctypes.c_int, [vp, vp, ctypes.c_uint64]),
         "mg_server_abandon": (ctypes.c_int, [vp, ctypes.c_uint64]),
+        "mg_server_inject_failure": (ctypes.c_int, [vp, ctypes.c_int]),
         "mg_server_destroy": (ctypes.c_int, [vp]),
         "mg_cuckoo_hash_indices": (ctypes.c_int, [u8p, ctypes.c_size_t, ctypes.c_uint64, u64p,
                                                   u64p]),
@@ -572,9 +577,10 @@ class GpuServer:
 
     def __init__(self, scanner: GpuScanner, matrix: GpuPageMatrix, segments=0, lane_depth=0,
                  max_active_keys=0, domain_bits=0, hbm_reserve_bytes=0, max_delta_entries=0,
-                 double_buffer=False):
+                 double_buffer=False, deadline_ms=0, use_graph=False):
         o = ServerOpts(segments, lane_depth, max_active_keys, domain_bits, hbm_reserve_bytes,
-                       max_delta_entries, 1 if double_buffer else 0)
+                       max_delta_entries, 1 if double_buffer else 0, deadline_ms,
+                       1 if use_graph else 0)
         h = ctypes.c_void_p()
         _check(lib.mg_server_create(scanner._h, matrix._h, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -596,7 +602,8 @@ class GpuServer:
         return out[:n_keys], info.as_dict()
 
     def abandon(self, ticket: int):
-        """Gives up a ticket: the request still runs; its answers are dropped when done."""
+        """Gives up a ticket: a queued request never runs, one in flight leaves the sweep at
+        the next step, a finished one is dropped."""
         _check(lib.mg_server_abandon(self._h, ticket))
 
     def update(self, row_idx: np.ndarray, diffs: np.ndarray, epoch: int):
@@ -604,6 +611,10 @@ class GpuServer:
         dif = np.ascontiguousarray(diffs, dtype=np.uint8).reshape(len(idx), self.row_bytes)
         _check(lib.mg_server_update(self._h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                     _u8(dif), len(idx), epoch))
+
+    def inject_failure(self, code: int):
+        """Makes the loop's next turn fail with `code` (tests of the failure path)."""
+        _check(lib.mg_server_inject_failure(self._h, code))
 
     def stats(self):
         st = ServerStats()

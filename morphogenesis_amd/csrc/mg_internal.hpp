@@ -76,6 +76,7 @@ struct Plan {
   const void* ws_partial;
   const void* ws_partial_delta;
   const void* ws_gstack;
+  const void* ws_key_idx;  // request loop: the launch's slot list
   Plan();
   bool operator==(const Plan& o) const;
 };

@@ -10,10 +10,19 @@
 //    keys join at the next step and finish after S steps. The XOR fold is
 //    order-free, so each answer equals a whole-matrix scan.
 //  - Updates (epochs) append to a cumulative delta over the current base
-//    rows. A key's accumulator starts at the delta contribution of every
-//    update admitted before it (scan_delta_for_gpu, scan.rs:434-485), so its
-//    answer is the scan of base XOR updates at admission, however the sweep
-//    and later updates interleave with it.
+//    rows, at the next sweep step, ahead of requests still waiting for slots
+//    (the reference's merge worker advances epochs whatever queries wait,
+//    epoch.rs:465-549, 685). A key's accumulator starts at the delta
+//    contribution of every update applied before its admission
+//    (scan_delta_for_gpu, scan.rs:434-485), so its answer is the scan of base
+//    XOR updates at admission, however the sweep and later updates interleave
+//    with it; the request reports that epoch.
+//  - Overload: with a deadline, a request whose predicted completion (work
+//    queued and in flight ahead of it, times the measured GPU time per key
+//    and sweep step) exceeds it is refused at submit (MG_ERR_BUSY, the
+//    reference's 503, network/api/mod.rs:638, 828). An abandoned request
+//    leaves the queue, or the sweep at the next step, so a client that gave
+//    up costs no more GPU time.
 //  - When the cumulative delta grows past half its capacity the loop merges it
 //    into a base copy: into the spare copy when double-buffered (in-flight
 //    keys keep the old base), otherwise in place once in-flight keys drain.
@@ -46,7 +55,13 @@ struct Request {
   int status = MG_OK;
   std::string error;
   bool done = false;
+  bool admitted = false;
   bool abandoned = false;  // nobody will collect it: dropped once done
+  bool cancel = false;     // abandoned in flight: its keys leave at the next step
+  uint32_t keys_left = 0;  // keys still in slots (admitted requests)
+  uint32_t steps_done = 0; // sweep steps its keys have run (they run in lockstep)
+  double due = 0;          // admission deadline, absolute ms (0 = none)
+  int waiting = 0;         // threads in mg_server_wait for it
 };
 
 struct Update {
@@ -91,6 +106,8 @@ struct mg_server {
   uint32_t seg_want = 0;       // segments option (0 = one per 2^25 rows)
   uint32_t cap_keys = 0;
   bool double_buffer = false;
+  uint32_t deadline_ms = 0;
+  bool use_graph = false;
 
   // Base row copies: 0 is the caller's matrix, 1 the spare (double buffer).
   uint4* base_rows[2] = {nullptr, nullptr};
@@ -104,6 +121,7 @@ struct mg_server {
   uint64_t* d_cidx = nullptr;
   uint4* d_cdiffs = nullptr;
   uint64_t rebases = 0;
+  uint64_t graph_launches = 0;  // loop thread only
 
   // Key slots, device-resident keys and accumulators.
   std::vector<Slot> slots;
@@ -117,6 +135,14 @@ struct mg_server {
   mg::DevBuf gstack;  // lane-DFS stack levels in global memory
   mg::HostBuf h_idx, h_keys, h_acc;
   uint32_t step = 0;
+  // Sweep-step graphs, one per (segment, base copy), each for the plan it
+  // was captured with.
+  struct StepGraph {
+    bool valid = false;
+    mg::Plan plan;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<StepGraph> graphs;
 
   // Queue, shared with the API threads under `mu`.
   std::mutex mu;
@@ -125,6 +151,19 @@ struct mg_server {
   std::map<uint64_t, std::unique_ptr<Request>> requests;
   uint64_t next_ticket = 1;
   uint64_t epoch = 0;
+  // Admission model (under mu): requests in flight, and a step's time as
+  // step_fixed_ms + key_step_ms x active keys, fitted by exponentially
+  // weighted least squares over the steps run so far.
+  std::vector<Request*> inflight;
+  uint64_t queued_keys = 0;
+  double fit_w = 0, fit_n = 0, fit_t = 0, fit_nn = 0, fit_nt = 0;
+  double key_step_ms = 0, step_fixed_ms = 0;  // the model in use
+  double prior_key_ms = 0, prior_fixed_ms = 0;  // before enough steps are timed
+  // Threads inside mg_server_wait / mg_server_snapshot (under mu): destroy
+  // frees the server only once they have left.
+  int waiters = 0;
+  std::condition_variable cv_gone;
+  int inject = MG_OK;  // mg_server_inject_failure: the next turn fails with this code
   bool stop = false;
   int fatal = MG_OK;
   std::string fatal_msg;
@@ -134,7 +173,15 @@ struct mg_server {
 
 namespace {
 
+void drop_graphs(mg_server* s) {
+  for (auto& g : s->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = mg_server::StepGraph{};
+  }
+}
+
 void free_server_memory(mg_server* s) {
+  drop_graphs(s);
   (void)hipFree(s->d_cidx);
   (void)hipFree(s->d_cdiffs);
   (void)hipFree(s->d_keys);
@@ -295,7 +342,39 @@ int sweep_step(mg_server* s, std::vector<uint32_t>* finished) {
     bufs.out = s->d_acc;
     bufs.out_idx = s->d_idx + start;
     bufs.accumulate = true;
-    if ((rc = mg::enqueue_scan(s->dev, pl, bufs, nullptr, nullptr))) return rc;
+    if (s->use_graph) {
+      // The plan with the buffers it reads and writes is the graph's signature.
+      pl.ws_keys = bufs.keys;
+      pl.ws_out = bufs.out;
+      pl.ws_partial = bufs.partial;
+      pl.ws_gstack = bufs.gstack;
+      pl.ws_key_idx = bufs.key_idx;
+      mg_server::StepGraph& g = s->graphs[(size_t)s->step * 2 + b];
+      if (!(g.valid && g.plan == pl)) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = mg_server::StepGraph{};
+        MG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rc = mg::enqueue_scan(s->dev, pl, bufs, nullptr, nullptr);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc || e != hipSuccess) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return rc ? rc : mg::hip_fail(e, "hipStreamEndCapture(sweep step)");
+        }
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+          g.exec = nullptr;
+          return mg::hip_fail(e, "hipGraphInstantiate(sweep step)");
+        }
+        g.plan = pl;
+        g.valid = true;
+      }
+      MG_HIP(hipGraphLaunch(g.exec, st));
+      s->graph_launches++;
+    } else if ((rc = mg::enqueue_scan(s->dev, pl, bufs, nullptr, nullptr))) {
+      return rc;
+    }
     // The slot list region is reused by the next base's launch only after
     // this one's kernels have read it.
     MG_HIP(hipStreamSynchronize(st));
@@ -330,12 +409,18 @@ void fail_all(mg_server* s, int code, const std::string& msg) {
       r->error = msg;
     }
   }
-  for (Event& ev : s->queue)
-    if (ev.snap && !ev.snap->done) {
-      ev.snap->done = true;
-      ev.snap->status = code;
-      ev.snap->error = msg;
+  // A queued snapshot's record lives on its caller's stack: complete it and
+  // drop the event, so nothing touches the record once the caller returns.
+  for (auto it = s->queue.begin(); it != s->queue.end();) {
+    if (it->snap) {
+      it->snap->done = true;
+      it->snap->status = code;
+      it->snap->error = msg;
+      it = s->queue.erase(it);
+    } else {
+      ++it;
     }
+  }
   s->cv_done.notify_all();
 }
 
@@ -354,6 +439,8 @@ void plan_segments(mg_server* s) {
   seg = std::max<uint64_t>(tile, (seg + tile - 1) / tile * tile);
   s->seg_rows = seg;
   s->segments = (uint32_t)((s->n_rows + seg - 1) / seg);
+  drop_graphs(s);
+  s->graphs.resize((size_t)s->segments * 2);
 }
 
 // Serves `m` from now on. Caller holds dev->mu; no key is in flight.
@@ -383,67 +470,197 @@ void apply_snapshot(mg_server* s, const Snapshot& sn) {
   s->rebase_wanted = false;
 }
 
+// Frees the slots of requests abandoned in flight (their answers would be
+// dropped anyway). Caller holds mu; the loop thread owns the slots.
+void release_cancelled(mg_server* s) {
+  for (uint32_t i = 0; i < s->cap_keys; i++) {
+    Slot& sl = s->slots[i];
+    if (!sl.req || !sl.req->cancel) continue;
+    Request* r = sl.req;
+    sl = Slot{};
+    s->free_slots.push_back(i);
+    s->active--;
+    if (--r->keys_left == 0) {
+      s->inflight.erase(std::find(s->inflight.begin(), s->inflight.end(), r));
+      s->requests.erase(r->ticket);  // `r` is gone from here on
+    }
+  }
+}
+
+// Steps timed before the fitted model replaces the prior.
+constexpr double kFitMinSteps = 4;
+
+// Starts the step-time model afresh for the current segments, from a prior:
+// per key, a segment's GGM nodes and folds (700 + W/4 int32 ops per row) at
+// half the 78.6 TOP/s VALU peak; per step, the segment's bytes at 6.5 TB/s
+// plus 0.1 ms of launches and host round trips. (2^31 x 32 B in 64 steps:
+// 0.60 ms per key against 0.55 ms measured.) Caller holds mu, or owns `s`.
+void reset_model(mg_server* s) {
+  s->fit_w = s->fit_n = s->fit_t = s->fit_nn = s->fit_nt = 0;
+  const double rows = (double)s->seg_rows;
+  s->prior_key_ms = rows * (700.0 + s->row_bytes / 4.0) / 39.3e12 * 1e3;
+  s->prior_fixed_ms = rows * s->row_bytes / 6.5e12 * 1e3 + 0.1;
+  s->key_step_ms = s->prior_key_ms;
+  s->step_fixed_ms = s->prior_fixed_ms;
+}
+
+// Refits step time = fixed + per-key x keys after a step of n keys that took t ms.
+void fit_step(mg_server* s, double n, double t) {
+  const double keep = 0.97;  // about the last 30 steps weigh in
+  s->fit_w = keep * s->fit_w + 1;
+  s->fit_n = keep * s->fit_n + n;
+  s->fit_t = keep * s->fit_t + t;
+  s->fit_nn = keep * s->fit_nn + n * n;
+  s->fit_nt = keep * s->fit_nt + n * t;
+  const double den = s->fit_w * s->fit_nn - s->fit_n * s->fit_n;
+  double c = 0, a = 0;
+  if (den > 1e-6 * s->fit_w * s->fit_nn) {
+    c = (s->fit_w * s->fit_nt - s->fit_n * s->fit_t) / den;
+    a = (s->fit_t - c * s->fit_n) / s->fit_w;
+  }
+  if (c <= 0 || a < 0) {  // too little spread in the key counts yet: all per key
+    c = s->fit_t / s->fit_n;
+    a = 0;
+  }
+  if (s->fit_w < kFitMinSteps) return;  // the prior stays in use
+  s->key_step_ms = c;
+  s->step_fixed_ms = a;
+}
+
+// Whether a request of n_keys submitted now, due at now + deadline_ms, can be
+// admitted: with it, it and every request queued or in flight that would
+// otherwise finish in time still finish in time. Requests queued are taken
+// as admitted now (FIFO, ahead of it); each step costs fixed + per-key x the
+// keys it serves. Caller holds mu.
+bool admissible(mg_server* s, uint32_t n_keys, double now, double* due_out) {
+  struct Job {
+    uint32_t steps_left, keys;
+    double due;
+  };
+  std::vector<Job> jobs;
+  jobs.reserve(s->inflight.size() + s->queue.size() + 1);
+  for (const Request* r : s->inflight)
+    if (!r->cancel) jobs.push_back({s->segments - std::min(r->steps_done, s->segments), r->n_keys,
+                                    r->due});
+  for (const Event& ev : s->queue)
+    if (ev.req) jobs.push_back({s->segments, ev.req->n_keys, ev.req->due});
+  // Keys served by step j (1-based) of the schedule, without the new request.
+  std::vector<double> keys_at(s->segments + 2, 0.0);
+  for (const Job& j : jobs) keys_at[j.steps_left] += j.keys;
+  for (int j = (int)s->segments - 1; j >= 1; --j) keys_at[j] += keys_at[j + 1];
+  // Finish times without (t0) and with (t1) the new request's keys in steps 1..S.
+  std::vector<double> t0(s->segments + 1, now), t1(s->segments + 1, now);
+  for (uint32_t j = 1; j <= s->segments; ++j) {
+    t0[j] = t0[j - 1] + s->step_fixed_ms + s->key_step_ms * keys_at[j];
+    t1[j] = t1[j - 1] + s->step_fixed_ms + s->key_step_ms * (keys_at[j] + n_keys);
+  }
+  const double due = now + s->deadline_ms;
+  *due_out = t1[s->segments];
+  if (t1[s->segments] > due) return false;
+  for (const Job& j : jobs)
+    if (j.due > 0 && t1[j.steps_left] > j.due && t0[j.steps_left] <= j.due) return false;
+  return true;
+}
+
 void server_loop(mg_server* s) {
   (void)hipSetDevice(s->dev->ordinal);
-  for (;;) {
-    std::vector<std::unique_ptr<Update>> ups_before;  // applied before `admit`
+  // One turn's work, in submission order: groups of requests to admit and
+  // updates between them.
+  struct Item {
     std::vector<Request*> admit;
+    std::unique_ptr<Update> upd;
+  };
+  for (;;) {
+    std::vector<Item> items;
     Snapshot* snap = nullptr;  // taken only when no key is in flight
     bool snap_applied = false;  // once set, `snap` may already be gone
-    std::unique_ptr<Update> upd_after;  // an update queued behind admitted requests
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] {
-        return s->stop || !s->queue.empty() || s->active > 0 || s->rebase_wanted;
+        return s->stop || !s->queue.empty() || s->active > 0 || s->rebase_wanted ||
+               s->inject != MG_OK;
       });
       if (s->stop) break;
-      // Take events in submission order: updates, then the requests that fit.
-      // An update behind admitted requests waits for the next turn so those
-      // requests start from the state before it.
+      if (s->inject != MG_OK) {
+        const int code = s->inject;
+        s->inject = MG_OK;
+        lk.unlock();
+        fail_all(s, code, "injected failure");
+        break;
+      }
+      release_cancelled(s);
+      // Events up to the first snapshot (which needs every key before it
+      // done), in order: requests are admitted FIFO while their keys fit; once
+      // one does not, later requests wait too, but updates behind it are still
+      // taken: an update waits for no request, and the requests it overtakes
+      // answer at its epoch. An update the delta has no room for (a rebase
+      // makes room first) holds back everything behind it.
+      // Single-buffered: a pending in-place merge stops admissions.
+      bool blocked = s->rebase_wanted && !s->double_buffer;
       uint32_t free_n = (uint32_t)s->free_slots.size();
       uint64_t c_n = s->c_idx.size();
-      while (!s->queue.empty()) {
-        Event& ev = s->queue.front();
-        if (ev.snap) {
-          // Requests admitted before it finish on the old rows first.
-          if (!admit.empty() || s->active > 0) break;
-          snap = ev.snap;
-          s->queue.pop_front();
-          break;
-        }
-        if (ev.upd) {
-          if (!admit.empty()) break;
-          if (c_n + ev.upd->idx.size() > s->c_cap) {
-            s->rebase_wanted = true;  // room is made by a rebase first
+      for (auto it = s->queue.begin(); it != s->queue.end();) {
+        if (it->snap) break;
+        if (it->upd) {
+          if (c_n + it->upd->idx.size() > s->c_cap) {
+            s->rebase_wanted = true;
             break;
           }
-          c_n += ev.upd->idx.size();
-          ups_before.push_back(std::move(ev.upd));
-          s->queue.pop_front();
-        } else {
-          // Single-buffered: a pending in-place merge stops admissions.
-          if (s->rebase_wanted && !s->double_buffer) break;
-          if (ev.req->n_keys > free_n) break;
-          free_n -= ev.req->n_keys;
-          admit.push_back(ev.req);
-          s->queue.pop_front();
+          c_n += it->upd->idx.size();
+          items.emplace_back();
+          items.back().upd = std::move(it->upd);
+          it = s->queue.erase(it);
+          continue;
         }
+        if (blocked || it->req->n_keys > free_n) {
+          blocked = true;
+          ++it;
+          continue;
+        }
+        free_n -= it->req->n_keys;
+        if (items.empty() || items.back().upd) items.emplace_back();
+        items.back().admit.push_back(it->req);
+        s->queued_keys -= it->req->n_keys;
+        it = s->queue.erase(it);
+      }
+      if (items.empty() && s->active == 0 && !s->queue.empty() && s->queue.front().snap) {
+        snap = s->queue.front().snap;
+        s->queue.pop_front();
       }
     }
     const double t0 = mg::now_ms();
     std::vector<uint32_t> finished;
     int rc = MG_OK;
+    uint32_t n_stepped = 0;
     {
       std::lock_guard<std::mutex> dlk(s->dev->mu);
       rc = mg::set_device(s->dev);
       if (!rc && !snap) rc = try_rebase(s);
-      for (auto& u : ups_before) {
+      for (Item& item : items) {
         if (rc) break;
-        rc = apply_update(s, *u);
-        if (!rc) {
-          std::lock_guard<std::mutex> lk(s->mu);
-          s->epoch = u->epoch;
+        if (item.upd) {
+          rc = apply_update(s, *item.upd);
+          if (!rc) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->epoch = item.upd->epoch;
+          }
+          continue;
         }
+        const double now = mg::now_ms();
+        uint32_t n_new = 0;
+        {
+          std::lock_guard<std::mutex> lk(s->mu);
+          for (Request* r : item.admit) {
+            r->t_admit = now;
+            r->epoch = s->epoch;
+            r->admitted = true;
+            r->keys_left = r->n_keys;
+            n_new += r->n_keys;
+            s->inflight.push_back(r);
+          }
+        }
+        rc = admit_keys(s, item.admit);
+        if (!rc) s->active += n_new;
       }
       if (!rc && snap) {
         apply_snapshot(s, *snap);
@@ -452,24 +669,12 @@ void server_loop(mg_server* s) {
         s->stats.segments = s->segments;
         s->stats.snapshots++;
         s->stats.n_rows = s->n_rows;
+        reset_model(s);
         snap->done = true;
         snap_applied = true;
         s->cv_done.notify_all();
       }
-      if (!rc && !admit.empty()) {
-        const double now = mg::now_ms();
-        uint32_t n_new = 0;
-        {
-          std::lock_guard<std::mutex> lk(s->mu);
-          for (Request* r : admit) {
-            r->t_admit = now;
-            r->epoch = s->epoch;
-            n_new += r->n_keys;
-          }
-        }
-        rc = admit_keys(s, admit);
-        if (!rc) s->active += n_new;
-      }
+      n_stepped = s->active;
       if (!rc && s->active) rc = sweep_step(s, &finished);
     }
     if (snap && !snap_applied) {  // the loop failed before the swap
@@ -487,19 +692,30 @@ void server_loop(mg_server* s) {
     {
       std::lock_guard<std::mutex> lk(s->mu);
       s->stats.busy_ms += t1 - t0;
-      if (s->active) s->stats.sweep_steps++;
+      if (n_stepped) {
+        s->stats.sweep_steps++;
+        for (Request* r : s->inflight) r->steps_done++;
+        // The admission model's step time (steps that also admitted, applied
+        // updates or merged count in full: such work delays requests too).
+        fit_step(s, n_stepped, t1 - t0);
+      }
       const uint8_t* h = (const uint8_t*)s->h_acc.p;
       for (size_t j = 0; j < finished.size(); j++) {
         Slot& sl = s->slots[finished[j]];
         Request* r = sl.req;
         memcpy(r->answers.data() + (size_t)sl.key * s->row_bytes, h + j * s->row_bytes,
                s->row_bytes);
-        if (++r->keys_done == r->n_keys) {
+        r->keys_done++;
+        if (--r->keys_left == 0)
+          s->inflight.erase(std::find(s->inflight.begin(), s->inflight.end(), r));
+        if (r->keys_done == r->n_keys) {
           r->done = true;
           r->t_done = t1;
           s->stats.completed++;
           s->stats.keys_completed += r->n_keys;
           if (r->abandoned) s->requests.erase(r->ticket);  // `r` is gone from here on
+        } else if (r->cancel && r->keys_left == 0) {
+          s->requests.erase(r->ticket);
         }
         sl = Slot{};
         s->free_slots.push_back(finished[j]);
@@ -508,6 +724,7 @@ void server_loop(mg_server* s) {
       s->stats.active_keys = s->active;
       s->stats.delta_entries = s->c_idx.size();
       s->stats.rebases = s->rebases;
+      s->stats.graph_launches = s->graph_launches;
       if (!finished.empty()) s->cv_done.notify_all();
     }
   }
@@ -544,10 +761,13 @@ int mg_server_create(mg_device* dev, mg_matrix* m, const mg_server_opts* opts, m
   s->d = d;
   s->n_rows = m->n_rows;
   s->double_buffer = o->double_buffer != 0;
-  s->kl = std::min<uint32_t>(o->lane_depth ? o->lane_depth : (s->chunks > 8 ? 6 : 10), d);
+  s->deadline_ms = o->deadline_ms;
+  s->use_graph = o->use_graph != 0;
+  s->kl = std::min<uint32_t>(o->lane_depth ? o->lane_depth : (s->chunks > 8 ? 7 : 10), d);
   s->seg_want = o->segments;
   s->queued_n_rows = s->n_rows;
   plan_segments(s.get());
+  reset_model(s.get());
   s->c_cap = o->max_delta_entries ? o->max_delta_entries : (1ull << 20);
 
   std::lock_guard<std::mutex> lk(dev->mu);
@@ -619,11 +839,22 @@ int mg_server_submit(mg_server* s, const uint8_t* keys, const size_t* key_lens, 
   r->t_submit = mg::now_ms();
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->fatal) return fail(s->fatal, "server stopped: %s", s->fatal_msg.c_str());
+  if (s->deadline_ms) {
+    r->due = r->t_submit + s->deadline_ms;
+    double finish = 0;
+    if (!admissible(s, n_keys, r->t_submit, &finish)) {
+      s->stats.rejected++;
+      return fail(MG_ERR_BUSY, "request of %u keys would finish in about %.0f ms, or delay "
+                               "requests admitted before it, past the %u ms deadline",
+                  n_keys, finish - r->t_submit, s->deadline_ms);
+    }
+  }
   r->ticket = s->next_ticket++;
   *ticket = r->ticket;
   Event ev;
   ev.req = r.get();
   s->queue.push_back(std::move(ev));
+  s->queued_keys += n_keys;
   s->requests[r->ticket] = std::move(r);
   s->stats.submitted++;
   s->cv_work.notify_one();
@@ -638,14 +869,24 @@ int mg_server_wait(mg_server* s, uint64_t ticket, uint8_t* out, uint32_t timeout
   if (it == s->requests.end()) return fail(MG_ERR_INVALID_ARG, "unknown ticket %llu",
                                            (unsigned long long)ticket);
   Request* r = it->second.get();
+  if (r->abandoned) return fail(MG_ERR_INVALID_ARG, "ticket %llu was abandoned",
+                                (unsigned long long)ticket);
+  if (r->waiting)
+    return fail(MG_ERR_STATE, "ticket %llu already has a thread waiting for it",
+                (unsigned long long)ticket);
   auto ready = [&] { return r->done; };
-  if (timeout_ms) {
-    if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
-      return fail(MG_ERR_BUSY, "request %llu not done after %u ms", (unsigned long long)ticket,
-                  timeout_ms);
-  } else {
+  s->waiters++;
+  r->waiting++;  // abandon() refuses the ticket meanwhile, so `r` stays valid
+  bool ok = true;
+  if (timeout_ms)
+    ok = s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+  else
     s->cv_done.wait(lk, ready);
-  }
+  r->waiting--;
+  if (--s->waiters == 0) s->cv_gone.notify_all();
+  if (!ok)
+    return fail(MG_ERR_BUSY, "request %llu not done after %u ms", (unsigned long long)ticket,
+                timeout_ms);
   if (r->status != MG_OK) {
     const int code = r->status;
     const std::string msg = r->error;
@@ -668,12 +909,30 @@ int mg_server_abandon(mg_server* s, uint64_t ticket) {
   if (!s) return fail(MG_ERR_INVALID_ARG, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);
   auto it = s->requests.find(ticket);
-  if (it == s->requests.end())
+  if (it == s->requests.end() || it->second->abandoned)
     return fail(MG_ERR_INVALID_ARG, "unknown ticket %llu", (unsigned long long)ticket);
-  if (it->second->done)
+  Request* r = it->second.get();
+  if (r->waiting)
+    return fail(MG_ERR_STATE, "ticket %llu has a thread waiting for it", (unsigned long long)ticket);
+  if (r->done) {
     s->requests.erase(it);
-  else
-    it->second->abandoned = true;
+    return MG_OK;
+  }
+  s->stats.abandoned++;
+  if (!r->admitted) {  // still queued: it never runs
+    for (auto q = s->queue.begin(); q != s->queue.end(); ++q)
+      if (q->req == r) {
+        s->queue.erase(q);
+        break;
+      }
+    s->queued_keys -= r->n_keys;
+    s->requests.erase(it);
+    return MG_OK;
+  }
+  // In flight: its keys leave the sweep at the loop's next step.
+  r->abandoned = true;
+  r->cancel = true;
+  s->cv_work.notify_one();
   return MG_OK;
 }
 
@@ -722,7 +981,9 @@ int mg_server_snapshot(mg_server* s, mg_matrix* m, uint64_t epoch) {
   s->queue.push_back(std::move(ev));
   s->queued_n_rows = m->n_rows;
   s->cv_work.notify_one();
+  s->waiters++;
   s->cv_done.wait(lk, [&] { return sn.done; });
+  if (--s->waiters == 0) s->cv_gone.notify_all();
   if (sn.status != MG_OK) {
     m->users--;
     return fail(sn.status, "snapshot failed: %s", sn.error.c_str());
@@ -735,6 +996,8 @@ int mg_server_stats_get(mg_server* s, mg_server_stats* out) {
   std::lock_guard<std::mutex> lk(s->mu);
   *out = s->stats;
   out->epoch = s->epoch;
+  out->key_step_ms = s->key_step_ms;
+  out->step_fixed_ms = s->step_fixed_ms;
   uint32_t q = 0;
   for (const Event& ev : s->queue)
     if (ev.req) q++;
@@ -742,10 +1005,18 @@ int mg_server_stats_get(mg_server* s, mg_server_stats* out) {
   return MG_OK;
 }
 
+int mg_server_inject_failure(mg_server* s, int code) {
+  if (!s || code == MG_OK) return fail(MG_ERR_INVALID_ARG, "null server or MG_OK code");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->inject = code;
+  s->cv_work.notify_one();
+  return MG_OK;
+}
+
 int mg_server_destroy(mg_server* s) {
   if (!s) return MG_OK;
   {
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     s->stop = true;
     for (auto& kv : s->requests)
       if (!kv.second->done) {
@@ -753,14 +1024,23 @@ int mg_server_destroy(mg_server* s) {
         kv.second->status = MG_ERR_STATE;
         kv.second->error = "server destroyed";
       }
-    for (Event& ev : s->queue)
-      if (ev.snap && !ev.snap->done) {
-        ev.snap->done = true;
-        ev.snap->status = MG_ERR_STATE;
-        ev.snap->error = "server destroyed";
+    // Queued snapshots: complete them and drop their events (the records live
+    // on their callers' stacks).
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+      if (it->snap) {
+        it->snap->done = true;
+        it->snap->status = MG_ERR_STATE;
+        it->snap->error = "server destroyed";
+        it = s->queue.erase(it);
+      } else {
+        ++it;
       }
+    }
     s->cv_work.notify_all();
     s->cv_done.notify_all();
+    // Callers still inside mg_server_wait / mg_server_snapshot read the
+    // server's state when they wake: free it only after they have left.
+    s->cv_gone.wait(lk, [&] { return s->waiters == 0; });
   }
   if (s->thread.joinable()) s->thread.join();
   {

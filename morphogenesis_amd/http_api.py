@@ -76,7 +76,10 @@ def main():
     srv = GpuServer(scanner, m, max_active_keys=args.max_active_keys)
     api = WireApi(srv, max_body_bytes=args.max_body_bytes)
     httpd = ThreadingHTTPServer((args.host, args.port), make_handler(api))
-    httpd.daemon_threads = True
+    # Handler threads are joined by server_close() (block_on_close), so none is
+    # still inside the request loop when it is destroyed below.
+    httpd.daemon_threads = False
+    httpd.block_on_close = True
     httpd.verbose = True
     print(f"serving 2^{args.rows_log2} x {args.row_bytes} B on "
           f"http://{args.host}:{httpd.server_address[1]}", file=sys.stderr, flush=True)

@@ -204,14 +204,12 @@ def test_abandoned_requests_are_dropped(mg, scanner):
     srv = mg.GpuServer(scanner, m, segments=4, lane_depth=2)
     keys = [oracle.keygen_bytes(d, t, seed=t)[0] for t in (1, 2, 3)]
     t_gone = srv.submit(keys)
-    srv.abandon(t_gone)                  # still running; its answers will be dropped
+    srv.abandon(t_gone)                  # queued or in flight: it stops costing GPU time
     t_kept = srv.submit(keys)
     ans, _ = srv.wait(t_kept, 3)
     assert np.array_equal(ans, scanner.scan_keys(m, keys))
-    deadline = time.time() + 30
-    while srv.stats()["completed"] < 2 and time.time() < deadline:
-        time.sleep(0.01)
-    assert srv.stats()["completed"] == 2
+    st = srv.stats()
+    assert st["abandoned"] == 1 and 1 <= st["completed"] <= 2
     with pytest.raises(mg.MgError):
         srv.wait(t_gone, 3, timeout_ms=100)  # no longer known
     t_done = srv.submit(keys)
@@ -221,3 +219,189 @@ def test_abandoned_requests_are_dropped(mg, scanner):
         srv.abandon(t_done + 1000)       # unknown ticket
     srv.close()
     m.close()
+
+
+def wait_until(pred, seconds=60):
+    end = time.time() + seconds
+    while not pred():
+        assert time.time() < end, "condition not reached"
+        time.sleep(0.002)
+
+
+def test_abandon_removes_queued_and_frees_in_flight_slots(mg, scanner):
+    # 64 key slots: a 48-key request runs (a pass over 2^28 rows takes a good
+    # fraction of a second), a second 48-key request must queue behind it.
+    n, w, d, seed = 1 << 28, 32, 28, 71
+    m = mg.GpuPageMatrix(scanner, n, w)
+    m.fill_seeded(seed)
+    srv = mg.GpuServer(scanner, m, segments=64, max_active_keys=64)
+    rng = np.random.default_rng(71)
+    big = make_keys(d, [int(x) for x in rng.integers(0, n, size=48)], 0, 700)
+    t_run = srv.submit(big)
+    wait_until(lambda: srv.stats()["active_keys"] == 48)
+    t_queued = srv.submit(make_keys(d, [int(x) for x in rng.integers(0, n, size=48)], 1, 800))
+    assert srv.stats()["queued_requests"] == 1
+    srv.abandon(t_queued)                # never admitted: it leaves the queue
+    st = srv.stats()
+    assert st["queued_requests"] == 0 and st["abandoned"] == 1
+    srv.abandon(t_run)                   # in flight: its keys leave at the next step
+    wait_until(lambda: srv.stats()["active_keys"] == 0)
+    st = srv.stats()
+    assert st["completed"] == 0 and st["abandoned"] == 2 and st["keys_completed"] == 0
+    assert st["sweep_steps"] < 64        # it left before finishing its pass
+    # The slots are free again and the loop answers correctly.
+    keys = make_keys(d, [n // 3, 17, n - 2], 1, 900)
+    ans, _ = srv.wait(srv.submit(keys), 3)
+    assert np.array_equal(ans, oracle.scan_seeded(keys, seed, n, w, n_threads=THREADS))
+    srv.close()
+    m.close()
+
+
+def test_admission_deadline_refuses_late_requests(mg, scanner):
+    n, w, d, seed = 1 << 24, 32, 24, 72
+    m = mg.GpuPageMatrix(scanner, n, w)
+    m.fill_seeded(seed)
+
+    def measured_key_step(srv):
+        # Before any step has been timed every request is admitted; one request
+        # gives the loop its time per key and sweep step.
+        warm = make_keys(d, [1, 2, 3], 0, 1)
+        srv.wait(srv.submit(warm), 3)
+        return srv.stats()["key_step_ms"]
+
+    probe = mg.GpuServer(scanner, m, segments=16)
+    per_key_step = measured_key_step(probe)
+    probe.close()
+    assert per_key_step > 0
+    # A deadline worth about 60 keys of backlog, then a burst of 720 keys: the
+    # late part is refused at submit with MG_ERR_BUSY (the wire layer's 503),
+    # the admitted part is answered exactly.
+    deadline = max(2, int(per_key_step * 16 * 60))
+    srv = mg.GpuServer(scanner, m, segments=16, deadline_ms=deadline)
+    measured_key_step(srv)
+    rng = np.random.default_rng(3)
+    accepted, refused = [], 0
+    for r in range(60):
+        keys = make_keys(d, [int(x) for x in rng.integers(0, n, size=12)], r % 2, 1000 + 20 * r)
+        try:
+            accepted.append((keys, srv.submit(keys)))
+        except mg.MgError as e:
+            assert e.code == mg.MG_ERR_BUSY
+            refused += 1
+    st = srv.stats()
+    assert refused > 0 and st["rejected"] == refused and accepted
+    lat = []
+    for keys, t in accepted:
+        ans, info = srv.wait(t, len(keys))
+        lat.append(info["latency_ms"])
+        assert np.array_equal(ans, scanner.scan_keys(m, keys))
+    ref = oracle.scan_seeded(accepted[0][0][:2], seed, n, w, n_threads=THREADS)
+    assert np.array_equal(srv_answers_first := scanner.scan_keys(m, accepted[0][0][:2]), ref)
+    print(f"deadline {deadline} ms: {len(accepted)} accepted, {refused} refused, "
+          f"max latency {max(lat):.0f} ms, key-step {per_key_step:.4f} ms", flush=True)
+    srv.close()
+    m.close()
+
+
+def test_update_applies_ahead_of_queued_requests(mg, scanner):
+    # The update is submitted after a request that waits for slots; it lands at
+    # the next sweep step, so that request answers at the newer epoch (and says
+    # so), while the request in flight keeps the epoch it was admitted at.
+    n, w, d, seed = 1 << 28, 32, 28, 73
+    m = mg.GpuPageMatrix(scanner, n, w)
+    m.fill_seeded(seed)
+    srv = mg.GpuServer(scanner, m, segments=64, max_active_keys=48)
+    rng = np.random.default_rng(73)
+    first = make_keys(d, [int(x) for x in rng.integers(0, n, size=48)], 0, 10)
+    t1 = srv.submit(first)
+    wait_until(lambda: srv.stats()["active_keys"] == 48)
+    second = make_keys(d, [10, 11, 12], 1, 20)
+    t2 = srv.submit(second)
+    idx = np.array([10, 11, 500], dtype=np.uint64)
+    diffs = rng.integers(0, 256, size=(3, w), dtype=np.uint8)
+    srv.update(idx, diffs, 1)
+    wait_until(lambda: srv.stats()["epoch"] == 1)
+    assert srv.stats()["completed"] == 0      # the update did not wait for the first request
+    a1, i1 = srv.wait(t1, 48)
+    a2, i2 = srv.wait(t2, 3)
+    assert i1["epoch"] == 0 and i2["epoch"] == 1
+    j = [0, 47]
+    assert np.array_equal(a1[j], oracle.scan_seeded([first[i] for i in j], seed, n, w,
+                                                    n_threads=THREADS))
+    want2 = (oracle.scan_seeded(second, seed, n, w, n_threads=THREADS)
+             ^ oracle.scan_delta(second, idx, diffs))
+    assert np.array_equal(a2, want2)
+    srv.close()
+    m.close()
+
+
+@pytest.mark.parametrize("w,segments", [(32, 8), (4096, 2)])
+def test_graph_replayed_sweep_steps_equal_oracle(mg, scanner, w, segments):
+    n, d = (1 << 17, 17) if w == 32 else (9000, 14)
+    rows = oracle.fill_rows(74 + w, 0, n, w)
+    m = mg.GpuPageMatrix.new(scanner, rows)
+    srv = mg.GpuServer(scanner, m, segments=segments, use_graph=True)
+    rng = np.random.default_rng(w)
+    reqs = [make_keys(d, [int(x) for x in rng.integers(0, n, size=k)], r % 2, 100 * r)
+            for r, k in enumerate([3, 6, 4, 3, 9])]
+    # Twice the same requests: the second round replays the captured graphs.
+    for rnd in range(2):
+        tickets = [srv.submit(keys) for keys in reqs]
+        for keys, t in zip(reqs, tickets):
+            ans, _ = srv.wait(t, len(keys))
+            assert np.array_equal(ans, oracle.scan_tiled(keys, rows, THREADS)), rnd
+    assert srv.stats()["graph_launches"] > 0
+    srv.close()
+    m.close()
+
+
+def test_wide_pages_in_column_halves_through_the_server(mg, scanner):
+    # 4 KiB rows from 4 keys per step run as two column halves per row; the
+    # loop's segments, slot lists and accumulation must see the same answers.
+    n, w, d = 9000, 4096, 14
+    rows = oracle.fill_rows(75, 0, n, w)
+    m = mg.GpuPageMatrix.new(scanner, rows)
+    srv = mg.GpuServer(scanner, m, segments=3)
+    keys = make_keys(d, [0, 8999, 4500, 123, 7777], 1, 15)
+    ans, _ = srv.wait(srv.submit(keys), 5)
+    assert np.array_equal(ans, oracle.scan_tiled(keys, rows, THREADS))
+    srv.close()
+    m.close()
+
+
+def test_injected_failure_reaches_waiters_and_queued_snapshot(mg, scanner):
+    # The failure path: a request in flight and a snapshot queued behind it both
+    # get the error; later calls are refused; destroying the loop is safe.
+    n, w, d = 1 << 28, 32, 28
+    m = mg.GpuPageMatrix(scanner, n, w)
+    m.fill_seeded(5)
+    spare = mg.GpuPageMatrix(scanner, n, w)
+    spare.fill_seeded(6)
+    srv = mg.GpuServer(scanner, m, segments=64)
+    rng = np.random.default_rng(76)
+    keys = make_keys(d, [int(x) for x in rng.integers(0, n, size=48)], 0, 3)
+    t = srv.submit(keys)
+    wait_until(lambda: srv.stats()["active_keys"] == 48)
+    snap_err = []
+
+    def swap():
+        try:
+            srv.submit_snapshot(spare, 9)
+        except mg.MgError as e:
+            snap_err.append(e)
+
+    th = threading.Thread(target=swap)
+    th.start()
+    time.sleep(0.05)                      # the snapshot queues behind the keys in flight
+    assert srv.stats()["completed"] == 0
+    srv.inject_failure(mg.MG_ERR_DEVICE)
+    with pytest.raises(mg.MgError) as e:
+        srv.wait(t, 48)
+    assert e.value.code == mg.MG_ERR_DEVICE
+    th.join(timeout=60)
+    assert not th.is_alive() and len(snap_err) == 1
+    with pytest.raises(mg.MgError):
+        srv.submit(make_keys(d, [4], 0, 4))
+    srv.close()
+    m.close()
+    spare.close()
