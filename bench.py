@@ -1,6 +1,6 @@
 """Benchmark of the fused ChaCha8-DPF masked-XOR PIR scan on one MI355X.
 
-Workload (default): the full-mainnet shape, 2^31 rows x 32 B (64 GiB) of
+Headline workload: the full-mainnet shape, 2^31 rows x 32 B (64 GiB) of
 seeded uniform bytes resident in HBM plus an epoch DeltaBuffer of 2^16
 updated rows (uniform indices, random diffs), DPF keys over a 31-bit domain
 with uniform targets. One step = one batched pass of --keys keys over the
@@ -8,15 +8,28 @@ whole matrix and the delta. Keys come in party-0/party-1 pairs so every step
 also checks, bit for bit, that each pair's answers XOR to the target row of
 the updated matrix.
 
-Reported: batched queries/s (a query = 3 cuckoo keys), per-pass p50/p99
-latency, the plain one-query-per-pass rate beside it, scan GB/s and the
-fraction of the integer-VALU and HBM rooflines for the fused kernel (HIP
-event time on the kernel's stream), and a CPU baseline (the oracle's tiled
-scan, all allowed host threads) on a bounded sample of the same workload.
+Reported on the same JSON line:
+  - batched queries/s (a query = 3 cuckoo keys), per-pass p50/p99, the plain
+    one-query-per-pass rate and the reference-shaped plain kernel beside it,
+    scan GB/s, and the roofline of the fused kernel (HIP-event time on the
+    kernel's stream);
+  - by_keys: single passes at 3, 96 and 384 keys on the same matrix;
+  - graph: the headline pass replayed from a HIP graph against eager launches;
+  - by_config: the other workloads, each with its roofline fraction and a
+    bit-exact flag: 2^20 x 32 B (the CPU-runnable pair), 2^27 x 32 B (one
+    key per server, and the pair in one pass), 78,643,200 x 1 KiB (1 and 3
+    keys), and a bounded serving run on the 2^31 x 32 B matrix (Poisson
+    arrivals, an admission deadline, an epoch update every 12 s);
+  - cpu_baseline: the CPU scan on the host cores of the same run, on a bounded
+    sample of the headline workload, in three forms (the reference's tiled
+    eval_fused_3dpf_cpu shape, its server scan's eight-rows-per-step shape,
+    and a 16-lane vector ChaCha form), with the host's core count, CPU quota,
+    model and memory.
 """
 import argparse
 import json
 import os
+import platform
 import statistics
 import sys
 import time
@@ -41,6 +54,9 @@ VALU_MIX_PEAK_TOPS = 256 * 96 * 2.4e9 / 1e12
 # elimination, correction word included) = 700, plus one v_bitop3 per 32-bit
 # word of the row for the masked fold.
 OPS_PER_NODE = 700
+# CPU threads for the oracle legs: the CPU share a one-GPU job gets on the
+# GPU hosts (16 of the host's cores, also its cgroup CPU quota there).
+CPU_SHARE = 16
 
 
 def log(*a):
@@ -56,36 +72,202 @@ def percentile(xs, p):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
+def valu_frac(n_rows, w, n_keys, kernel_ms):
+    return n_keys * n_rows * (OPS_PER_NODE + w // 4) / (kernel_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS
+
+
+def hbm_frac(n_rows, w, kernel_ms):
+    return n_rows * w / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+
+
+def cpu_threads():
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, min(n, CPU_SHARE))
+
+
+def host_info():
+    info = {"os_cpu_count": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "machine": platform.machine()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            info["cpu_model"] = next(l.split(":", 1)[1].strip() for l in f
+                                     if l.startswith("model name"))
+    except Exception:
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()
+            info["cgroup_cpu_quota"] = None if q == "max" else int(q) / int(period)
+    except Exception:
+        info["cgroup_cpu_quota"] = None
+    try:
+        with open("/proc/meminfo") as f:
+            kb = next(int(l.split()[1]) for l in f if l.startswith("MemTotal"))
+        info["mem_total_gib"] = round(kb / 2**20, 1)
+    except Exception:
+        pass
+    return info
+
+
 def cpu_baseline(n_rows, row_bytes, domain_bits, sample_rows_log2, seed):
-    """Oracle tiled scan (C restatement of eval_fused_3dpf_cpu) on a prefix
-    of the same matrix; throughput scaled to the full row count."""
+    """The CPU scan of one query (3 keys) on a prefix of the same matrix, on the
+    host cores of this run, in three forms; throughput scaled to the full row
+    count. value = the fastest form."""
     import numpy as np
 
     import oracle
-    threads = max(1, min(16, len(os.sched_getaffinity(0)),
-                         int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+    threads = cpu_threads()
     n = 1 << sample_rows_log2
     rows = oracle.fill_rows(seed, 0, n, row_bytes)
     keys = [oracle.keygen_bytes(domain_bits, (i * 2654435761) % n, seed=100 + i)[0]
             for i in range(KEYS_PER_QUERY)]
-    times = []
-    for _ in range(5):
-        t0 = time.perf_counter()
-        oracle.scan_tiled(keys, rows, threads)
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
+    forms = [
+        ("tiled", "eval_fused_3dpf_cpu's shape (kernel.rs:1276-1371): 2048-row tiles handed "
+                  "to threads, scalar ChaCha8 eval_subtree per key and tile, masked XOR",
+         lambda: oracle.scan_tiled(keys, rows, threads)),
+        ("rows8", "the server scan's shape (scan.rs:600-649 fold over chunks, :721-895 eight "
+                  "rows x 64 B per step in 512-bit vectors) with scalar ChaCha8 masks",
+         lambda: oracle.scan_rows8(keys, rows, threads)),
+        ("simd", "16 tree nodes per ChaCha8 evaluation in 512-bit vectors, breadth-first "
+                 "4096-leaf subtrees, 512-bit masked folds (oracle_fast.c)",
+         lambda: oracle.scan_fast(keys, rows, threads)),
+    ]
+    ref = None
+    variants = []
     full_scale = n_rows / n
-    q_per_s = 1.0 / (t * full_scale)
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
-    except Exception:
-        pass
-    return {"value": q_per_s, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"oracle scan_tiled, 3 keys (1 query) over the first 2^{sample_rows_log2} of "
-                      f"{n_rows} rows x {row_bytes} B, median of 5 ({t:.2f} s each), scaled "
-                      f"x{full_scale:.1f} to the full matrix; cpu: {model}"}
+    for name, what, fn in forms:
+        fn()  # warm: page in the rows and the code
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ans = fn()
+            times.append(time.perf_counter() - t0)
+        if ref is None:
+            ref = ans
+        elif not np.array_equal(ans, ref):
+            raise SystemExit(f"cpu baseline form {name} differs from the tiled form")
+        t = statistics.median(times)
+        variants.append({"form": name, "what": what, "seconds_per_sample": t,
+                         "queries_per_s": 1.0 / (t * full_scale)})
+        log(f"cpu {name}: {t:.2f} s per sample ({1.0 / (t * full_scale):.4f} queries/s)")
+    best = max(variants, key=lambda v: v["queries_per_s"])
+    host = host_info()
+    return {"value": best["queries_per_s"], "unit": "queries/s", "cores": threads,
+            "kind": "port",
+            "sample": f"3 keys (1 query) over the first 2^{sample_rows_log2} of {n_rows} rows x "
+                      f"{row_bytes} B, median of 3 passes, scaled x{full_scale:.0f} to the full "
+                      f"matrix; value = fastest form ({best['form']}); {threads} threads "
+                      f"(the job's CPU share) on {host.get('cpu_model', 'unknown CPU')}",
+            "isa": oracle.simd_isa(), "variants": variants, "host": host}
+
+
+def scan_passes(scanner, m, keys, opts, reps, warm=1):
+    ans = None
+    kern, wall = [], []
+    t = None
+    for i in range(warm + reps):
+        s0 = time.perf_counter()
+        ans, t = scanner.scan_keys(m, keys, opts, with_timing=True)
+        if i >= warm:
+            wall.append((time.perf_counter() - s0) * 1e3)
+            kern.append(t["scan_kernel_ms"])
+    return ans, kern, wall, t
+
+
+def config_small(mg, scanner, n_rows, w, d, seed, label, one_key_passes=True):
+    """A matrix of n_rows seeded rows and one key pair with a uniform target:
+    each server's key alone (and the pair in one pass), every answer compared
+    with the oracle (the CPU scan) and the pair's XOR with the target row."""
+    import numpy as np
+
+    import oracle
+    threads = cpu_threads()
+    m = mg.GpuPageMatrix(scanner, n_rows, w)
+    m.fill_seeded(seed)
+    rng = np.random.default_rng(seed)
+    target = int(rng.integers(0, n_rows))
+    pair = list(mg.generate_keys(d, target, seed=seed + 7))
+    row = oracle.fill_rows(seed, target, 1, w)[0]
+    exact = True
+    out = {"workload": label, "rows": n_rows, "row_bytes": w, "domain_bits": d}
+    per_server = []
+    for party in (0, 1):
+        ans, kern, wall, t = scan_passes(scanner, m, [pair[party]], None, 5, warm=2)
+        t0 = time.perf_counter()
+        ref = oracle.scan_seeded([pair[party]], seed, n_rows, w, n_threads=threads)
+        cpu_s = time.perf_counter() - t0
+        exact &= bool(np.array_equal(ans, ref))
+        per_server.append(ans[0])
+        km = statistics.mean(kern)
+        out.setdefault("one_key", []).append(
+            {"party": party, "kernel_ms": km, "pass_ms": statistics.mean(wall),
+             "kernel": t["kernel"], "lane_depth": t["lane_depth"], "waves": t["waves"],
+             "gbs": n_rows * w / (km * 1e-3) / 1e9, "hbm_frac": hbm_frac(n_rows, w, km),
+             "valu_frac": valu_frac(n_rows, w, 1, km),
+             "oracle_s": cpu_s})
+    exact &= bool(np.array_equal(per_server[0] ^ per_server[1], row))
+    ans, kern, wall, t = scan_passes(scanner, m, pair, None, 5, warm=1)
+    exact &= bool(np.array_equal(ans[0] ^ ans[1], row))
+    km = statistics.mean(kern)
+    out["pair_one_pass"] = {"kernel_ms": km, "pass_ms": statistics.mean(wall),
+                            "kernel": t["kernel"], "lane_depth": t["lane_depth"],
+                            "waves": t["waves"], "gbs": n_rows * w / (km * 1e-3) / 1e9,
+                            "hbm_frac": hbm_frac(n_rows, w, km),
+                            "valu_frac": valu_frac(n_rows, w, 2, km),
+                            "valu_mix_frac": valu_frac(n_rows, w, 2, km) * VALU_PEAK_TOPS
+                            / VALU_MIX_PEAK_TOPS}
+    out["bit_exact"] = exact
+    m.close()
+    return out
+
+
+def config_pages(mg, scanner, seed, key_counts=(1, 3)):
+    """78,643,200 rows x 1 KiB, 27-bit keys: one pass of k keys per server, each
+    server's answers compared with the oracle, pairs XOR-ed to their target rows."""
+    import numpy as np
+
+    import oracle
+    n, w, d = 78_643_200, 1024, 27
+    threads = cpu_threads()
+    m = mg.GpuPageMatrix(scanner, n, w)
+    m.fill_seeded(seed)
+    rng = np.random.default_rng(seed)
+    out = {"workload": "78,643,200 rows x 1 KiB, 27-bit keys, k keys per pass (k cuckoo "
+                       "lookups of one query for k = 3)", "rows": n, "row_bytes": w,
+           "domain_bits": d, "by_keys": {}}
+    # Two untimed passes: the first passes over freshly written rows run slower.
+    warm_keys = [mg.generate_keys(d, 1, seed=1)[0]]
+    for _ in range(2):
+        scanner.scan_keys(m, warm_keys)
+    for k in key_counts:
+        targets = [int(x) for x in rng.integers(0, n, size=k)]
+        pairs = [mg.generate_keys(d, t, seed=seed * 10 + 100 * k + i) for i, t in enumerate(targets)]
+        exact = True
+        answers = []
+        kern_all = []
+        t = None
+        for party in (0, 1):
+            keys = [p[party] for p in pairs]
+            ans, kern, wall, t = scan_passes(scanner, m, keys, None, 5, warm=1)
+            kern_all += kern
+            exact &= bool(np.array_equal(
+                ans, oracle.scan_seeded(keys, seed, n, w, n_threads=threads)))
+            answers.append(ans)
+        for q, tg in enumerate(targets):
+            exact &= bool(np.array_equal(answers[0][q] ^ answers[1][q],
+                                         oracle.fill_rows(seed, tg, 1, w)[0]))
+        km = statistics.mean(kern_all)
+        out["by_keys"][k] = {"kernel_ms": km, "kernel_ms_min": min(kern_all),
+                             "gbs": n * w / (km * 1e-3) / 1e9, "hbm_frac": hbm_frac(n, w, km),
+                             "valu_frac": valu_frac(n, w, k, km), "kernel": t["kernel"],
+                             "lane_depth": t["lane_depth"], "keys_per_group": t["keys_per_wave"],
+                             "waves": t["waves"], "bit_exact": exact}
+        log(f"pages {k} keys: {km:.2f} ms, HBM {hbm_frac(n, w, km):.3f}, exact {exact}")
+    m.close()
+    return out
 
 
 def main():
@@ -108,15 +290,22 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--lane-depth", type=int, default=0)
     ap.add_argument("--waves-per-key", type=int, default=0)
-    ap.add_argument("--use-graph", action="store_true")
+    ap.add_argument("--use-graph", action="store_true",
+                    help="time the headline steps from a HIP graph")
     ap.add_argument("--cpu-sample-log2", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=31)
-    ap.add_argument("--q-sweep", default="",
+    ap.add_argument("--q-sweep", default="3,96,384",
                     help="comma-separated key counts: one extra timed pass each, reported in by_keys")
     ap.add_argument("--delta-rows", type=int, default=1 << 16,
                     help="epoch DeltaBuffer entries applied in every pass (uniform rows, random "
                          "diffs); 0 = none")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other workloads (by_config)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="seconds of Poisson arrivals in the bounded serving run (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=5.0)
+    ap.add_argument("--serve-deadline-ms", type=int, default=600)
     ap.add_argument("--tuned", action="store_true",
                     help="time the autotuner's stored setting for this shape "
                          "(morphogenesis_amd/tuned.json) when there is one")
@@ -138,6 +327,7 @@ def main():
 
     import morphogenesis_amd as mg
 
+    t_bench = time.perf_counter()
     n_rows = args.rows or (1 << args.rows_log2)
     w = args.row_bytes
     d = args.domain_bits or max(1, (n_rows - 1).bit_length())
@@ -202,13 +392,16 @@ def main():
         else:
             tuned = autotune.lookup(autotune.device_arch(scanner), w, n_rows, q)
         log(f"tuned setting: {tuned}")
-    if tuned:
-        opts = mg.make_opts(**tuned, use_graph=args.use_graph, delta=delta)
-    else:
-        opts = mg.make_opts(kernel=kernel, lane_depth=args.lane_depth,
+
+    def opts_for(use_graph):
+        if tuned:
+            return mg.make_opts(**tuned, use_graph=use_graph, delta=delta)
+        return mg.make_opts(kernel=kernel, lane_depth=args.lane_depth,
                             keys_per_wave=args.keys_per_wave, waves_per_key=args.waves_per_key,
-                            waves_per_block=args.waves_per_block, use_graph=args.use_graph,
+                            waves_per_block=args.waves_per_block, use_graph=use_graph,
                             delta=delta)
+
+    opts = opts_for(args.use_graph)
     odd_partner = scanner.scan_keys(m, [pairs[-1][1]], opts)[0] if q % 2 else None
 
     def check(ans):
@@ -236,6 +429,24 @@ def main():
     wall = time.perf_counter() - t_start
     timing = t
 
+    # The same pass from a HIP graph (or eagerly, when the steps used the graph).
+    alt = opts_for(not args.use_graph)
+    alt_ms, alt_kern = [], []
+    for i in range(3):
+        s0 = time.perf_counter()
+        ans, t_alt = scanner.scan_keys(m, keys, alt, with_timing=True)
+        check(ans)
+        if i:
+            alt_ms.append((time.perf_counter() - s0) * 1e3)
+            alt_kern.append(t_alt["scan_kernel_ms"])
+    graph = {"steps_use_graph": bool(args.use_graph),
+             "eager_pass_ms": statistics.mean(alt_ms if args.use_graph else step_ms),
+             "graph_pass_ms": statistics.mean(step_ms if args.use_graph else alt_ms),
+             "eager_kernel_ms": statistics.mean(alt_kern if args.use_graph else scan_ms),
+             "graph_kernel_ms": statistics.mean(scan_ms if args.use_graph else alt_kern),
+             "bit_exact": True}
+    log(f"graph replay: {graph['graph_pass_ms']:.1f} ms against eager {graph['eager_pass_ms']:.1f} ms")
+
     # Plain path: one query (3 keys) per pass over the whole matrix.
     plain_ms = []
     plain_keys = (keys * KEYS_PER_QUERY)[:KEYS_PER_QUERY]
@@ -258,23 +469,25 @@ def main():
                 raise SystemExit("bit-exactness failure in the suffix-walk baseline")
             log(f"reference-shaped plain pass {i}: {ref_ms[-1]:.1f} ms per 3 keys")
 
-    # Optional single passes at other batch sizes (2^31 x 32 B: Q in {3, 24, 96, 384}).
+    # Single passes at other batch sizes (2^31 x 32 B: Q in {3, 24, 96, 384}).
     by_keys = {}
     for qs in [int(x) for x in args.q_sweep.split(",") if x.strip()]:
-        qs = max(2, qs - qs % 2)
-        sw_targets = [int(x) for x in rng.integers(0, n_rows, size=qs // 2)]
+        qs = max(1, qs)
+        sw_targets = [int(x) for x in rng.integers(0, n_rows, size=(qs + 1) // 2)]
         sw_keys = [k for i, t in enumerate(sw_targets)
-                   for k in mg.generate_keys(d, t, seed=50_000 + i)]
+                   for k in mg.generate_keys(d, t, seed=50_000 + i)][:qs]
         s0 = time.perf_counter()
         ans, t = scanner.scan_keys(m, sw_keys, opts, with_timing=True)
         wall_q = time.perf_counter() - s0
-        for i, tt in enumerate(sw_targets[:4]):
-            if not np.array_equal(ans[2 * i] ^ ans[2 * i + 1], want_row(tt)):
-                raise SystemExit(f"bit-exactness failure in the {qs}-key pass")
+        exact = all(np.array_equal(ans[2 * i] ^ ans[2 * i + 1], want_row(tt))
+                    for i, tt in enumerate(sw_targets[:qs // 2]))
+        if not exact:
+            raise SystemExit(f"bit-exactness failure in the {qs}-key pass")
         by_keys[qs] = {"pass_ms": wall_q * 1e3, "kernel_ms": t["scan_kernel_ms"],
                        "queries_per_s": qs / KEYS_PER_QUERY / wall_q,
-                       "valu_frac": qs * n_rows * (OPS_PER_NODE + w // 4)
-                       / (t["scan_kernel_ms"] * 1e-3) / 1e12 / VALU_PEAK_TOPS}
+                       "valu_frac": valu_frac(n_rows, w, qs, t["scan_kernel_ms"]),
+                       "kernel": t["kernel"], "lane_depth": t["lane_depth"],
+                       "waves": t["waves"], "pairs_checked": qs // 2, "bit_exact": exact}
         log(f"{qs}-key pass: {wall_q * 1e3:.1f} ms ({by_keys[qs]['queries_per_s']:.2f} queries/s)")
 
     queries_per_step = q / KEYS_PER_QUERY
@@ -309,7 +522,43 @@ def main():
         roofline = dict(bound="hbm", **hbm_side, traffic=traffic, kernel_ms=avg_scan_ms,
                         valu=valu_side)
 
+    by_config = {}
+    if not args.no_configs and args.serve_seconds > 0:
+        # Serving on the same matrix (its rows are the seeded ones: the delta
+        # above is a separate buffer the loop never sees).
+        from morphogenesis_amd import serve_bench
+        log(f"serving: {args.serve_seconds} s of Poisson arrivals at {args.serve_rate} req/s, "
+            f"deadline {args.serve_deadline_ms} ms")
+        srv = serve_bench.run(scanner, m, args.seed, rate=args.serve_rate,
+                              duration=args.serve_seconds, drain=30.0,
+                              deadline_ms=args.serve_deadline_ms, log=log)
+        srv_stats = srv.pop("server")
+        srv["sweep_steps"] = srv_stats["sweep_steps"]
+        srv["model_key_step_ms"] = srv_stats["key_step_ms"]
+        srv["model_step_fixed_ms"] = srv_stats["step_fixed_ms"]
+        srv["vs_plain"] = (srv["queries_per_s"] / srv["plain_queries_per_s"]
+                           if srv["queries_per_s"] and srv["plain_queries_per_s"] else None)
+        by_config["serving_2e31x32"] = srv
+        log(f"serving: {srv['completed']} done, {srv['refused']} refused, p50 {srv['p50_ms']} "
+            f"p99 {srv['p99_ms']} ms, exact {srv['bit_exact']}")
+
     rows_label = f"2^{n_rows.bit_length() - 1}" if n_rows & (n_rows - 1) == 0 else str(n_rows)
+    # Objects go before the device that holds them (closing it first is refused).
+    if delta is not None:
+        delta.close()
+    m.close()
+
+    if not args.no_configs:
+        by_config["plain_cpu_2e20x32"] = config_small(
+            mg, scanner, 1 << 20, 32, 20, 2020, "2^20 rows x 32 B, one key pair, 20-bit domain")
+        log(f"2^20: {by_config['plain_cpu_2e20x32']['one_key'][0]['kernel_ms']:.3f} ms per key")
+        by_config["plain_gpu_2e27x32"] = config_small(
+            mg, scanner, 1 << 27, 32, 27, 2727, "2^27 rows x 32 B, one key pair, 27-bit domain")
+        c2 = by_config["plain_gpu_2e27x32"]
+        log(f"2^27: one key {c2['one_key'][0]['kernel_ms']:.2f} ms, pair "
+            f"{c2['pair_one_pass']['kernel_ms']:.2f} ms (VALU {c2['pair_one_pass']['valu_frac']:.3f})")
+        by_config["pages_78643200x1024"] = config_pages(mg, scanner, 7864)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -350,15 +599,14 @@ def main():
         "batched_vs_reference_shaped_plain": (qps * statistics.mean(ref_ms) / 1e3) if ref_ms else None,
         "scan_gbs": scan_gbs,
         "by_keys": by_keys or None,
+        "graph": graph,
+        "by_config": by_config or None,
         "bit_exact": True,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "bench_seconds": time.perf_counter() - t_bench,
     }
     print(json.dumps(out), flush=True)
-    # Objects go before the device that holds them (closing it first is refused).
-    if delta is not None:
-        delta.close()
-    m.close()
     scanner.close()
 
 
