@@ -137,10 +137,6 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   const uint32_t total = p.parts * p.n_keys;
   const uint32_t w = xcd_logical_wave(total, blockDim.x >> 6, wib);
   if (w - (uint32_t)wib >= total) return;  // the whole block is past the work
-  // The block's waves (consecutive keys of one part) meet at a barrier after
-  // every tile, so they stay in step and their identical row reads hit in
-  // L2 instead of each fetching the rows again. Waves past the work, and
-  // waves of a part with fewer tiles, take the same number of barriers.
   const bool live = w < total;
   const uint32_t key_i = live ? w % p.n_keys : 0;
   const uint32_t part = live ? w / p.n_keys : 0;
@@ -169,81 +165,112 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
 #pragma unroll
   for (int c = 0; c < C; c++) acc[c] = make_uint4(0, 0, 0, 0);
 
-  const uint64_t t_begin = p.tile_begin + (uint64_t)part * p.tiles_per_wave;
-  uint64_t t_end = t_begin + p.tiles_per_wave;
-  if (t_end > p.tile_end) t_end = p.tile_end;
-  if (!live || t_end < t_begin) t_end = t_begin;
-  const bool lane_active = lane < (1 << lb);
+  // This wave's leaf-pair steps: global steps gs0 .. gs0 + n_steps - 1, step gs
+  // being pair gs mod 2^(kl-1) of tile tile_begin + gs / 2^(kl-1).
   const uint32_t n_pairs = 1u << (kl - 1);
+  const uint64_t total_steps = (p.tile_end - p.tile_begin) << (kl - 1);
+  const uint64_t gs0 = (uint64_t)part * p.steps_per_wave;
+  uint64_t n_steps = 0;
+  if (live && gs0 < total_steps)
+    n_steps = total_steps - gs0 < p.steps_per_wave ? total_steps - gs0 : p.steps_per_wave;
+  const bool lane_active = lane < (1 << lb);
   const int leaf_level = d - kl;
+
+  auto push = [&](int slot, const Node& R) {  // the right child at depth slot + 1
+    const u32x4 rv = {R.s0, R.s1, R.s2, R.s3};
+    if (kGlobalStack && slot < gs)
+      st_glob[slot * 64 + lane] = rv;
+    else
+      st_seed[(slot - gs) * 64 + lane] = rv;
+    const uint32_t bit = 1u << (slot + 1);
+    st_tbits = R.t == 1u ? (st_tbits | bit) : (st_tbits & ~bit);
+  };
 
   LanePath path;
   {
     const Node r = key_root(k);
     path.s0 = r.s0; path.s1 = r.s1; path.s2 = r.s2; path.s3 = r.s3; path.t = r.t;
   }
-  for (uint64_t ti = 0; ti < p.tiles_per_wave; ++ti) {
-    const uint64_t tile = t_begin + ti;
-    if (ti) __syncthreads();
-    if (tile >= t_end) continue;
-    Node node = tile_lane_root(k, path, lane, tile, tile - 1, tile == t_begin, tb, lb);
-    const uint64_t base = (tile << (kl + lb)) | ((uint64_t)lane << kl);
-
-    // One iteration per leaf pair: pop the pending right child (if any), walk
-    // down to the bottom node pushing right children, expand it into the two
-    // leaves and fold the pair's rows into the accumulators. The rows are
-    // loaded at the start of the step (kLateLoads false: in flight during the
-    // expansions) or after the expansions (true: the rows stay out of the
-    // registers the ChaCha rounds need, 69 VGPRs for 32-byte rows against 78,
-    // seven waves per SIMD against six, and the other waves on the SIMD run
-    // while a wave waits for its rows). Loads are unconditional: a row past
-    // n_rows (or of an idle lane) is read from a zero buffer instead.
-    int lvl = 0;
-    for (uint32_t pair = 0; pair < n_pairs; ++pair) {
-      if (pair) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
-        lvl = kl - 1 - __builtin_ctz(pair);
-        u32x4 sv;
-        if (kGlobalStack && lvl - 1 < gs)
-          sv = st_glob[(lvl - 1) * 64 + lane];
-        else
-          sv = st_seed[(lvl - 1 - gs) * 64 + lane];
-        node.s0 = sv.x; node.s1 = sv.y; node.s2 = sv.z; node.s3 = sv.w;
-        node.t = (st_tbits >> lvl) & 1u;
-      }
-      uint4 r0[C], r1[C];
-      auto load_rows = [&]() {
-        const uint64_t x = base + 2 * (uint64_t)pair;
-        const uint4* src0 = (lane_active && x < p.n_rows) ? p.rows + x * C : p.zero_rows;
-        const uint4* src1 =
-            (lane_active && x + 1 < p.n_rows) ? p.rows + (x + 1) * C : p.zero_rows;
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-          r0[c] = src0[c];
-          r1[c] = src1[c];
+  Node node;
+  uint64_t base = 0;
+  // One iteration per leaf pair: pop the pending right child (if any), walk
+  // down to the bottom node pushing right children, expand it into the two
+  // leaves and fold the pair's rows into the accumulators. The rows are
+  // loaded at the start of the step (kLateLoads false: in flight during the
+  // expansions) or after the expansions (true: the rows stay out of the
+  // registers the ChaCha rounds need, 69 VGPRs for 32-byte rows against 78,
+  // seven waves per SIMD against six, and the other waves on the SIMD run
+  // while a wave waits for its rows). Loads are unconditional: a row past
+  // n_rows (or of an idle lane) is read from a zero buffer instead.
+  // The block's waves (consecutive keys, and parts) take the same number of
+  // steps and meet at a barrier every 2^(kl-1) of them (every tile when runs
+  // start at tile boundaries), so they stay in step and their identical row
+  // reads hit in L2 instead of each fetching the rows again. Waves past the
+  // work, and waves with shorter runs, take the same barriers.
+  for (uint64_t s = 0; s < p.steps_per_wave; ++s) {
+    if (s && (s & (n_pairs - 1)) == 0) __syncthreads();
+    if (s >= n_steps) continue;
+    const uint64_t step = gs0 + s;  // (gs is the count of global stack levels)
+    const uint64_t tile = p.tile_begin + (step >> (kl - 1));
+    const uint32_t pair = (uint32_t)(step & (n_pairs - 1));
+    int lvl;
+    if (s == 0 || pair == 0) {  // the run's first step, or a new tile
+      node = tile_lane_root(k, path, lane, tile, tile - 1, s == 0, tb, lb);
+      base = (tile << (kl + lb)) | ((uint64_t)lane << kl);
+      lvl = 0;
+      if (pair != 0) {
+        // The run starts inside a tile: walk from the lane root down to leaf
+        // pair `pair`, keeping the right children that lie ahead of it, as the
+        // depth-first walk from pair 0 would have left them.
+        for (int l = 0; l < kl - 1; ++l) {
+          Node L, R;
+          expand_node(k, leaf_level + l, node, L, R);
+          if ((pair >> (kl - 2 - l)) & 1u) {
+            node = R;
+          } else {
+            push(l, R);
+            node = L;
+          }
         }
-      };
-      if constexpr (!kLateLoads) load_rows();
-      for (; lvl < kl - 1; ++lvl) {
-        Node L, R;
-        expand_node(k, leaf_level + lvl, node, L, R);
-        const u32x4 rv = {R.s0, R.s1, R.s2, R.s3};  // depth lvl + 1
-        if (kGlobalStack && lvl < gs)
-          st_glob[lvl * 64 + lane] = rv;
-        else
-          st_seed[(lvl - gs) * 64 + lane] = rv;
-        const uint32_t bit = 1u << (lvl + 1);
-        st_tbits = R.t == 1u ? (st_tbits | bit) : (st_tbits & ~bit);
-        node = L;
+        lvl = kl - 1;
       }
-      Node L, R;
-      expand_node(k, leaf_level + kl - 1, node, L, R);
-      if constexpr (kLateLoads) load_rows();
-      const uint4 m0 = opaque4(leaf_mask(k, L)), m1 = opaque4(leaf_mask(k, R));
+    } else {  // the right child at depth lvl >= 1 sits in slot lvl - 1
+      lvl = kl - 1 - __builtin_ctz(pair);
+      u32x4 sv;
+      if (kGlobalStack && lvl - 1 < gs)
+        sv = st_glob[(lvl - 1) * 64 + lane];
+      else
+        sv = st_seed[(lvl - 1 - gs) * 64 + lane];
+      node.s0 = sv.x; node.s1 = sv.y; node.s2 = sv.z; node.s3 = sv.w;
+      node.t = (st_tbits >> lvl) & 1u;
+    }
+    uint4 r0[C], r1[C];
+    auto load_rows = [&]() {
+      const uint64_t x = base + 2 * (uint64_t)pair;
+      const uint4* src0 = (lane_active && x < p.n_rows) ? p.rows + x * C : p.zero_rows;
+      const uint4* src1 =
+          (lane_active && x + 1 < p.n_rows) ? p.rows + (x + 1) * C : p.zero_rows;
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        xor_masked(acc[c], r0[c], m0);
-        xor_masked(acc[c], r1[c], m1);
+        r0[c] = src0[c];
+        r1[c] = src1[c];
       }
+    };
+    if constexpr (!kLateLoads) load_rows();
+    for (; lvl < kl - 1; ++lvl) {
+      Node L, R;
+      expand_node(k, leaf_level + lvl, node, L, R);
+      push(lvl, R);  // depth lvl + 1
+      node = L;
+    }
+    Node L, R;
+    expand_node(k, leaf_level + kl - 1, node, L, R);
+    if constexpr (kLateLoads) load_rows();
+    const uint4 m0 = opaque4(leaf_mask(k, L)), m1 = opaque4(leaf_mask(k, R));
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      xor_masked(acc[c], r0[c], m0);
+      xor_masked(acc[c], r1[c], m1);
     }
   }
 

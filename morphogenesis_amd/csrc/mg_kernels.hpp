@@ -25,7 +25,10 @@ struct LaneDfsParams {
   uint32_t d, kl, lb, tb;
   uint64_t tile_begin;  // tiles [tile_begin, tile_end) are scanned; rows
   uint64_t tile_end;    // at or past n_rows are skipped
-  uint64_t tiles_per_wave;
+  // Wave w of part `part` takes leaf-pair steps [part * steps_per_wave, +steps_per_wave) of
+  // the range, step = (tile - tile_begin) * 2^(kl-1) + pair; a run may start and end inside
+  // a tile.
+  uint64_t steps_per_wave;
   uint32_t parts;  // waves per key; wave w serves key w % n_keys
   uint4* partial;  // [parts][n_keys][chunks]
   // The top gslots levels of each lane's DFS stack (depths 1 .. gslots) live in

@@ -235,17 +235,18 @@ void split_levels(uint32_t d, uint32_t kl_req, Plan* pl) {
 
 // Lane-DFS launch shape. Blocks of `wpb` waves are dealt to the CUs evenly and
 // their waves to the 4 SIMDs, so the most loaded SIMD runs ceil(waves / SIMDs)
-// waves of `tiles_per_wave` tiles. A tile costs each lane 2^kl leaf
-// expansions plus about 8 more (its lb-level path to the lane root and the
-// tile prefix). The SIMD's rate depends on how many waves share it (the
-// ChaCha node rate at 1 .. 8 waves per SIMD, measured: 0.488, 0.834, 0.878,
-// 0.908, 0.946, 0.966, 0.986, 1.0 of the 8-wave rate; the stage-spaced s_nops
-// of the node routine leave a lone wave idle, more waves fill them). The shape minimising
-// waves x tiles x (2^kl + 8) / rate on the most loaded SIMD wins; kl is
-// searched when `kl_fixed` is 0, tiles per wave when `parts_fixed` is 0.
-// 2^27 x 32 B, 3 keys: kl = 9 as 1366 parts x 3 tiles gives 4098 waves, one
-// CU takes a fifth block, and the pass runs 9.43 ms against 7.35 ms for kl = 7
-// as 6144 waves of 8 tiles, which this picks.
+// waves. A wave takes an equal run of leaf-pair steps (2^(kl-1) per tile,
+// runs may start and end inside a tile): per lane about two node expansions
+// per step, 8 more per tile it touches (the lb-level path to the lane root and
+// the tile prefix) and kl - 1 to seek its first step. The SIMD's rate depends
+// on how many waves share it, relative to 8 waves: the node routine alone
+// measured 0.488, 0.834, 0.878, 0.908, 0.946, 0.966, 0.986, 1.0 at 1 .. 8 waves
+// (the stage-spaced s_nops leave a lone wave idle, more waves fill them); the
+// whole kernel loses more at 3-4 waves (2^27 x 32 B, one key, kl = 9: 0.499,
+// 0.827, 0.869 and 1.0 of the 7-wave rate at about 1, 2, 3.5 and 7 waves per
+// SIMD), which the table below follows. The shape minimising the most loaded
+// SIMD's work / rate wins; kl is searched when `kl_fixed` is 0, waves per key
+// when `parts_fixed` is 0.
 // Lane-DFS stack levels the plan may keep in global memory instead of LDS.
 constexpr uint32_t kLaneMaxGlobalSlots = 4;
 
@@ -254,14 +255,14 @@ struct LaneShape {
   uint32_t wpb = 0;
   uint32_t gslots = 0;
   uint32_t late_loads = 0;
-  uint64_t tiles_per_wave = 0;
+  uint64_t parts = 0;
   double cost = 0;
 };
 
 int choose_lane_shape(mg_device* dev, const Plan& pl0, uint32_t d, uint32_t n_keys, uint32_t kl_fixed,
                       uint32_t wpb_req, uint64_t parts_fixed, uint32_t gs_fixed,
                       uint32_t loads_fixed, LaneShape* out) {
-  static const double kRate[9] = {0, 0.488, 0.834, 0.878, 0.908, 0.946, 0.966, 0.986, 1.0};
+  static const double kRate[9] = {0, 0.49, 0.82, 0.84, 0.86, 0.90, 0.94, 0.986, 1.0};
   const uint64_t simds = (uint64_t)dev->cus * 4;
   LaneShape best;
   const uint32_t kl_lo = kl_fixed ? std::min(kl_fixed, d) : 1;
@@ -292,7 +293,9 @@ int choose_lane_shape(mg_device* dev, const Plan& pl0, uint32_t d, uint32_t n_ke
         if (rc) return rc;
         const uint64_t cap = std::max<uint64_t>((uint64_t)slots, 1);
         // Candidate waves per key: the most that stay resident and a few fewer, the counts that
-        // fill a whole number of waves per SIMD, and those of the smallest tiles per wave.
+        // fill a whole number of waves per SIMD, and those of whole tiles per wave.
+        const uint64_t n_pairs = 1ull << (kl - 1);
+        const uint64_t total_steps = n_tiles * n_pairs;
         uint64_t cand[32];
         int nc = 0;
         if (parts_fixed) {
@@ -306,26 +309,31 @@ int choose_lane_shape(mg_device* dev, const Plan& pl0, uint32_t d, uint32_t n_ke
           for (uint64_t t = t_lo; t < t_lo + 8; ++t) cand[nc++] = (n_tiles + t - 1) / t;
         }
         for (int ci = 0; ci < nc; ++ci) {
-          const uint64_t p0 = std::max<uint64_t>(1, std::min(cand[ci], n_tiles));
-          const uint64_t tpw = (n_tiles + p0 - 1) / p0;
-          const uint64_t parts = (n_tiles + tpw - 1) / tpw;
+          const uint64_t p0 = std::max<uint64_t>(1, std::min(cand[ci], total_steps));
+          const uint64_t spw = (total_steps + p0 - 1) / p0;
+          const uint64_t parts = (total_steps + spw - 1) / spw;
           const uint64_t waves = parts * n_keys;
           // Waves beyond the resident ones run in later rounds.
           const uint64_t rounds = (waves + cap - 1) / cap;
           const uint64_t per_round = (waves + rounds - 1) / rounds;
           const uint64_t wps = (per_round + simds - 1) / simds;
           const double rate = kRate[std::min<uint64_t>(wps, 8)];
+          // Per lane: two expansions per step, 8 per tile touched, the seek when a
+          // run starts inside a tile.
+          const bool whole_tiles = spw % n_pairs == 0;
+          const double tiles_touched = (double)((spw + n_pairs - 1) / n_pairs + (whole_tiles ? 0 : 1));
+          const double work = 2.0 * (double)spw + 8.0 * tiles_touched + (whole_tiles ? 0.0 : kl - 1.0);
           // Rows loaded after a step's expansions wait in the open: 1 % dearer at
           // equal waves per SIMD (2^27 x 32 B, one key, four waves per SIMD: 2.87
           // against 2.83 ms), so they win only where they add waves.
-          const double cost = (double)rounds * (double)wps * (double)tpw *
-                              (double)((1ull << kl) + 8) / rate * (late ? 1.01 : 1.0);
+          const double cost =
+              (double)rounds * (double)wps * work / rate * (late ? 1.01 : 1.0);
           if (best.kl == 0 || cost < best.cost * 0.999) {
             best.kl = kl;
             best.wpb = wpb;
             best.gslots = gs;
             best.late_loads = late;
-            best.tiles_per_wave = tpw;
+            best.parts = parts;
             best.cost = cost;
           }
         }
@@ -570,14 +578,14 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   }
   if (rc) return rc;
   uint64_t parts = o->waves_per_key;
-  if (!parts && kernel == MG_KERNEL_LANE_DFS && lane_shape.tiles_per_wave)
-    parts = (n_tiles + lane_shape.tiles_per_wave - 1) / lane_shape.tiles_per_wave;
+  if (!parts && kernel == MG_KERNEL_LANE_DFS && lane_shape.kl == pl->kl) parts = lane_shape.parts;
   if (!parts) parts = std::max<uint64_t>(1, (uint64_t)resident / (groups * pl->col_splits));
-  if (kernel == MG_KERNEL_STREAM_SPLIT) {
-    // Pairs take equal runs of leaf-pair steps, which may start and end inside a tile, so
-    // every resident pair gets work (78,643,200 x 1 KiB at kl 7, 1 key: 9600 tiles are 13 per
-    // pair on 739 of 768 pairs, 800 steps on all 768; 11.4-11.7 against 11.7-12.0 ms). The
-    // pairs of a block never wait for each other, so runs out of phase cost nothing.
+  if (kernel == MG_KERNEL_STREAM_SPLIT || kernel == MG_KERNEL_LANE_DFS) {
+    // Waves (lane-DFS) and pairs (split) take equal runs of leaf-pair steps, which may start
+    // and end inside a tile, so every resident wave or pair gets work (78,643,200 x 1 KiB at
+    // kl 7, 1 key: 9600 tiles are 13 per pair on 739 of 768 pairs, 800 steps on all 768;
+    // 11.4-11.7 against 11.7-12.0 ms). The pairs of a split block never wait for each
+    // other; lane-DFS blocks meet at a barrier every 2^(kl-1) steps.
     const uint64_t n_pairs = 1ull << (pl->kl - 1);
     const uint64_t total_steps = n_tiles * n_pairs;
     parts = std::min<uint64_t>(parts, total_steps ? total_steps : 1);
@@ -618,7 +626,7 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       p.d = pl.d; p.kl = pl.kl; p.lb = pl.lb; p.tb = pl.tb;
       p.tile_begin = pl.tile_begin;
       p.tile_end = pl.tile_end;
-      p.tiles_per_wave = pl.tiles_per_wave;
+      p.steps_per_wave = pl.steps_per_wave;
       p.parts = pl.parts;
       p.partial = b.partial;
       p.gstack = b.gstack;
