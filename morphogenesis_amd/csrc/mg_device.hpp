@@ -8,7 +8,6 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include <utility>
 
 #include "mg_key.hpp"
 
@@ -33,47 +32,8 @@ constexpr uint32_t kC0 = 0x61707865u, kC1 = 0x3320646eu, kC2 = 0x79622d32u,
 
 // PRG expand (chacha_prg.rs:153-178): key = seed||seed, nonce 0. Block 0
 // gives the two child seeds (words 0-3, 4-7); block 1 only contributes the
-// low bits of words 0 and 1. Both blocks are written out in full; the
-// compiler drops block 1's unused words and shares its three counter-free
-// first-round column quarter-rounds with block 0.
-__device__ __forceinline__ void prg_expand(uint32_t s0, uint32_t s1,
-                                           uint32_t s2, uint32_t s3,
-                                           uint32_t l[4], uint32_t r[4],
-                                           uint32_t& lt, uint32_t& rt) {
-  uint32_t x0 = kC0, x1 = kC1, x2 = kC2, x3 = kC3;
-  uint32_t x4 = s0, x5 = s1, x6 = s2, x7 = s3;
-  uint32_t x8 = s0, x9 = s1, x10 = s2, x11 = s3;
-  uint32_t x12 = 0, x13 = 0, x14 = 0, x15 = 0;
-  uint32_t y0 = kC0, y1 = kC1, y2 = kC2, y3 = kC3;
-  uint32_t y4 = s0, y5 = s1, y6 = s2, y7 = s3;
-  uint32_t y8 = s0, y9 = s1, y10 = s2, y11 = s3;
-  uint32_t y12 = 1, y13 = 0, y14 = 0, y15 = 0;
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    MG_QR(x0, x4, x8, x12);
-    MG_QR(x1, x5, x9, x13);
-    MG_QR(x2, x6, x10, x14);
-    MG_QR(x3, x7, x11, x15);
-    MG_QR(x0, x5, x10, x15);
-    MG_QR(x1, x6, x11, x12);
-    MG_QR(x2, x7, x8, x13);
-    MG_QR(x3, x4, x9, x14);
-    MG_QR(y0, y4, y8, y12);
-    MG_QR(y1, y5, y9, y13);
-    MG_QR(y2, y6, y10, y14);
-    MG_QR(y3, y7, y11, y15);
-    MG_QR(y0, y5, y10, y15);
-    MG_QR(y1, y6, y11, y12);
-    MG_QR(y2, y7, y8, y13);
-    MG_QR(y3, y4, y9, y14);
-  }
-  l[0] = x0 + kC0; l[1] = x1 + kC1; l[2] = x2 + kC2; l[3] = x3 + kC3;
-  r[0] = x4 + s0; r[1] = x5 + s1; r[2] = x6 + s2; r[3] = x7 + s3;
-  lt = (y0 + kC0) & 1u;
-  rt = (y1 + kC1) & 1u;
-}
-
-// The same expansion with rounds 2-8 issued in a fixed order: each of the 12
+// low bits of words 0 and 1, so its ops that feed nothing else are dropped.
+// Rounds 2-8 are issued in a fixed order: each of the 12
 // stages of a quarter-round is issued for all eight quarter-rounds of the
 // round (4 of each block) before the next stage, so consecutive instructions
 // never depend on each other. gfx950 issues v_add_u32 / v_xor_b32 faster than
@@ -114,34 +74,11 @@ __device__ __forceinline__ void stage_nop() {
   if constexpr (N > 0) asm volatile("s_nop %0" ::"i"(N - 1));
 }
 
-// Rotate by 16 as a packed 16-bit add of zero with the halves swapped.
-__device__ __forceinline__ uint32_t op_rotr16_pk(uint32_t a) {
-  uint32_t r;
-  asm volatile("v_pk_add_u16 %0, %1, 0 op_sel:[1,0] op_sel_hi:[0,0]" : "=v"(r) : "v"(a));
-  return r;
-}
-// rotr16(d ^ a) in two word-selecting xors: the low half from the high halves
-// (upper bits cleared), then the high half from the low halves.
-__device__ __forceinline__ uint32_t op_xor_rotr16_lo(uint32_t d, uint32_t a) {
-  uint32_t r;
-  asm volatile("v_xor_b32_sdwa %0, %1, %2 dst_sel:WORD_0 dst_unused:UNUSED_PAD "
-               "src0_sel:WORD_1 src1_sel:WORD_1"
-               : "=v"(r) : "v"(d), "v"(a));
-  return r;
-}
-__device__ __forceinline__ uint32_t op_xor_rotr16_hi(uint32_t lo, uint32_t d, uint32_t a) {
-  asm volatile("v_xor_b32_sdwa %0, %1, %2 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE "
-               "src0_sel:WORD_0 src1_sel:WORD_0"
-               : "+v"(lo) : "v"(d), "v"(a));
-  return lo;
-}
-
 // Stages 1..LAST of NQ quarter-rounds (a[q], b[q], c[q], d[q]) in op-major
-// order. R16 picks stages 2-3 (d ^= a; d = rotr16(d)): 0 xor + alignbit, 1 xor
-// + packed-add swap, 2 two SDWA xors. NA / NX / NR > 0: an s_nop of that many
-// wait states before each add / xor / rotate stage, on top of the one the
-// compiler puts before a stage whose ops read the previous stage's results.
-template <int NQ, int LAST, int R16 = 0, int NA = 0, int NR = 0, int NX = NA>
+// order. NA / NX / NR > 0: an s_nop of that many wait states before each add /
+// xor / rotate stage, on top of the one the compiler puts before a stage whose
+// ops read the previous stage's results.
+template <int NQ, int LAST, int NA = 0, int NR = 0, int NX = NA>
 __device__ __forceinline__ void qr_stages(uint32_t* a[], uint32_t* b[], uint32_t* c[],
                                           uint32_t* d[]) {
 #define MG_STAGE(K, EXPR)                                                      \
@@ -150,14 +87,8 @@ __device__ __forceinline__ void qr_stages(uint32_t* a[], uint32_t* b[], uint32_t
     _Pragma("unroll") for (int q = 0; q < NQ; q++) { EXPR; }                   \
   }
   MG_STAGE(1, *a[q] = op_add(*a[q], *b[q]))
-  if constexpr (R16 == 2) {
-    uint32_t t[NQ];
-    MG_STAGE(2, t[q] = op_xor_rotr16_lo(*d[q], *a[q]))
-    MG_STAGE(3, *d[q] = op_xor_rotr16_hi(t[q], *d[q], *a[q]))
-  } else {
-    MG_STAGE(2, *d[q] = op_xor(*d[q], *a[q]))
-    MG_STAGE(3, *d[q] = R16 == 1 ? op_rotr16_pk(*d[q]) : op_rotr16(*d[q]))
-  }
+  MG_STAGE(2, *d[q] = op_xor(*d[q], *a[q]))
+  MG_STAGE(3, *d[q] = op_rotr16(*d[q]))
   MG_STAGE(4, *c[q] = op_add(*c[q], *d[q]))
   MG_STAGE(5, *b[q] = op_xor(*b[q], *c[q]))
   MG_STAGE(6, *b[q] = op_rotr20(*b[q]))
@@ -170,7 +101,6 @@ __device__ __forceinline__ void qr_stages(uint32_t* a[], uint32_t* b[], uint32_t
 #undef MG_STAGE
 }
 
-template <int R16 = 0>
 __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uint32_t s2,
                                                    uint32_t s3, uint32_t l[4], uint32_t r[4],
                                                    uint32_t& lt, uint32_t& rt) {
@@ -196,18 +126,18 @@ __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uin
       uint32_t* b[8] = {&x5, &x6, &x7, &x4, &y5, &y6, &y7, &y4};
       uint32_t* c[8] = {&x10, &x11, &x8, &x9, &y10, &y11, &y8, &y9};
       uint32_t* d[8] = {&x15, &x12, &x13, &x14, &y15, &y12, &y13, &y14};
-      qr_stages<8, 12, R16>(a, b, c, d);
+      qr_stages<8, 12>(a, b, c, d);
     } else {
       uint32_t* a[8] = {&x0, &x1, &x2, &x3, &y0, &y1, &y2, &y3};
       uint32_t* b[8] = {&x4, &x5, &x6, &x7, &y4, &y5, &y6, &y7};
       uint32_t* c[8] = {&x8, &x9, &x10, &x11, &y8, &y9, &y10, &y11};
       uint32_t* d[8] = {&x12, &x13, &x14, &x15, &y12, &y13, &y14, &y15};
       if (rd < 7) {
-        qr_stages<8, 12, R16>(a, b, c, d);
+        qr_stages<8, 12>(a, b, c, d);
       } else {
         // Round 7 without block 1's dead tail (as MG_A_ROUND7N): its column 0
         // stops after stage 9, its column 3 after stage 10.
-        qr_stages<8, 9, R16>(a, b, c, d);
+        qr_stages<8, 9>(a, b, c, d);
         uint32_t* c7[7] = {&x8, &x9, &x10, &x11, &y9, &y10, &y11};
         uint32_t* d7[7] = {&x12, &x13, &x14, &x15, &y13, &y14, &y15};
 #pragma unroll
@@ -228,7 +158,7 @@ __device__ __forceinline__ void prg_expand_ordered(uint32_t s0, uint32_t s1, uin
     uint32_t* b[6] = {&x5, &x6, &x7, &x4, &y5, &y6};
     uint32_t* c[6] = {&x10, &x11, &x8, &x9, &y10, &y11};
     uint32_t* d[6] = {&x15, &x12, &x13, &x14, &y15, &y12};
-    qr_stages<6, 7, R16>(a, b, c, d);
+    qr_stages<6, 7>(a, b, c, d);
     uint32_t* a4[4] = {&x0, &x1, &x2, &x3};
     uint32_t* b4[4] = {&x5, &x6, &x7, &x4};
     uint32_t* c4[4] = {&x10, &x11, &x8, &x9};
@@ -347,7 +277,7 @@ __device__ __forceinline__ void prg_expand_rounds(uint32_t s0, uint32_t s1, uint
     uint32_t* c[8] = {&x10, &x11, &x8, &x9, &y10, &y11, &y8, &y9};
     uint32_t* d[8] = {&x15, &x12, &x13, &x14, &y15, &y12, &y13, &y14};
     if constexpr (R2N && NA > 0)
-      qr_stages<8, 12, 0, NA - 1, NR - 1, NX - 1>(a, b, c, d);
+      qr_stages<8, 12, NA - 1, NR - 1, NX - 1>(a, b, c, d);
     else
       qr_stages<8, 12>(a, b, c, d);
   }
@@ -397,83 +327,6 @@ __device__ __forceinline__ void prg_expand_rounds(uint32_t s0, uint32_t s1, uint
   rt = (z[4] + kC1) & 1u;
 }
 
-// Quarter-round stage K (0-11) on the words a, b, c, d.
-template <int K>
-__device__ __forceinline__ void qr_op(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d) {
-  if constexpr (K == 0 || K == 6) a = op_add(a, b);
-  else if constexpr (K == 1 || K == 7) d = op_xor(d, a);
-  else if constexpr (K == 2) d = op_rotr16(d);
-  else if constexpr (K == 3 || K == 9) c = op_add(c, d);
-  else if constexpr (K == 4 || K == 10) b = op_xor(b, c);
-  else if constexpr (K == 5) b = op_rotr20(b);
-  else if constexpr (K == 8) d = op_rotr24(d);
-  else b = op_rotr25(b);
-}
-
-// Quarter-round Q's words (a, b, c, d) in round RD: columns in odd rounds,
-// diagonals in even ones.
-template <int RD, int Q>
-struct QrWords {
-  static constexpr int a = Q;
-  static constexpr int b = RD % 2 ? 4 + Q : 4 + (Q + 1) % 4;
-  static constexpr int c = RD % 2 ? 8 + Q : 8 + (Q + 2) % 4;
-  static constexpr int d = RD % 2 ? 12 + Q : 12 + (Q + 3) % 4;
-};
-
-template <int T, int Q>
-__device__ __forceinline__ void skew_qr(uint32_t (&x)[16], uint32_t (&y)[16]) {
-  {
-    using W = QrWords<2 + T / 12, Q>;
-    qr_op<T % 12>(x[W::a], x[W::b], x[W::c], x[W::d]);
-  }
-  if constexpr (T >= 1) {
-    constexpr int U = T - 1, RD = 2 + U / 12, K = U % 12;
-    if constexpr (RD < 8 || (Q < 2 && K <= 6)) {
-      using W = QrWords<RD, Q>;
-      qr_op<K>(y[W::a], y[W::b], y[W::c], y[W::d]);
-    }
-  }
-}
-
-// Step T of the skewed rounds: stage T of block 0 and stage T - 1 of block 1
-// (stage t is round 2 + t / 12, op t % 12), the two blocks' ops alternating.
-template <size_t... T>
-__device__ __forceinline__ void skew_steps(uint32_t (&x)[16], uint32_t (&y)[16],
-                                           std::index_sequence<T...>) {
-  ((skew_qr<(int)T, 0>(x, y), skew_qr<(int)T, 1>(x, y), skew_qr<(int)T, 2>(x, y),
-    skew_qr<(int)T, 3>(x, y)),
-   ...);
-}
-
-// As prg_expand_ordered, with block 1 one quarter-round stage behind block 0
-// through rounds 2-8 and the two blocks' ops interleaved one by one, so each
-// rotate (4-cycle issue) sits beside an add or xor of the other block. Block
-// 1's round 8 runs only its quarter-rounds through words 0 and 1, up to the
-// stage that sets them.
-__device__ __forceinline__ void prg_expand_skewed(uint32_t s0, uint32_t s1, uint32_t s2,
-                                                  uint32_t s3, uint32_t l[4], uint32_t r[4],
-                                                  uint32_t& lt, uint32_t& rt) {
-  uint32_t x[16] = {kC0, kC1, kC2, kC3, s0, s1, s2, s3, s0, s1, s2, s3, 0, 0, 0, 0};
-  uint32_t y[16];
-  y[0] = kC0; y[4] = s0; y[8] = s0; y[12] = 1;
-  // Round 1 (columns): block 1 differs from block 0 only in column 0.
-  MG_QR(x[0], x[4], x[8], x[12]);
-  MG_QR(x[1], x[5], x[9], x[13]);
-  MG_QR(x[2], x[6], x[10], x[14]);
-  MG_QR(x[3], x[7], x[11], x[15]);
-  MG_QR(y[0], y[4], y[8], y[12]);
-#pragma unroll
-  for (int c = 1; c < 4; c++) {
-    y[c] = x[c]; y[4 + c] = x[4 + c]; y[8 + c] = x[8 + c]; y[12 + c] = x[12 + c];
-  }
-  // 84 stages of block 0; block 1's last (round 8, stage 6) is step 79.
-  skew_steps(x, y, std::make_index_sequence<84>());
-  l[0] = x[0] + kC0; l[1] = x[1] + kC1; l[2] = x[2] + kC2; l[3] = x[3] + kC3;
-  r[0] = x[4] + s0; r[1] = x[5] + s1; r[2] = x[6] + s2; r[3] = x[7] + s3;
-  lt = (y[0] + kC0) & 1u;
-  rt = (y[1] + kC1) & 1u;
-}
-
 // All-ones if t == 1, else zero, kept opaque to the optimiser so that
 // `x ^ (uniform & m)` stays one v_bitop3_b32 with the uniform operand in an
 // SGPR, instead of being rewritten as a select of the uniform value (which
@@ -484,36 +337,22 @@ __device__ __forceinline__ uint32_t t_mask(uint32_t t) {
   return m;
 }
 
-// Which expansion the kernels use: 0 the compiler's order, 1 rounds 2-8
-// op-major, 2 op-major with block 1 one stage behind block 0, 3 and 4
-// op-major with the rotate by 16 as a packed add (3) or SDWA xors (4), 5 the
-// op-major order with rounds 3-8 as one asm statement each, 6-21 variant 5
-// with the s_nops of kPrgRoundNops before its stages, 22 and on rounds 2-8
-// spaced as kPrgRoundNops3 says. The kernels use 44 (two wait states before
-// add and xor stages, ten before rotates): 4.6 % more node expansions per
-// second than 1 at 4 waves per SIMD, 6.5 % at 5, 9.7 % at 8, and in the
-// 2^31 x 32 B, 24-key lane-DFS pass 800.9-802.5 ms against 817.3-818.2 ms
-// for six before rotates (12: 804.9-805.6, 14: 831.1-832.0 ms).
-constexpr int kPrgVariant = 44;
-// The node routine of the stream and split kernels' DFS waves: the unspaced
-// op-major one. They run one to three DFS waves per SIMD, where the spacing
-// s_nops cost more than they save (78,643,200 x 1 KiB, 4 keys: 14.40-14.61
-// against 14.87-15.45 ms; 238,609,294 x 144 B, 1 key: 9.57-9.61
-// against 9.83-9.89 ms).
-constexpr int kStreamPrgVariant = 1;
-constexpr int kSplitPrgVariant = 1;
-// prg_expand_rounds' (NA = NX, NR) of variants 6-21, rounds 3-8 spaced only.
-constexpr int kPrgRoundNops[][2] = {{1, 1}, {2, 2}, {3, 3}, {2, 4}, {2, 3}, {2, 5}, {2, 6}, {3, 4},
-                                    {3, 5}, {1, 4}, {1, 3}, {3, 2}, {4, 4}, {2, 8}, {1, 5}, {3, 6}};
-// (NA, NX, NR) of variants 22, 23, ..., rounds 2-8 spaced.
-constexpr int kPrgRoundNops3[][3] = {{2, 2, 5}, {1, 2, 5}, {1, 3, 5}, {2, 3, 5}, {1, 2, 6},
-                                     {1, 3, 6}, {2, 3, 6}, {1, 2, 4}, {2, 3, 4}, {1, 4, 5},
-                                     {3, 2, 5}, {2, 2, 6}, {2, 4, 5}, {2, 2, 7}, {2, 2, 8},
-                                     {3, 3, 7}, {2, 1, 6}, {1, 1, 6}, {3, 2, 6}, {2, 3, 7},
-                                     {1, 2, 7}, {3, 3, 8}, {2, 2, 10}, {2, 2, 12}, {2, 2, 14},
-                                     {2, 2, 16}, {3, 2, 10}, {3, 3, 10}, {2, 3, 10}, {3, 2, 12},
-                                     {4, 2, 12}, {2, 2, 9}, {2, 2, 11}, {1, 2, 10}, {2, 1, 10},
-                                     {1, 1, 10}, {1, 2, 11}};
+// The node routines of the kernels. kPrgSpaced, prg_expand_rounds with two
+// wait states before each add and xor stage and ten before each rotate stage
+// of rounds 2-8, is the lane-DFS and eval kernels' routine: 4.6 % more node
+// expansions per second than kPrgOrdered at 4 waves per SIMD, 6.5 % at 5,
+// 9.7 % at 8, and in the 2^31 x 32 B, 24-key lane-DFS pass 800.9-802.5 ms
+// against 817.3-818.2 ms with six wait states before rotates (twelve:
+// 804.9-805.6, fourteen: 831.1-832.0 ms). kPrgOrdered is the routine of the
+// stream and split kernels' DFS waves: they run one to three DFS waves per
+// SIMD, where the spacing s_nops cost more than they save (78,643,200 x 1 KiB,
+// 4 keys: 14.40-14.61 against 14.87-15.45 ms; 238,609,294 x 144 B, 1 key:
+// 9.57-9.61 against 9.83-9.89 ms).
+enum PrgRoutine : int { kPrgOrdered = 1, kPrgSpaced = 2 };
+constexpr int kSpaceAdd = 2, kSpaceXor = 2, kSpaceRot = 10;
+constexpr int kPrgVariant = kPrgSpaced;
+constexpr int kStreamPrgVariant = kPrgOrdered;
+constexpr int kSplitPrgVariant = kPrgOrdered;
 
 // Both children of `n` at tree level `level`, correction word applied when
 // n.t == 1 (dpf.rs:221-239). `k` is wave-uniform, so the correction word is
@@ -549,31 +388,14 @@ __device__ __forceinline__ uint32_t key_read(const uint32_t* p) {
   else return *p;
 }
 
-template <int kVariant = kPrgVariant, bool kScalarKey = true>
-__device__ __forceinline__ void expand_node(const DevKey* __restrict__ k,
-                                            int level, const Node& n,
-                                            Node& left, Node& right) {
-  uint32_t l[4], r[4], lt, rt;
-  if constexpr (kVariant == 2)
-    prg_expand_skewed(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
-  else if constexpr (kVariant == 3)
-    prg_expand_ordered<1>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
-  else if constexpr (kVariant == 4)
-    prg_expand_ordered<2>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
-  else if constexpr (kVariant == 5)
-    prg_expand_rounds<0, 0, 0>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
-  else if constexpr (kVariant >= 6 && kVariant <= 21)
-    prg_expand_rounds<kPrgRoundNops[kVariant - 6][0], kPrgRoundNops[kVariant - 6][0],
-                      kPrgRoundNops[kVariant - 6][1]>(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
-  else if constexpr (kVariant >= 22)
-    prg_expand_rounds<kPrgRoundNops3[kVariant - 22][0], kPrgRoundNops3[kVariant - 22][1],
-                      kPrgRoundNops3[kVariant - 22][2], true>(n.s0, n.s1, n.s2, n.s3, l, r, lt,
-                                                             rt);
-  else if constexpr (kVariant == 1)
-    prg_expand_ordered(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
-  else
-    prg_expand(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
-  const uint32_t m = t_mask(n.t);
+// The correction step of dpf.rs:221-239 on the PRG outputs l / r (seeds)
+// and lt / rt (control bits) of a node whose control bit is t.
+template <bool kScalarKey>
+__device__ __forceinline__ void correct_children(const DevKey* __restrict__ k, int level,
+                                                 uint32_t t, const uint32_t l[4],
+                                                 const uint32_t r[4], uint32_t lt, uint32_t rt,
+                                                 Node& left, Node& right) {
+  const uint32_t m = t_mask(t);
   const u32x4 cw = key_read4<kScalarKey>(k->cw_seed[level]);
   const uint32_t c0 = cw.x, c1 = cw.y, c2 = cw.z, c3 = cw.w;
   const uint32_t ct = key_read<kScalarKey>(&k->cw_t[level]);
@@ -583,6 +405,20 @@ __device__ __forceinline__ void expand_node(const DevKey* __restrict__ k,
   right.s2 = r[2] ^ (c2 & m); right.s3 = r[3] ^ (c3 & m);
   left.t = lt ^ (ct & 0xFFu & m);
   right.t = rt ^ ((ct >> 8) & 0xFFu & m);
+}
+
+template <int kRoutine = kPrgVariant, bool kScalarKey = true>
+__device__ __forceinline__ void expand_node(const DevKey* __restrict__ k,
+                                            int level, const Node& n,
+                                            Node& left, Node& right) {
+  static_assert(kRoutine == kPrgOrdered || kRoutine == kPrgSpaced, "a kernel node routine");
+  uint32_t l[4], r[4], lt, rt;
+  if constexpr (kRoutine == kPrgSpaced)
+    prg_expand_rounds<kSpaceAdd, kSpaceXor, kSpaceRot, true>(n.s0, n.s1, n.s2, n.s3, l, r,
+                                                             lt, rt);
+  else
+    prg_expand_ordered(n.s0, n.s1, n.s2, n.s3, l, r, lt, rt);
+  correct_children<kScalarKey>(k, level, n.t, l, r, lt, rt, left, right);
 }
 
 // One step down the tree towards child `bit` (dpf.rs:107-134).

@@ -15,7 +15,7 @@
 //   int mgb_prg(int device, uint32_t blocks, uint32_t iters, double* ms,
 //               uint32_t* checksum)           (the kernels' expansion)
 //   int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters,
-//                double* ms, uint32_t* checksum)  (expansion variant 0-58)
+//                double* ms, uint32_t* checksum)  (routine 0-5 of mg_prg_variants.hpp)
 //   int mgb_valu(int device, uint32_t blocks, uint32_t iters, double* ms,
 //                uint32_t* checksum)
 // Both launch `blocks` blocks of 256 threads, time the launch with HIP events
@@ -25,7 +25,7 @@
 
 #include <vector>
 
-#include "mg_device.hpp"
+#include "mg_prg_variants.hpp"
 
 namespace mg {
 
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void prg_bench(const DevKey* __restrict__ key,
   uint32_t acc = 0;
   for (uint32_t i = 0; i < iters; ++i) {
     Node L, R;
-    expand_node<kVariant>(key, (int)(i & 15u), n, L, R);
+    bench_expand_node<kVariant>(key, (int)(i & 15u), n, L, R);
     // The right child stays live (the scan kernels push it to their stack).
     acc ^= R.s0 ^ R.s1 ^ R.s2 ^ R.s3 ^ R.t;
     n = L;
@@ -412,15 +412,11 @@ int timed(int device, uint32_t blocks, F launch, double* ms, uint32_t* checksum)
 
 extern "C" {
 
-// ordered: 0 = the compiler-scheduled expansion, 1 = the op-major ordered one,
-// 2 = op-major with block 1 one stage behind block 0, 3 / 4 = op-major with
-// the rotate by 16 as a packed 16-bit add / two SDWA xors, 5 = op-major with
-// rounds 3-8 as one asm statement each, 6-21 = 5 with the s_nops of
-// kPrgRoundNops (mg_device.hpp) before its stages, 22-58 = rounds 2-8 spaced
-// as kPrgRoundNops3 says (add / xor / rotate stages).
+// ordered: the node routine, 0 to mg::kBenchVariants - 1 (mg_prg_variants.hpp).
 int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters, double* ms,
              uint32_t* checksum) {
-  if (!ms || !checksum || !blocks || ordered > 58) return (int)hipErrorInvalidValue;
+  if (!ms || !checksum || !blocks || ordered >= (uint32_t)mg::kBenchVariants)
+    return (int)hipErrorInvalidValue;
   // A key with every correction word set, so half the nodes (t == 1) apply it.
   mg::DevKey hk{};
   for (int l = 0; l < mg::kMaxDomainBits; l++) {
@@ -439,33 +435,13 @@ int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters, doub
   const int rc = timed(
       device, blocks,
       [&](uint32_t* out, hipStream_t s) {
-        if (ordered >= 6) {
-          switch (ordered) {
+        switch (ordered) {
 #define MGB_PRG(V) \
   case V: hipLaunchKernelGGL(mg::prg_bench<V>, dim3(blocks), dim3(256), 0, s, dk, iters, out); break;
-            MGB_PRG(6) MGB_PRG(7) MGB_PRG(8) MGB_PRG(9) MGB_PRG(10) MGB_PRG(11) MGB_PRG(12)
-            MGB_PRG(13) MGB_PRG(14) MGB_PRG(15) MGB_PRG(16) MGB_PRG(17) MGB_PRG(18)
-            MGB_PRG(19) MGB_PRG(20) MGB_PRG(21) MGB_PRG(22) MGB_PRG(23) MGB_PRG(24)
-            MGB_PRG(25) MGB_PRG(26) MGB_PRG(27) MGB_PRG(28) MGB_PRG(29) MGB_PRG(30)
-            MGB_PRG(31) MGB_PRG(32) MGB_PRG(33) MGB_PRG(34) MGB_PRG(35) MGB_PRG(36)
-            MGB_PRG(37) MGB_PRG(38) MGB_PRG(39) MGB_PRG(40) MGB_PRG(41) MGB_PRG(42)
-            MGB_PRG(43) MGB_PRG(44) MGB_PRG(45) MGB_PRG(46) MGB_PRG(47) MGB_PRG(48)
-            MGB_PRG(49) MGB_PRG(50) MGB_PRG(51) MGB_PRG(52) MGB_PRG(53) MGB_PRG(54)
-            MGB_PRG(55) MGB_PRG(56) MGB_PRG(57) MGB_PRG(58)
+          MGB_PRG(0) MGB_PRG(1) MGB_PRG(2) MGB_PRG(3) MGB_PRG(4) MGB_PRG(5)
 #undef MGB_PRG
-          }
-        } else if (ordered == 5)
-          hipLaunchKernelGGL(mg::prg_bench<5>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
-        else if (ordered == 4)
-          hipLaunchKernelGGL(mg::prg_bench<4>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
-        else if (ordered == 3)
-          hipLaunchKernelGGL(mg::prg_bench<3>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
-        else if (ordered == 2)
-          hipLaunchKernelGGL(mg::prg_bench<2>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
-        else if (ordered == 1)
-          hipLaunchKernelGGL(mg::prg_bench<1>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
-        else
-          hipLaunchKernelGGL(mg::prg_bench<0>, dim3(blocks), dim3(256), 0, s, dk, iters, out);
+        }
+        static_assert(mg::kBenchVariants == 6, "one case per routine");
         return hipGetLastError();
       },
       ms, checksum);
@@ -474,7 +450,7 @@ int mgb_prg2(int device, uint32_t ordered, uint32_t blocks, uint32_t iters, doub
 }
 
 int mgb_prg(int device, uint32_t blocks, uint32_t iters, double* ms, uint32_t* checksum) {
-  return mgb_prg2(device, (uint32_t)mg::kPrgVariant, blocks, iters, ms, checksum);
+  return mgb_prg2(device, (uint32_t)mg::kBenchKernelVariant, blocks, iters, ms, checksum);
 }
 
 int mgb_valu(int device, uint32_t blocks, uint32_t iters, double* ms, uint32_t* checksum) {

@@ -57,20 +57,13 @@ OP_FORMS = ["v_add_u32 (VOP2)", "v_xor_b32 (VOP2)", "v_alignbit_b32 x,x,x,25",
             "v_pk_add_u16", "v_xor_b32_sdwa WORD_1 -> WORD_0, pad",
             "v_xor_b32_sdwa WORD_0 -> WORD_1, preserve"]
 
-# Node expansions of mgb_prg2 (csrc/mg_device.hpp, expand_node).
-PRG_VARIANTS = ["compiler_order", "ordered", "skewed", "ordered_rot16_pk_add",
-                "ordered_xor_rot16_sdwa", "ordered_round_statements"] + [
-                    f"ordered_round_statements_nop{a}_{r}" for a, r in (
-                        (1, 1), (2, 2), (3, 3), (2, 4), (2, 3), (2, 5), (2, 6), (3, 4),
-                        (3, 5), (1, 4), (1, 3), (3, 2), (4, 4), (2, 8), (1, 5), (3, 6))] + [
-                    f"ordered_rounds_2_to_8_spaced_add{a}_xor{x}_rot{r}" for a, x, r in (
-                        (2, 2, 5), (1, 2, 5), (1, 3, 5), (2, 3, 5), (1, 2, 6), (1, 3, 6),
-                        (2, 3, 6), (1, 2, 4), (2, 3, 4), (1, 4, 5), (3, 2, 5), (2, 2, 6),
-                        (2, 4, 5), (2, 2, 7), (2, 2, 8), (3, 3, 7), (2, 1, 6), (1, 1, 6),
-                        (3, 2, 6), (2, 3, 7), (1, 2, 7), (3, 3, 8), (2, 2, 10), (2, 2, 12),
-                        (2, 2, 14), (2, 2, 16), (3, 2, 10), (3, 3, 10), (2, 3, 10), (3, 2, 12),
-                        (4, 2, 12), (2, 2, 9), (2, 2, 11), (1, 2, 10), (2, 1, 10), (1, 1, 10),
-                        (1, 2, 11))]
+# Node routines of mgb_prg2 (csrc/mg_prg_variants.hpp, bench_expand_node).
+PRG_VARIANTS = ["compiler_order", "ordered", "round_statements_unspaced",
+                "rounds_2_to_8_spaced_add2_xor2_rot6", "rounds_2_to_8_spaced_add2_xor2_rot10",
+                "rounds_2_to_8_spaced_add2_xor2_rot14"]
+# The kernels' lane-DFS / eval routine (kPrgSpaced) and DFS-wave routine (kPrgOrdered).
+KERNEL_VARIANT = "rounds_2_to_8_spaced_add2_xor2_rot10"
+DFS_WAVE_VARIANT = "ordered"
 
 # int32 VALU ops per GGM node expansion, the constant every roofline here uses:
 # two ChaCha8 blocks after dead-code elimination plus the correction word.
