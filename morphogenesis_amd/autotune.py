@@ -1,8 +1,9 @@
 """Launch-parameter autotuner for the fused scan.
 
 Times the fused scan kernel (HIP events on its stream) over a grid of
-(kernel, lane depth kl, keys per wave, waves per block, waves per key) for one
-matrix shape and key count, and keeps the fastest bit-exact setting. Every
+(kernel, lane depth kl, keys per wave, waves per block; lane-DFS row-load
+placement; split-kernel column halves and mask-ring depth) for one matrix shape
+and key count, and keeps the fastest bit-exact setting. Every
 candidate's answers are compared with the first candidate's, so a setting
 that changes a single byte is rejected.
 
@@ -26,7 +27,8 @@ from . import (KERNEL_AUTO, KERNEL_LANE_DFS, KERNEL_STREAM, KERNEL_STREAM_SPLIT,
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 # The make_opts fields a stored setting holds.
-SETTING_KEYS = ("kernel", "lane_depth", "keys_per_wave", "waves_per_block", "waves_per_key")
+SETTING_KEYS = ("kernel", "lane_depth", "keys_per_wave", "waves_per_block", "waves_per_key",
+                "stack_global", "late_loads", "row_halves", "mask_depth")
 
 
 def keys_bucket(n_keys: int) -> int:
@@ -74,6 +76,11 @@ def candidates(row_bytes: int, n_keys: int):
     if chunks <= 8:
         for kl, wpb in itertools.product((7, 8, 9, 10, 11, 12), (1, 2, 4)):
             out.append(dict(kernel=KERNEL_LANE_DFS, lane_depth=kl, waves_per_block=wpb))
+        # Rows loaded before or after a leaf pair's expansions (the planner picks one by
+        # register pressure).
+        for kl, late in itertools.product((8, 9, 10, 11), (False, True)):
+            out.append(dict(kernel=KERNEL_LANE_DFS, lane_depth=kl, waves_per_block=4,
+                            late_loads=late))
     if chunks <= 256:
         gmax = 4 if chunks <= 64 else (3 if chunks <= 128 else 2)
         for kl, g, wpb in itertools.product((4, 5, 6, 7, 8), range(1, gmax + 1), (1, 2)):
@@ -81,14 +88,22 @@ def candidates(row_bytes: int, n_keys: int):
                 continue
             out.append(dict(kernel=KERNEL_STREAM, lane_depth=kl, keys_per_wave=g,
                             waves_per_block=wpb))
-        # Split blocks of 1 or 4 producer/consumer pairs (2 or 8 waves); their
-        # streaming waves hold up to 4 keys' accumulators at 2 KiB rows, 3 at 4 KiB.
-        gmax_split = 4 if chunks <= 128 else 3
-        for kl, g, wpb in itertools.product((5, 6, 7, 8), range(1, gmax_split + 1), (2, 8)):
-            if g > n_keys:
-                continue
-            out.append(dict(kernel=KERNEL_STREAM_SPLIT, lane_depth=kl, keys_per_wave=g,
-                            waves_per_block=wpb))
+        # Split blocks of 1 or 4 producer/consumer pairs (2 or 8 waves); their streaming
+        # waves hold up to 4 keys' accumulators at 2 KiB rows (or per column half of a
+        # wider row), 3 at 4 KiB whole rows. Mask ring: the default depth or 2 steps.
+        for halves in ((None,) if chunks <= 64 else (False, True)):
+            gmax_split = 4 if (chunks <= 128 or halves) else 3
+            for kl, g, wpb, md in itertools.product((5, 6, 7, 8, 9), range(1, gmax_split + 1),
+                                                    (2, 8), (0, 2)):
+                if g > n_keys:
+                    continue
+                c = dict(kernel=KERNEL_STREAM_SPLIT, lane_depth=kl, keys_per_wave=g,
+                         waves_per_block=wpb)
+                if halves is not None:
+                    c["row_halves"] = halves
+                if md:
+                    c["mask_depth"] = md
+                out.append(c)
     return out
 
 
@@ -102,27 +117,42 @@ def tune(scanner, matrix, keys, reps=2, grid=None, log=print, warmup=2):
     grid = grid or candidates(matrix.row_bytes, len(keys))
     for _ in range(warmup):
         scanner.scan_keys(matrix, keys)
+
+    def timed(c):
+        opts = make_opts(**c)
+        ans, t = scanner.scan_keys(matrix, keys, opts, with_timing=True)
+        times = [t["scan_kernel_ms"]]
+        for _ in range(reps - 1):
+            ans2, t2 = scanner.scan_keys(matrix, keys, opts, with_timing=True)
+            times.append(t2["scan_kernel_ms"])
+            if not np.array_equal(ans, ans2):
+                raise RuntimeError("non-deterministic answers")
+        return ans, min(times), t["waves"]
+
     ref = None
     rows = []
     for c in grid:
         try:
-            opts = make_opts(**c)
-            ans, t = scanner.scan_keys(matrix, keys, opts, with_timing=True)
-            times = [t["scan_kernel_ms"]]
-            for _ in range(reps - 1):
-                ans2, t2 = scanner.scan_keys(matrix, keys, opts, with_timing=True)
-                times.append(t2["scan_kernel_ms"])
-                if not np.array_equal(ans, ans2):
-                    raise RuntimeError("non-deterministic answers")
+            ans, ms, waves = timed(c)
         except Exception as e:  # a setting the device cannot launch is skipped
             log(f"skip {c}: {e}")
             continue
         if ref is None:
             ref = ans
         exact = bool(np.array_equal(ans, ref))
-        ms = min(times)
-        rows.append(dict(c, ms=ms, exact=exact, waves=t["waves"]))
+        rows.append(dict(c, ms=ms, exact=exact, waves=waves))
         log(f"{c} -> {ms:.3f} ms{'' if exact else '  MISMATCH'}")
+    # The first candidate is timed again last: the first passes after a matrix is
+    # written can still run slow (78,643,200 x 1 KiB, 3 keys: the default setting 12.6 ms
+    # first, 11.9 ms for the same launch later), which would count against it.
+    if rows and len(grid) > 1:
+        try:
+            ans, ms, _ = timed({k: v for k, v in rows[0].items()
+                                if k not in ("ms", "exact", "waves")})
+            rows[0]["ms"] = min(rows[0]["ms"], ms)
+            rows[0]["exact"] = rows[0]["exact"] and bool(np.array_equal(ans, ref))
+        except Exception as e:
+            log(f"re-timing {grid[0]} failed: {e}")
     good = [r for r in rows if r["exact"]]
     best = min(good, key=lambda r: r["ms"]) if good else None
     return best, rows

@@ -41,11 +41,20 @@ def test_candidates_cover_each_kernel_for_its_widths(at):
     assert any(c["kernel"] == mg.KERNEL_STREAM_SPLIT and c["keys_per_wave"] == 3 for c in wide)
     # Never more keys per wave than keys, nor than the width allows.
     assert all(c.get("keys_per_wave", 1) <= 3 for c in wide)
-    assert all(c.get("keys_per_wave", 1) <= (3 if c["kernel"] == mg.KERNEL_STREAM_SPLIT else 2)
-               for c in at.candidates(4096, 8))
+    assert all(c.get("keys_per_wave", 1) <= (
+        (4 if c.get("row_halves") else 3) if c["kernel"] == mg.KERNEL_STREAM_SPLIT else 2)
+        for c in at.candidates(4096, 8))
+    # Column halves only where rows are wider than 1 KiB; both ways there.
+    assert not any("row_halves" in c for c in wide)
+    halves = {c.get("row_halves") for c in at.candidates(2048, 4)
+              if c["kernel"] == mg.KERNEL_STREAM_SPLIT}
+    assert halves == {False, True}
+    assert any(c.get("mask_depth") == 2 for c in wide)
+    assert {c.get("late_loads") for c in narrow if c["kernel"] == mg.KERNEL_LANE_DFS} == {
+        None, False, True}
     assert any(c.get("keys_per_wave") == 3 for c in at.candidates(4096, 8))
     assert any(c.get("keys_per_wave") == 4 for c in at.candidates(2048, 8))
     # Every stored field is one make_opts accepts.
-    for c in narrow + wide:
+    for c in narrow + wide + at.candidates(4096, 8):
         assert set(c) <= set(at.SETTING_KEYS)
         mg.make_opts(**c)

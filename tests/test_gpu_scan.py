@@ -156,7 +156,8 @@ def split_pair_bytes(g, kl):
 
 @pytest.mark.parametrize("n,w,d", [(3000, 1024, 12), (5000, 2048, 13), (777, 4096, 10),
                                    (2500, 272, 12), (600, 32, 10), (70_001, 1024, 17),
-                                   (3001, 3088, 12), (9000, 4096, 14)])
+                                   (3001, 3088, 12), (9000, 4096, 14), (2100, 1040, 12),
+                                   (1800, 1552, 11)])
 def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
     # Producer/consumer blocks: masks from a DFS wave, rows from a streaming wave.
     print(f"case split n={n} w={w} d={d}", flush=True)
@@ -211,7 +212,9 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
                                         keys_per_wave=g, waves_per_key=wpk, waves_per_block=wpb)
                     got = scanner.scan_keys(m, keys, opts)
                     assert np.array_equal(got, ref), (g, kl, wpk, wpb)
-    # Rows over 1 KiB in two column halves, one pair each, up to 4 keys per pair.
+    # Rows over 1 KiB in two column halves, one pair each, up to 4 keys per pair. At
+    # 1040 and 1552 B the second half is ragged (1 and 33 of its 64 chunks), at 3088 B
+    # ragged with two chunks per lane.
     for g in range(1, 5):
         for kl in sorted({3, d - 6}):
             for wpb in (0, 2, 4, 8):
