@@ -7,6 +7,15 @@
  * non-zero MG_* code is returned and mg_last_error() describes it (per
  * calling thread).
  *
+ * Threading (the reference shares one GpuScanner across threads with a
+ * ticket-based pool of workspace slots, kernel.rs:50-72, 733-739): every
+ * entry point is safe to call from many threads. Scans and key evaluations
+ * take one of the device's scan workspaces (a HIP stream with its own
+ * buffers; mg_device_set_workspaces) and run concurrently with each other on
+ * distinct streams. Writes to a matrix or delta (upload, fill, copy, apply,
+ * set, destroy) wait for the scans in flight and hold new ones back while
+ * they run, so a scan never sees a matrix half written.
+ *
  * Each entry point names the reference interface it replaces
  * (igor53627/morphogenesis, paths relative to the repository root). The
  * reference's page leaves (4096 B) and 25-bit domain cap are generalised to
@@ -133,6 +142,12 @@ int mg_device_open(int ordinal, mg_device **out);
 /* MG_ERR_STATE (nothing freed) while matrices, deltas or request loops
  * opened on the device are still open. */
 int mg_device_close(mg_device *dev);
+/* Scan workspaces the device's calls spread over, 1-8 (default 4: the
+ * reference pool's default and cap, kernel.rs:50-65). A call takes a free
+ * one, or waits for one when all are busy; mg_scan_per_query deals its
+ * queries over the first n. Workspaces beyond n keep their buffers. */
+#define MG_MAX_WORKSPACES 8u
+int mg_device_set_workspaces(mg_device *dev, uint32_t n);
 /* Free/total HBM and compute-unit count of the device. */
 int mg_device_info(mg_device *dev, uint64_t *free_bytes, uint64_t *total_bytes,
                    uint32_t *compute_units, char *name, size_t name_len);
@@ -255,8 +270,9 @@ int mg_scan(mg_device *dev, mg_matrix *m, const uint8_t *keys,
  * mg_device_open. The call allocates nothing. A scan that needs more than
  * the array, or more than MG_NOALLOC_MAX_KEYS keys, is refused with
  * MG_ERR_INVALID_ARG; use_graph is refused too (instantiating a graph
- * allocates). Calls through one mg_device serialise on its lock; calls
- * through two handles opened on the same GPU share the array and race. */
+ * allocates). Calls through one mg_device serialise on its control stream's
+ * lock (and with matrix and delta writes); calls through two handles opened
+ * on the same GPU share the array and race. */
 #define MG_NOALLOC_SCRATCH_BYTES (64u << 20)
 #define MG_NOALLOC_MAX_KEYS 1024u
 int mg_scan_noalloc(mg_device *dev, mg_matrix *m, const uint8_t *keys,
@@ -265,7 +281,11 @@ int mg_scan_noalloc(mg_device *dev, mg_matrix *m, const uint8_t *keys,
 
 /* scan_batch_single_query_multistream_optimized(_with_graph)
  * (kernel.rs:955-1097): one pass per query of keys_per_query keys, the
- * reference's "plain" serving mode; same answers as mg_scan. */
+ * reference's "plain" serving mode; same answers as mg_scan. The queries are
+ * dealt round-robin over the device's workspaces, each on its own stream
+ * (the reference's forked streams, kernel.rs:1024-1037), so their launches,
+ * key uploads and answer downloads overlap. timing: kernel, copy and scan
+ * times summed over the queries, total_ms the call's wall time. */
 int mg_scan_per_query(mg_device *dev, mg_matrix *m, const uint8_t *keys,
                       const size_t *key_lens, uint32_t n_keys,
                       uint32_t keys_per_query, uint8_t *out,

@@ -160,7 +160,7 @@ class GpuPageMatrix {
 };
 
 // GpuScanner (kernel.rs:158-171): one device. Safe to share across threads;
-// the library serialises work per device.
+// scans from several threads run concurrently on the device's workspaces.
 class GpuScanner {
  public:
   explicit GpuScanner(int device_ord = 0) { check(mg_device_open(device_ord, &dev_), "GpuScanner::new"); }
@@ -170,6 +170,8 @@ class GpuScanner {
     if (dev_) mg_device_close(dev_);
   }
   mg_device* raw() const { return dev_; }
+  // Workspace slots (streams) the device's calls spread over, 1-8 (kernel.rs:50-65).
+  void set_workspaces(uint32_t n) { check(mg_device_set_workspaces(dev_, n), "set_workspaces"); }
 
   using Seed128 = std::array<uint8_t, 16>;  // a leaf: four little-endian words
   // ChaChaKey::eval (dpf.rs:103-142), on this device.

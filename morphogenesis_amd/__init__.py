@@ -146,6 +146,7 @@ def _load():
         "mg_last_error": (ctypes.c_char_p, []),
         "mg_device_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
         "mg_device_close": (ctypes.c_int, [vp]),
+        "mg_device_set_workspaces": (ctypes.c_int, [vp, ctypes.c_uint32]),
         "mg_device_info": (ctypes.c_int, [vp, u64p, u64p, ctypes.POINTER(ctypes.c_uint32),
                                           ctypes.c_char_p, ctypes.c_size_t]),
         "mg_matrix_create": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32,
@@ -308,8 +309,9 @@ def make_opts(kernel=KERNEL_AUTO, lane_depth=0, keys_per_wave=0, waves_per_key=0
 
 
 class GpuScanner:
-    """GpuScanner (kernel.rs:158-171): one gfx950 device, its stream and
-    workspace."""
+    """GpuScanner (kernel.rs:158-171): one gfx950 device and its pool of scan
+    workspaces (a stream with its own buffers each); calls from several threads
+    run concurrently on distinct ones."""
 
     def __init__(self, device_ord: int = 0):
         h = ctypes.c_void_p()
@@ -329,6 +331,10 @@ class GpuScanner:
             self.close()
         except Exception:
             pass
+
+    def set_workspaces(self, n: int):
+        """Scan workspaces (streams) calls spread over, 1-8 (default 4)."""
+        _check(lib.mg_device_set_workspaces(self._h, n))
 
     def info(self):
         fr, tot = ctypes.c_uint64(), ctypes.c_uint64()

@@ -7,7 +7,9 @@
 
 #include <atomic>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -93,24 +95,77 @@ struct ScanBuffers {
   uint4* gstack = nullptr;            // >= plan.parts * n_keys * plan.gslots * 1 KiB
 };
 
+// One scan workspace: a stream of its own with the device buffers, pinned
+// staging, events and graph cache that one call at a time uses on it
+// (the role of a slot of the reference's ticket-based workspace pool,
+// crates/morphogen-gpu-dpf/src/kernel.rs:50-72, 733-739). Calls on distinct
+// workspaces run concurrently on their own streams.
+struct Workspace {
+  int index = 0;
+  std::mutex mu;  // held by the call using this workspace
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[5] = {};
+  DevBuf keys, partial, partial_delta, out;
+  DevBuf gstack;  // lane-DFS stack levels kept in global memory
+  DevBuf fault;   // the split kernel's hand-over timeout word for launches on `stream`
+  HostBuf h_keys, h_out;
+  // HIP graph of the kernel sequence for the last plan run with use_graph.
+  bool graph_valid = false;
+  Plan graph_plan;
+  hipGraphExec_t graph_exec = nullptr;
+  // Creates the stream, events and fault word on first use (device current).
+  int ready();
+  void invalidate_graph();
+  void release();  // frees everything; the stream is synchronised first
+};
+
+// Readers-writer lock that lets a waiting writer in: a writer holds `turn_` while it waits
+// for the readers in flight to leave, so new readers queue behind it instead of keeping it
+// out for good. Works with std::shared_lock (readers) and std::unique_lock (writers).
+class ScanGate {
+ public:
+  void lock() {
+    std::lock_guard<std::mutex> t(turn_);
+    rw_.lock();
+  }
+  void unlock() { rw_.unlock(); }
+  void lock_shared() {
+    { std::lock_guard<std::mutex> t(turn_); }
+    rw_.lock_shared();
+  }
+  void unlock_shared() { rw_.unlock_shared(); }
+
+ private:
+  std::mutex turn_;
+  std::shared_mutex rw_;
+};
+
+// Scan workspaces a device holds at most, and by default.
+constexpr uint32_t kMaxWorkspaces = MG_MAX_WORKSPACES;
+constexpr uint32_t kDefaultWorkspaces = 4;
+
 }  // namespace mg
 
 struct mg_device {
   int ordinal = 0;
   int cus = 0;
   size_t max_lds = 65536;  // dynamic LDS bytes one block may use
+  // Control stream: matrix and delta writes, request-loop steps and the no-alloc
+  // scan run here, each under `mu`.
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {};
-  std::mutex mu;  // serialises every use of `stream` and the workspace
-  mg::DevBuf keys, partial, partial_delta, out;
-  mg::DevBuf gstack;  // lane-DFS stack levels kept in global memory
+  std::mutex mu;
+  // Scans hold this shared for their whole call; writers of matrices and deltas (and a
+  // request loop merging into the matrix it serves) hold it exclusively, after `mu`.
+  mg::ScanGate state;
+  // Scan workspaces (kMaxWorkspaces, streams created on first use); calls pick one of
+  // the first `n_workspaces` by ticket.
+  mg::Workspace ws[mg::kMaxWorkspaces];
+  std::atomic<uint32_t> n_workspaces{mg::kDefaultWorkspaces};
+  std::atomic<uint32_t> ticket{0};
   mg::DevBuf zero_rows;  // 4 KiB of zeros, read in place of rows past the end
-  mg::DevBuf fault;      // one word the split kernel sets if a hand-over wait runs out
-  mg::HostBuf h_keys, h_out;
-  // HIP graph of the kernel sequence for the last plan.
-  bool graph_valid = false;
-  mg::Plan graph_plan;
-  hipGraphExec_t graph_exec = nullptr;
+  mg::DevBuf fault;      // the control stream's split-kernel timeout word
+  std::mutex occ_mu;     // guards `occupancy`
   std::map<std::pair<uint32_t, uint32_t>, int> occupancy;  // (variant, lds) -> blocks/CU
   // mg_scan_noalloc: this device's address of the library's fixed scratch
   // array (set at open) and host room for MG_NOALLOC_MAX_KEYS parsed keys.
@@ -158,26 +213,55 @@ int parse_keys(const uint8_t* keys, const size_t* key_lens, uint32_t n_keys, Dev
 int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_bytes,
               uint32_t n_keys, uint32_t d, const mg_scan_opts* o, Plan* pl);
 
-// Enqueues scan kernel, optional delta kernel and reductions on dev->stream.
-// Events (may be null) bracket the fused scan kernel.
 // Bytes of the lane-DFS global stack levels a plan's launch needs (0 if none).
 inline size_t lane_gstack_bytes(const Plan& pl, uint32_t n_keys) {
   return (size_t)pl.parts * n_keys * pl.gslots * 64 * 16;
 }
 
-int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_t ev_scan0,
-                 hipEvent_t ev_scan1);
+// Where a launch sequence goes: the stream, and the word a split-kernel launch on it
+// sets when a hand-over wait runs out.
+struct Exec {
+  hipStream_t stream = nullptr;
+  uint32_t* fault = nullptr;
+};
+inline Exec control_exec(mg_device* dev) { return Exec{dev->stream, (uint32_t*)dev->fault.p}; }
+inline Exec ws_exec(Workspace* w) { return Exec{w->stream, (uint32_t*)w->fault.p}; }
+
+// Enqueues scan kernel, optional delta kernel and reductions on x.stream.
+// Events (may be null) bracket the fused scan kernel.
+int enqueue_scan(mg_device* dev, const Exec& x, const Plan& pl, const ScanBuffers& b,
+                 hipEvent_t ev_scan0, hipEvent_t ev_scan1);
 
 // Sets the plan's delta fields from opts->delta after checking it against
 // the matrix and device.
 int attach_delta(mg_device* dev, const mg_matrix* m, const mg_scan_opts* opts, Plan* pl);
 
-// Fills *t from dev->ev[0..4] (H2D, scan kernel, reductions, D2H) and the plan.
-int fill_timing(mg_device* dev, const Plan& pl, double t0, mg_timing* t);
+// Fills *t from ev[0..4] (H2D, scan kernel, reductions, D2H) and the plan.
+int fill_timing(const hipEvent_t ev[5], const Plan& pl, double t0, mg_timing* t);
 
-// After a stream sync: MG_ERR_DEVICE (and the word cleared) if a split-kernel launch
-// since the last check reported a hand-over wait that ran out; a no-op for other plans.
-int check_fault(mg_device* dev, const Plan& pl);
+// After x.stream has been synchronised: MG_ERR_DEVICE (and the word cleared) if a
+// split-kernel launch since the last check reported a hand-over wait that ran out; a
+// no-op for other plans.
+int check_fault(const Exec& x, const Plan& pl);
+
+// A workspace held for one call: the next free one of the device's first
+// n_workspaces (by ticket), or, when all are busy, the ticket's one once free.
+class WorkspaceLease {
+ public:
+  explicit WorkspaceLease(mg_device* dev);
+  // Workspace `i`, waiting for it if busy.
+  WorkspaceLease(mg_device* dev, uint32_t i);
+  Workspace* operator->() const { return w_; }
+  Workspace* get() const { return w_; }
+
+ private:
+  Workspace* w_ = nullptr;
+  std::unique_lock<std::mutex> lk_;
+};
+
+// Drops every workspace's graph (they bake in matrix and delta addresses). Caller holds
+// dev->state exclusively.
+void invalidate_graphs(mg_device* dev);
 
 // Resolves this device's address of the no-alloc scratch array (mg_noalloc.hip);
 // called by mg_device_open with the device current.

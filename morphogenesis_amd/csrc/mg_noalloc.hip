@@ -117,12 +117,13 @@ extern "C" int mg_scan_noalloc(mg_device* dev, mg_matrix* m, const uint8_t* keys
   MG_HIP(hipEventRecord(dev->ev[0], s));
   MG_HIP(hipMemcpyAsync(base + l.keys, dev->noalloc_keys.data(), sizeof(mg::DevKey) * n_keys,
                         hipMemcpyHostToDevice, s));
-  if ((rc = mg::enqueue_scan(dev, pl, b, dev->ev[1], dev->ev[2]))) return rc;
+  if ((rc = mg::enqueue_scan(dev, mg::control_exec(dev), pl, b, dev->ev[1], dev->ev[2])))
+    return rc;
   MG_HIP(hipEventRecord(dev->ev[3], s));
   MG_HIP(hipMemcpyAsync(out, base + l.out, out_bytes, hipMemcpyDeviceToHost, s));
   MG_HIP(hipEventRecord(dev->ev[4], s));
   MG_HIP(hipStreamSynchronize(s));
-  if ((rc = mg::check_fault(dev, pl))) return rc;
-  if (timing) return mg::fill_timing(dev, pl, t0, timing);
+  if ((rc = mg::check_fault(mg::control_exec(dev), pl))) return rc;
+  if (timing) return mg::fill_timing(dev->ev, pl, t0, timing);
   return MG_OK;
 }

@@ -1,9 +1,11 @@
 // Host runtime behind include/morphogen_hip.h: device, matrix, key, delta and
 // scan entry points. It plays the role of the reference's GpuScanner /
 // GpuPageMatrix host side (crates/morphogen-gpu-dpf/src/kernel.rs:156-1211,
-// src/storage.rs:18-145) on one gfx950 GPU: one non-blocking HIP stream per
-// device, a grow-only workspace, pinned staging buffers, HIP-event timing and
-// an optional HIP-graph replay of the launch sequence.
+// src/storage.rs:18-145) on one gfx950 GPU: a pool of scan workspaces, each a
+// non-blocking HIP stream with grow-only device buffers, pinned staging,
+// HIP-event timing and an optional HIP-graph replay of the launch sequence,
+// so calls from several threads run on distinct streams; a control stream for
+// matrix and delta writes.
 #include <algorithm>
 #include <chrono>
 #include <cstdarg>
@@ -90,6 +92,80 @@ Plan::Plan() {
 
 bool Plan::operator==(const Plan& o) const { return memcmp(this, &o, sizeof(Plan)) == 0; }
 
+int Workspace::ready() {
+  if (stream) return MG_OK;
+  hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    stream = nullptr;
+    return hip_fail(e, "hipStreamCreateWithFlags(workspace)");
+  }
+  int rc = MG_OK;
+  for (auto& v : ev) {
+    if ((e = hipEventCreate(&v)) != hipSuccess) {
+      v = nullptr;
+      rc = hip_fail(e, "hipEventCreate(workspace)");
+      break;
+    }
+  }
+  if (!rc) rc = fault.reserve(16);
+  if (!rc && (launch_zero((uint4*)fault.p, 1, stream) != hipSuccess ||
+              hipStreamSynchronize(stream) != hipSuccess))
+    rc = fail(MG_ERR_DEVICE, "zeroing a workspace's fault word failed");
+  if (rc) release();
+  return rc;
+}
+
+void Workspace::invalidate_graph() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  graph_exec = nullptr;
+  graph_valid = false;
+}
+
+void Workspace::release() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  invalidate_graph();
+  keys.release();
+  partial.release();
+  partial_delta.release();
+  out.release();
+  gstack.release();
+  fault.release();
+  h_keys.release();
+  h_out.release();
+  for (auto& v : ev) {
+    if (v) (void)hipEventDestroy(v);
+    v = nullptr;
+  }
+  if (stream) (void)hipStreamDestroy(stream);
+  stream = nullptr;
+}
+
+WorkspaceLease::WorkspaceLease(mg_device* dev) {
+  const uint32_t n = std::max(1u, std::min(dev->n_workspaces.load(), kMaxWorkspaces));
+  const uint32_t t = dev->ticket.fetch_add(1) % n;
+  for (uint32_t j = 0; j < n; ++j) {
+    Workspace* w = &dev->ws[(t + j) % n];
+    std::unique_lock<std::mutex> lk(w->mu, std::try_to_lock);
+    if (lk.owns_lock()) {
+      w_ = w;
+      lk_ = std::move(lk);
+      return;
+    }
+  }
+  w_ = &dev->ws[t];
+  lk_ = std::unique_lock<std::mutex>(w_->mu);
+}
+
+WorkspaceLease::WorkspaceLease(mg_device* dev, uint32_t i)
+    : w_(&dev->ws[i]), lk_(dev->ws[i].mu) {}
+
+void invalidate_graphs(mg_device* dev) {
+  for (Workspace& w : dev->ws) {
+    std::lock_guard<std::mutex> lk(w.mu);
+    w.invalidate_graph();
+  }
+}
+
 int set_device(mg_device* dev) {
   hipError_t e = hipSetDevice(dev->ordinal);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
@@ -148,6 +224,7 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chun
   auto key = std::make_pair(((kind * 8 + (uint32_t)PK) * 8 + (uint32_t)G) * 64 +
                                 (uint32_t)waves_per_block,
                             (uint32_t)lds);
+  std::lock_guard<std::mutex> olk(dev->occ_mu);
   auto it = dev->occupancy.find(key);
   int blocks = 0;
   if (it != dev->occupancy.end()) {
@@ -609,9 +686,9 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
   return MG_OK;
 }
 
-int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_t ev_scan0,
-                 hipEvent_t ev_scan1) {
-  hipStream_t s = dev->stream;
+int enqueue_scan(mg_device* dev, const Exec& x, const Plan& pl, const ScanBuffers& b,
+                 hipEvent_t ev_scan0, hipEvent_t ev_scan1) {
+  hipStream_t s = x.stream;
   if (ev_scan0) MG_HIP(hipEventRecord(ev_scan0, s));
   const bool any_rows = pl.row_end > pl.row_begin;
   if (any_rows) {
@@ -653,7 +730,7 @@ int enqueue_scan(mg_device* dev, const Plan& pl, const ScanBuffers& b, hipEvent_
       if (pl.kernel == MG_KERNEL_STREAM_SPLIT) {
         p.col_splits = pl.col_splits;
         p.mask_depth = pl.mask_depth;
-        p.fault = (uint32_t*)dev->fault.p;
+        p.fault = x.fault;
         MG_HIP(launch_split(split_s(&pl), (int)pl.keys_per_wave,
                             (int)pl.waves_per_block / 2, p, s));
       } else {
@@ -726,23 +803,25 @@ int attach_delta(mg_device* dev, const mg_matrix* m, const mg_scan_opts* opts, P
   return MG_OK;
 }
 
-int check_fault(mg_device* dev, const Plan& pl) {
+int check_fault(const Exec& x, const Plan& pl) {
   if (pl.kernel != MG_KERNEL_STREAM_SPLIT) return MG_OK;
   uint32_t word = 0;
-  MG_HIP(hipMemcpy(&word, dev->fault.p, sizeof word, hipMemcpyDeviceToHost));
+  MG_HIP(hipMemcpyAsync(&word, x.fault, sizeof word, hipMemcpyDeviceToHost, x.stream));
+  MG_HIP(hipStreamSynchronize(x.stream));
   if (!word) return MG_OK;
   const uint32_t zero = 0;
-  MG_HIP(hipMemcpy(dev->fault.p, &zero, sizeof zero, hipMemcpyHostToDevice));
+  MG_HIP(hipMemcpyAsync(x.fault, &zero, sizeof zero, hipMemcpyHostToDevice, x.stream));
+  MG_HIP(hipStreamSynchronize(x.stream));
   return fail(MG_ERR_DEVICE, "split kernel: a DFS/streaming hand-over wait ran out; answers "
                              "of this call are void");
 }
 
-int fill_timing(mg_device* dev, const Plan& pl, double t0, mg_timing* t) {
+int fill_timing(const hipEvent_t ev[5], const Plan& pl, double t0, mg_timing* t) {
   float h2d = 0, scan = 0, kern = 0, d2h = 0;
-  MG_HIP(hipEventElapsedTime(&h2d, dev->ev[0], dev->ev[1]));
-  MG_HIP(hipEventElapsedTime(&scan, dev->ev[1], dev->ev[2]));
-  MG_HIP(hipEventElapsedTime(&kern, dev->ev[1], dev->ev[3]));
-  MG_HIP(hipEventElapsedTime(&d2h, dev->ev[3], dev->ev[4]));
+  MG_HIP(hipEventElapsedTime(&h2d, ev[0], ev[1]));
+  MG_HIP(hipEventElapsedTime(&scan, ev[1], ev[2]));
+  MG_HIP(hipEventElapsedTime(&kern, ev[1], ev[3]));
+  MG_HIP(hipEventElapsedTime(&d2h, ev[3], ev[4]));
   t->h2d_ms = h2d;
   t->scan_kernel_ms = scan;
   t->kernel_ms = kern;
@@ -769,93 +848,110 @@ using mg::fail;
 
 namespace {
 
-mg::ScanBuffers device_buffers(mg_device* dev) {
+mg::ScanBuffers workspace_buffers(const mg::Workspace* w) {
   mg::ScanBuffers b;
-  b.keys = (const DevKey*)dev->keys.p;
-  b.partial = (uint4*)dev->partial.p;
-  b.partial_delta = (uint4*)dev->partial_delta.p;
-  b.out = (uint4*)dev->out.p;
-  b.gstack = (uint4*)dev->gstack.p;
+  b.keys = (const DevKey*)w->keys.p;
+  b.partial = (uint4*)w->partial.p;
+  b.partial_delta = (uint4*)w->partial_delta.p;
+  b.out = (uint4*)w->out.p;
+  b.gstack = (uint4*)w->gstack.p;
   return b;
 }
 
-void invalidate_graph(mg_device* dev) {
-  if (dev->graph_exec) (void)hipGraphExecDestroy(dev->graph_exec);
-  dev->graph_exec = nullptr;
-  dev->graph_valid = false;
-}
-
-int run_plan_graph(mg_device* dev, const mg::Plan& pl) {
-  if (!(dev->graph_valid && dev->graph_plan == pl)) {
-    invalidate_graph(dev);
-    MG_HIP(hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal));
-    int rc = mg::enqueue_scan(dev, pl, device_buffers(dev), nullptr, nullptr);
+int run_plan_graph(mg_device* dev, mg::Workspace* w, const mg::Plan& pl) {
+  if (!(w->graph_valid && w->graph_plan == pl)) {
+    w->invalidate_graph();
+    MG_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+    int rc = mg::enqueue_scan(dev, mg::ws_exec(w), pl, workspace_buffers(w), nullptr, nullptr);
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(dev->stream, &g);
+    hipError_t e = hipStreamEndCapture(w->stream, &g);
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
     }
     if (e != hipSuccess) return mg::hip_fail(e, "hipStreamEndCapture");
-    e = hipGraphInstantiate(&dev->graph_exec, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&w->graph_exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return mg::hip_fail(e, "hipGraphInstantiate");
-    dev->graph_plan = pl;
-    dev->graph_valid = true;
+    if (e != hipSuccess) {
+      w->graph_exec = nullptr;
+      return mg::hip_fail(e, "hipGraphInstantiate");
+    }
+    w->graph_plan = pl;
+    w->graph_valid = true;
   }
-  MG_HIP(hipGraphLaunch(dev->graph_exec, dev->stream));
+  MG_HIP(hipGraphLaunch(w->graph_exec, w->stream));
   return MG_OK;
 }
 
-// One scan of n_keys parsed keys (already in dev->h_keys) into `out`.
-int scan_locked(mg_device* dev, mg_matrix* m, uint32_t n_keys, uint32_t d, uint8_t* out,
-                const mg_scan_opts* opts, mg_timing* t) {
-  const double t0 = mg::now_ms();
+// A scan enqueued on a workspace, waiting for scan_finish.
+struct PendingScan {
   mg::Plan pl;
+  double t0 = 0;
+  size_t out_bytes = 0;
+  uint8_t* out = nullptr;
+};
+
+// Parses n_keys wire keys into the workspace's pinned staging.
+int stage_keys(mg::Workspace* w, const uint8_t* keys, const size_t* key_lens, uint32_t n_keys,
+               uint32_t* d) {
+  int rc = w->h_keys.reserve(sizeof(DevKey) * std::max<uint32_t>(n_keys, 1));
+  if (rc) return rc;
+  return mg::parse_keys(keys, key_lens, n_keys, (DevKey*)w->h_keys.p, d);
+}
+
+// Enqueues one scan of the n_keys keys staged in w->h_keys on w's stream: key upload,
+// kernels, answer download into w->h_out. Nothing waits for it.
+int scan_enqueue(mg_device* dev, mg::Workspace* w, mg_matrix* m, uint32_t n_keys, uint32_t d,
+                 uint8_t* out, const mg_scan_opts* opts, PendingScan* ps) {
+  ps->t0 = mg::now_ms();
+  mg::Plan& pl = ps->pl;
+  pl = mg::Plan();
   int rc = mg::make_plan(dev, m->rows, m->n_rows, m->row_bytes, n_keys, d, opts, &pl);
   if (rc) return rc;
   if ((rc = mg::attach_delta(dev, m, opts, &pl))) return rc;
   const size_t out_bytes = (size_t)n_keys * m->row_bytes;
-  if ((rc = dev->keys.reserve(sizeof(DevKey) * n_keys))) return rc;
-  if ((rc = dev->out.reserve(out_bytes))) return rc;
-  if ((rc = dev->partial.reserve((size_t)pl.parts * n_keys * m->row_bytes))) return rc;
-  if (pl.gslots && (rc = dev->gstack.reserve(lane_gstack_bytes(pl, n_keys)))) return rc;
+  if ((rc = w->keys.reserve(sizeof(DevKey) * n_keys))) return rc;
+  if ((rc = w->out.reserve(out_bytes))) return rc;
+  if ((rc = w->partial.reserve((size_t)pl.parts * n_keys * m->row_bytes))) return rc;
+  if (pl.gslots && (rc = w->gstack.reserve(mg::lane_gstack_bytes(pl, n_keys)))) return rc;
   if (pl.n_delta &&
-      (rc = dev->partial_delta.reserve((size_t)pl.delta_parts * n_keys * m->row_bytes)))
+      (rc = w->partial_delta.reserve((size_t)pl.delta_parts * n_keys * m->row_bytes)))
     return rc;
-  if ((rc = dev->h_out.reserve(out_bytes))) return rc;
-  pl.ws_keys = dev->keys.p;
-  pl.ws_out = dev->out.p;
-  pl.ws_partial = dev->partial.p;
-  pl.ws_partial_delta = pl.n_delta ? dev->partial_delta.p : nullptr;
-  pl.ws_gstack = pl.gslots ? dev->gstack.p : nullptr;
+  if ((rc = w->h_out.reserve(out_bytes))) return rc;
+  pl.ws_keys = w->keys.p;
+  pl.ws_out = w->out.p;
+  pl.ws_partial = w->partial.p;
+  pl.ws_partial_delta = pl.n_delta ? w->partial_delta.p : nullptr;
+  pl.ws_gstack = pl.gslots ? w->gstack.p : nullptr;
 
-  hipStream_t s = dev->stream;
-  MG_HIP(hipEventRecord(dev->ev[0], s));
-  MG_HIP(hipMemcpyAsync(dev->keys.p, dev->h_keys.p, sizeof(DevKey) * n_keys,
+  hipStream_t s = w->stream;
+  MG_HIP(hipEventRecord(w->ev[0], s));
+  MG_HIP(hipMemcpyAsync(w->keys.p, w->h_keys.p, sizeof(DevKey) * n_keys,
                         hipMemcpyHostToDevice, s));
   if (opts && opts->use_graph) {
-    MG_HIP(hipEventRecord(dev->ev[1], s));
-    if ((rc = run_plan_graph(dev, pl))) return rc;
-    MG_HIP(hipEventRecord(dev->ev[2], s));
+    MG_HIP(hipEventRecord(w->ev[1], s));
+    if ((rc = run_plan_graph(dev, w, pl))) return rc;
+    MG_HIP(hipEventRecord(w->ev[2], s));
   } else {
-    if ((rc = mg::enqueue_scan(dev, pl, device_buffers(dev), dev->ev[1], dev->ev[2]))) return rc;
+    rc = mg::enqueue_scan(dev, mg::ws_exec(w), pl, workspace_buffers(w), w->ev[1], w->ev[2]);
+    if (rc) return rc;
   }
-  MG_HIP(hipEventRecord(dev->ev[3], s));
-  MG_HIP(hipMemcpyAsync(dev->h_out.p, dev->out.p, out_bytes, hipMemcpyDeviceToHost, s));
-  MG_HIP(hipEventRecord(dev->ev[4], s));
-  MG_HIP(hipStreamSynchronize(s));
-  if ((rc = mg::check_fault(dev, pl))) return rc;
-  memcpy(out, dev->h_out.p, out_bytes);
-  if (t) return mg::fill_timing(dev, pl, t0, t);
+  MG_HIP(hipEventRecord(w->ev[3], s));
+  MG_HIP(hipMemcpyAsync(w->h_out.p, w->out.p, out_bytes, hipMemcpyDeviceToHost, s));
+  MG_HIP(hipEventRecord(w->ev[4], s));
+  ps->out_bytes = out_bytes;
+  ps->out = out;
   return MG_OK;
 }
 
-int stage_keys(mg_device* dev, const uint8_t* keys, const size_t* key_lens, uint32_t n_keys,
-               uint32_t* d) {
-  int rc = dev->h_keys.reserve(sizeof(DevKey) * std::max<uint32_t>(n_keys, 1));
+// Waits for a scan_enqueue'd scan and copies its answers out.
+int scan_finish(mg::Workspace* w, const PendingScan& ps, mg_timing* t) {
+  MG_HIP(hipStreamSynchronize(w->stream));
+  int rc = mg::check_fault(mg::ws_exec(w), ps.pl);
   if (rc) return rc;
-  return mg::parse_keys(keys, key_lens, n_keys, (DevKey*)dev->h_keys.p, d);
+  memcpy(ps.out, w->h_out.p, ps.out_bytes);
+  if (t) return mg::fill_timing(w->ev, ps.pl, ps.t0, t);
+  return MG_OK;
 }
 
 }  // namespace
@@ -919,20 +1015,24 @@ int mg_device_close(mg_device* dev) {
     return fail(MG_ERR_STATE, "%d matrices, deltas or request loops still use this device", n);
   (void)hipSetDevice(dev->ordinal);
   if (dev->stream) (void)hipStreamSynchronize(dev->stream);
-  invalidate_graph(dev);
-  dev->keys.release();
-  dev->partial.release();
-  dev->partial_delta.release();
-  dev->out.release();
-  dev->gstack.release();
+  for (mg::Workspace& w : dev->ws) {
+    std::lock_guard<std::mutex> lk(w.mu);
+    w.release();
+  }
   dev->zero_rows.release();
   dev->fault.release();
-  dev->h_keys.release();
-  dev->h_out.release();
   for (auto& ev : dev->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (dev->stream) (void)hipStreamDestroy(dev->stream);
   delete dev;
+  return MG_OK;
+}
+
+int mg_device_set_workspaces(mg_device* dev, uint32_t n) {
+  if (!dev) return fail(MG_ERR_INVALID_ARG, "device is null");
+  if (n < 1 || n > mg::kMaxWorkspaces)
+    return fail(MG_ERR_INVALID_ARG, "%u workspaces: 1 to %u", n, mg::kMaxWorkspaces);
+  dev->n_workspaces = n;
   return MG_OK;
 }
 
@@ -991,6 +1091,7 @@ int mg_matrix_upload(mg_matrix* m, uint64_t first_row, const void* rows, uint64_
                 (unsigned long long)first_row, (unsigned long long)n_rows,
                 (unsigned long long)m->n_rows);
   std::lock_guard<std::mutex> lk(m->dev->mu);
+  std::unique_lock<mg::ScanGate> wlk(m->dev->state);
   int rc = mg::set_device(m->dev);
   if (rc) return rc;
   MG_HIP(hipMemcpyAsync((uint8_t*)m->rows + first_row * m->row_bytes, rows,
@@ -1017,6 +1118,7 @@ int mg_matrix_download(mg_matrix* m, uint64_t first_row, void* rows, uint64_t n_
 int mg_matrix_fill_seeded(mg_matrix* m, uint64_t seed) {
   if (!m) return fail(MG_ERR_INVALID_ARG, "matrix is null");
   std::lock_guard<std::mutex> lk(m->dev->mu);
+  std::unique_lock<mg::ScanGate> wlk(m->dev->state);
   int rc = mg::set_device(m->dev);
   if (rc) return rc;
   const uint64_t pairs = (uint64_t)m->n_rows * m->row_bytes / 16;
@@ -1028,6 +1130,7 @@ int mg_matrix_fill_seeded(mg_matrix* m, uint64_t seed) {
 int mg_matrix_fill_pattern(mg_matrix* m, uint64_t seed) {
   if (!m) return fail(MG_ERR_INVALID_ARG, "matrix is null");
   std::lock_guard<std::mutex> lk(m->dev->mu);
+  std::unique_lock<mg::ScanGate> wlk(m->dev->state);
   int rc = mg::set_device(m->dev);
   if (rc) return rc;
   const uint64_t blocks = (uint64_t)m->n_rows * m->row_bytes / 16;
@@ -1049,6 +1152,7 @@ int mg_matrix_copy(mg_matrix* dst, const mg_matrix* src) {
   if (dst->n_rows != src->n_rows || dst->row_bytes != src->row_bytes)
     return fail(MG_ERR_ROWS, "matrix shapes differ");
   std::lock_guard<std::mutex> lk(dst->dev->mu);
+  std::unique_lock<mg::ScanGate> wlk(dst->dev->state);
   int rc = mg::set_device(dst->dev);
   if (rc) return rc;
   const uint64_t n = dst->n_rows * dst->row_bytes / 16;
@@ -1061,13 +1165,15 @@ int mg_matrix_destroy(mg_matrix* m) {
   if (!m) return MG_OK;
   if (const int n = m->users.load())
     return fail(MG_ERR_STATE, "%d request loops still serve this matrix", n);
-  m->dev->children--;
-  std::lock_guard<std::mutex> lk(m->dev->mu);
-  (void)mg::set_device(m->dev);
-  (void)hipStreamSynchronize(m->dev->stream);
-  invalidate_graph(m->dev);
+  mg_device* dev = m->dev;
+  std::lock_guard<std::mutex> lk(dev->mu);
+  std::unique_lock<mg::ScanGate> wlk(dev->state);
+  (void)mg::set_device(dev);
+  (void)hipStreamSynchronize(dev->stream);
+  mg::invalidate_graphs(dev);
   (void)hipFree(m->rows);
   delete m;
+  dev->children--;
   return MG_OK;
 }
 
@@ -1096,39 +1202,84 @@ int mg_scan(mg_device* dev, mg_matrix* m, const uint8_t* keys, const size_t* key
   if (m->dev != dev) return fail(MG_ERR_INVALID_ARG, "matrix belongs to another device");
   if (n_keys == 0) return MG_OK;
   if (!keys || !key_lens || !out) return fail(MG_ERR_INVALID_ARG, "null keys or output");
-  std::lock_guard<std::mutex> lk(dev->mu);
+  std::shared_lock<mg::ScanGate> slk(dev->state);
   int rc = mg::set_device(dev);
   if (rc) return rc;
+  mg::WorkspaceLease w(dev);
+  if ((rc = w->ready())) return rc;
   uint32_t d = 0;
-  if ((rc = stage_keys(dev, keys, key_lens, n_keys, &d))) return rc;
-  return scan_locked(dev, m, n_keys, d, out, opts, timing);
+  if ((rc = stage_keys(w.get(), keys, key_lens, n_keys, &d))) return rc;
+  PendingScan ps;
+  rc = scan_enqueue(dev, w.get(), m, n_keys, d, out, opts, &ps);
+  if (rc) {
+    (void)hipStreamSynchronize(w->stream);  // nothing may still use the staging buffers
+    return rc;
+  }
+  return scan_finish(w.get(), ps, timing);
 }
 
 int mg_scan_per_query(mg_device* dev, mg_matrix* m, const uint8_t* keys, const size_t* key_lens,
                       uint32_t n_keys, uint32_t keys_per_query, uint8_t* out,
                       const mg_scan_opts* opts, mg_timing* timing) {
   if (keys_per_query == 0) return fail(MG_ERR_INVALID_ARG, "keys_per_query is 0");
-  if (!m) return fail(MG_ERR_INVALID_ARG, "null matrix");
-  if (!key_lens && n_keys) return fail(MG_ERR_INVALID_ARG, "null key lengths");
+  if (!dev || !m) return fail(MG_ERR_INVALID_ARG, "null device or matrix");
+  if (m->dev != dev) return fail(MG_ERR_INVALID_ARG, "matrix belongs to another device");
+  if (n_keys == 0) return MG_OK;
+  if (!keys || !key_lens || !out) return fail(MG_ERR_INVALID_ARG, "null keys or output");
+  std::shared_lock<mg::ScanGate> slk(dev->state);
+  int rc = mg::set_device(dev);
+  if (rc) return rc;
+  // One launch sequence per query, dealt round-robin over the device's first k
+  // workspaces (taken in index order, so callers doing the same cannot deadlock), each
+  // on its own stream: query q + k is enqueued once query q's answers are out.
+  const uint32_t n_q = (n_keys + keys_per_query - 1) / keys_per_query;
+  const uint32_t k = std::min(std::min(dev->n_workspaces.load(), mg::kMaxWorkspaces), n_q);
+  std::vector<std::unique_ptr<mg::WorkspaceLease>> ws;
+  for (uint32_t j = 0; j < k; j++) {
+    ws.emplace_back(new mg::WorkspaceLease(dev, j));
+    if ((rc = (*ws.back())->ready())) return rc;
+  }
+  std::vector<PendingScan> pend(k);
+  std::vector<bool> busy(k, false);
   mg_timing acc{};
-  size_t off = 0;
-  for (uint32_t q = 0; q < n_keys; q += keys_per_query) {
-    const uint32_t nk = std::min(keys_per_query, n_keys - q);
+  const double t0 = mg::now_ms();
+  auto finish = [&](uint32_t j) -> int {
+    busy[j] = false;
     mg_timing t{};
-    int rc = mg_scan(dev, m, keys + off, key_lens + q, nk, out + (size_t)q * m->row_bytes, opts,
-                     &t);
-    if (rc) return rc;
-    for (uint32_t i = 0; i < nk; i++) off += key_lens[q + i];
+    int r = scan_finish(ws[j]->get(), pend[j], &t);
+    if (r) return r;
     acc.h2d_ms += t.h2d_ms;
     acc.kernel_ms += t.kernel_ms;
     acc.d2h_ms += t.d2h_ms;
-    acc.total_ms += t.total_ms;
     acc.scan_kernel_ms += t.scan_kernel_ms;
     acc.kernel = t.kernel;
     acc.lane_depth = t.lane_depth;
     acc.keys_per_wave = t.keys_per_wave;
+    acc.stack_global = t.stack_global;
     acc.waves = std::max(acc.waves, t.waves);
+    return MG_OK;
+  };
+  size_t off = 0;
+  for (uint32_t q = 0, qi = 0; q < n_keys && !rc; q += keys_per_query, qi++) {
+    const uint32_t j = qi % k;
+    const uint32_t nk = std::min(keys_per_query, n_keys - q);
+    mg::Workspace* w = ws[j]->get();
+    if (busy[j] && (rc = finish(j))) break;
+    uint32_t d = 0;
+    if ((rc = stage_keys(w, keys + off, key_lens + q, nk, &d))) break;
+    for (uint32_t i = 0; i < nk; i++) off += key_lens[q + i];
+    rc = scan_enqueue(dev, w, m, nk, d, out + (size_t)q * m->row_bytes, opts, &pend[j]);
+    busy[j] = true;
   }
+  for (uint32_t j = 0; j < k; j++) {
+    if (!busy[j]) continue;
+    if (rc)
+      (void)hipStreamSynchronize((*ws[j])->stream);  // drain before the staging is reused
+    else
+      rc = finish(j);
+  }
+  if (rc) return rc;
+  acc.total_ms = mg::now_ms() - t0;
   if (timing) *timing = acc;
   return MG_OK;
 }
@@ -1175,6 +1326,7 @@ int mg_delta_set(mg_delta* dl, const uint64_t* row_idx, const uint8_t* diffs, ui
   std::vector<uint8_t> udiff;
   mg::dedupe_delta(row_idx, diffs, n_entries, dl->row_bytes, &uidx, &udiff);
   std::lock_guard<std::mutex> lk(dl->dev->mu);
+  std::unique_lock<mg::ScanGate> wlk(dl->dev->state);
   int rc = mg::set_device(dl->dev);
   if (rc) return rc;
   hipStream_t s = dl->dev->stream;
@@ -1201,16 +1353,18 @@ int mg_delta_info(const mg_delta* dl, uint64_t* n_entries, uint64_t* epoch) {
 
 int mg_delta_destroy(mg_delta* dl) {
   if (!dl) return MG_OK;
-  dl->dev->children--;
-  std::lock_guard<std::mutex> lk(dl->dev->mu);
-  (void)mg::set_device(dl->dev);
-  (void)hipStreamSynchronize(dl->dev->stream);
-  invalidate_graph(dl->dev);
+  mg_device* dev = dl->dev;
+  std::lock_guard<std::mutex> lk(dev->mu);
+  std::unique_lock<mg::ScanGate> wlk(dev->state);
+  (void)mg::set_device(dev);
+  (void)hipStreamSynchronize(dev->stream);
+  mg::invalidate_graphs(dev);
   (void)hipFree(dl->d_idx);
   (void)hipFree(dl->d_diffs);
   (void)hipFree(dl->d_uidx);
   (void)hipFree(dl->d_udiffs);
   delete dl;
+  dev->children--;
   return MG_OK;
 }
 
@@ -1220,23 +1374,25 @@ int mg_scan_delta(mg_device* dev, const mg_delta* dl, const uint8_t* keys, const
   if (dl->dev != dev) return fail(MG_ERR_INVALID_ARG, "delta belongs to another device");
   if (n_keys == 0) return MG_OK;
   if (!keys || !key_lens || !out) return fail(MG_ERR_INVALID_ARG, "null keys or output");
-  std::lock_guard<std::mutex> lk(dev->mu);
+  std::shared_lock<mg::ScanGate> slk(dev->state);
   int rc = mg::set_device(dev);
   if (rc) return rc;
+  mg::WorkspaceLease w(dev);
+  if ((rc = w->ready())) return rc;
   uint32_t d = 0;
-  if ((rc = stage_keys(dev, keys, key_lens, n_keys, &d))) return rc;
+  if ((rc = stage_keys(w.get(), keys, key_lens, n_keys, &d))) return rc;
   if (d < 64 && dl->n && (dl->max_idx >> d) != 0)
     return fail(MG_ERR_ROWS, "delta row %llu outside the 2^%u-leaf domain",
                 (unsigned long long)dl->max_idx, d);
   const uint32_t chunks = dl->row_bytes / 16;
   const size_t out_bytes = (size_t)n_keys * dl->row_bytes;
   const uint32_t parts = (uint32_t)((dl->n + 63) / 64);
-  if ((rc = dev->keys.reserve(sizeof(DevKey) * n_keys))) return rc;
-  if ((rc = dev->out.reserve(out_bytes))) return rc;
-  if ((rc = dev->partial_delta.reserve((size_t)std::max(parts, 1u) * out_bytes))) return rc;
-  if ((rc = dev->h_out.reserve(out_bytes))) return rc;
-  hipStream_t s = dev->stream;
-  MG_HIP(hipMemcpyAsync(dev->keys.p, dev->h_keys.p, sizeof(DevKey) * n_keys,
+  if ((rc = w->keys.reserve(sizeof(DevKey) * n_keys))) return rc;
+  if ((rc = w->out.reserve(out_bytes))) return rc;
+  if ((rc = w->partial_delta.reserve((size_t)std::max(parts, 1u) * out_bytes))) return rc;
+  if ((rc = w->h_out.reserve(out_bytes))) return rc;
+  hipStream_t s = w->stream;
+  MG_HIP(hipMemcpyAsync(w->keys.p, w->h_keys.p, sizeof(DevKey) * n_keys,
                         hipMemcpyHostToDevice, s));
   if (dl->n) {
     mg::PointParams p{};
@@ -1244,16 +1400,16 @@ int mg_scan_delta(mg_device* dev, const mg_delta* dl, const uint8_t* keys, const
     p.row_idx = dl->d_idx;
     p.n_entries = dl->n;
     p.chunks = chunks;
-    p.keys = (const DevKey*)dev->keys.p;
+    p.keys = (const DevKey*)w->keys.p;
     p.n_keys = n_keys;
-    p.partial = (uint4*)dev->partial_delta.p;
+    p.partial = (uint4*)w->partial_delta.p;
     MG_HIP(mg::launch_point(p, s));
   }
-  MG_HIP(mg::launch_reduce((const uint4*)dev->partial_delta.p, parts, n_keys, chunks,
-                           (uint4*)dev->out.p, nullptr, false, s));
-  MG_HIP(hipMemcpyAsync(dev->h_out.p, dev->out.p, out_bytes, hipMemcpyDeviceToHost, s));
+  MG_HIP(mg::launch_reduce((const uint4*)w->partial_delta.p, parts, n_keys, chunks,
+                           (uint4*)w->out.p, nullptr, false, s));
+  MG_HIP(hipMemcpyAsync(w->h_out.p, w->out.p, out_bytes, hipMemcpyDeviceToHost, s));
   MG_HIP(hipStreamSynchronize(s));
-  memcpy(out, dev->h_out.p, out_bytes);
+  memcpy(out, w->h_out.p, out_bytes);
   return MG_OK;
 }
 
@@ -1267,6 +1423,7 @@ int mg_matrix_apply_delta(mg_matrix* m, const mg_delta* dl) {
     return fail(MG_ERR_ROWS, "delta row %llu outside the %llu-row matrix",
                 (unsigned long long)dl->max_idx, (unsigned long long)m->n_rows);
   std::lock_guard<std::mutex> lk(m->dev->mu);
+  std::unique_lock<mg::ScanGate> wlk(m->dev->state);
   int rc = mg::set_device(m->dev);
   if (rc) return rc;
   if (dl->n_unique)

@@ -214,12 +214,16 @@ int try_rebase(mg_server* s) {
   if (s->double_buffer) {
     target = 1 - s->cur_base;
     if (keys_on_base(s, target)) return MG_OK;  // old keys still read it
-    MG_HIP(hipMemcpyAsync(s->base_rows[target], s->base_rows[s->cur_base],
-                          (size_t)s->n_rows * s->row_bytes, hipMemcpyDeviceToDevice, st));
   } else {
     target = s->cur_base;
     if (s->active) return MG_OK;  // merge in place once nothing reads the base
   }
+  // Base 0 is the caller's matrix: scans of it on other streams wait while it changes.
+  std::unique_lock<mg::ScanGate> wlk(s->dev->state, std::defer_lock);
+  if (target == 0) wlk.lock();
+  if (s->double_buffer)
+    MG_HIP(hipMemcpyAsync(s->base_rows[target], s->base_rows[s->cur_base],
+                          (size_t)s->n_rows * s->row_bytes, hipMemcpyDeviceToDevice, st));
   std::vector<uint64_t> uidx;
   std::vector<uint8_t> udiff;
   mg::dedupe_delta(s->c_idx.data(), s->c_diffs.data(), s->c_idx.size(), s->row_bytes, &uidx,
@@ -354,7 +358,7 @@ int sweep_step(mg_server* s, std::vector<uint32_t>* finished) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g = mg_server::StepGraph{};
         MG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        rc = mg::enqueue_scan(s->dev, pl, bufs, nullptr, nullptr);
+        rc = mg::enqueue_scan(s->dev, mg::control_exec(s->dev), pl, bufs, nullptr, nullptr);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(st, &graph);
         if (rc || e != hipSuccess) {
@@ -372,13 +376,13 @@ int sweep_step(mg_server* s, std::vector<uint32_t>* finished) {
       }
       MG_HIP(hipGraphLaunch(g.exec, st));
       s->graph_launches++;
-    } else if ((rc = mg::enqueue_scan(s->dev, pl, bufs, nullptr, nullptr))) {
+    } else if ((rc = mg::enqueue_scan(s->dev, mg::control_exec(s->dev), pl, bufs, nullptr, nullptr))) {
       return rc;
     }
     // The slot list region is reused by the next base's launch only after
     // this one's kernels have read it.
     MG_HIP(hipStreamSynchronize(st));
-    if ((rc = mg::check_fault(s->dev, pl))) return rc;
+    if ((rc = mg::check_fault(mg::control_exec(s->dev), pl))) return rc;
   }
   for (uint32_t i = 0; i < s->cap_keys; i++) {
     Slot& sl = s->slots[i];
@@ -1045,6 +1049,7 @@ int mg_server_destroy(mg_server* s) {
   if (s->thread.joinable()) s->thread.join();
   {
     std::lock_guard<std::mutex> dlk(s->dev->mu);
+    std::unique_lock<mg::ScanGate> wlk(s->dev->state);
     (void)mg::set_device(s->dev);
     (void)hipStreamSynchronize(s->dev->stream);
     // A double-buffered server may have left the newest rows in its spare copy.
