@@ -270,6 +270,67 @@ def config_pages(mg, scanner, seed, key_counts=(1, 3)):
     return out
 
 
+# The reference's own published shapes (4 KiB pages, 3 keys per query) and its H100 figures
+# for them: docs/benchmarks/2026-02-20-task-30-page-gpu-batch-h100-matrix.csv:4 (single
+# query on the 16,817,159-page mainnet matrix), :21 (32 queries on 4 streams), and
+# docs/benchmarks/2026-02-19-task-24-page-gpu-batch-h100.csv:5 (65,536 pages).
+REFERENCE_PAGE_SHAPES = [
+    ("pages_16817159x4096", 16_817_159, 25, {"single_query_ms": 30.24,
+                                             "four_streams_32_queries_ms": 948.0}),
+    ("pages_65536x4096", 65_536, 16, {"single_query_ms": 0.50}),
+]
+
+
+def config_reference_pages(mg, scanner, n, d, ref_h100, seed, n_queries=32):
+    """A matrix of n seeded 4 KiB pages, queries of 3 keys (the reference's page
+    shape): one query per pass, and n_queries queries dealt over the device's scan
+    workspaces (one pass per query, each on its own stream: the reference's
+    multistream mode). Bit-exact by two-party recovery of every target page."""
+    import numpy as np
+
+    import oracle
+    w = 4096
+    m = mg.GpuPageMatrix(scanner, n, w)
+    m.fill_seeded(seed)
+    rng = np.random.default_rng(seed)
+    targets = [int(x) for x in rng.integers(0, n, size=3 * n_queries)]
+    pairs = [mg.generate_keys(d, t, seed=seed + 13 * i) for i, t in enumerate(targets)]
+    keys = [[p[party] for p in pairs] for party in (0, 1)]
+    for _ in range(2):  # the first passes over freshly written rows run slower
+        scanner.scan_keys(m, keys[0][:3])
+    exact = True
+    ans1, kern, wall, t = scan_passes(scanner, m, keys[0][:3], None, 10, warm=1)
+    ans1b = scanner.scan_keys(m, keys[1][:3])
+    for q in range(3):
+        exact &= bool(np.array_equal(ans1[q] ^ ans1b[q], oracle.fill_rows(seed, targets[q], 1, w)[0]))
+    km, pm = statistics.median(kern), statistics.median(wall)
+    out = {"workload": f"{n:,} pages x 4 KiB, {d}-bit keys, 3 keys per query", "rows": n,
+           "row_bytes": w, "domain_bits": d,
+           "single_query": {"kernel_ms": km, "pass_ms": pm, "queries_per_s": 1e3 / pm,
+                            "gbs": n * w / (km * 1e-3) / 1e9, "hbm_frac": hbm_frac(n, w, km),
+                            "kernel": t["kernel"], "lane_depth": t["lane_depth"],
+                            "waves": t["waves"]},
+           "reference_h100": ref_h100}
+    walls = []
+    answers = None
+    for rep in range(3):
+        s0 = time.perf_counter()
+        a0 = scanner.scan_keys_per_query(m, keys[0], 3)
+        walls.append((time.perf_counter() - s0) * 1e3)
+        if rep == 0:
+            answers = a0
+    a1 = scanner.scan_keys_per_query(m, keys[1], 3)
+    for q, tg in enumerate(targets):
+        exact &= bool(np.array_equal(answers[q] ^ a1[q], oracle.fill_rows(seed, tg, 1, w)[0]))
+    bm = statistics.median(walls)
+    out["per_query_streams"] = {"queries": n_queries, "workspaces": 4, "batch_ms": bm,
+                                "queries_per_s": n_queries * 1e3 / bm,
+                                "vs_single_query": (n_queries * 1e3 / bm) / (1e3 / pm)}
+    out["bit_exact"] = exact
+    m.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -558,6 +619,13 @@ def main():
         log(f"2^27: one key {c2['one_key'][0]['kernel_ms']:.2f} ms, pair "
             f"{c2['pair_one_pass']['kernel_ms']:.2f} ms (VALU {c2['pair_one_pass']['valu_frac']:.3f})")
         by_config["pages_78643200x1024"] = config_pages(mg, scanner, 7864)
+        for label, n_pages, d_pages, ref in REFERENCE_PAGE_SHAPES:
+            by_config[label] = config_reference_pages(mg, scanner, n_pages, d_pages, ref,
+                                                      seed=n_pages % 1000 + 1)
+            c = by_config[label]
+            log(f"{label}: single query {c['single_query']['pass_ms']:.3f} ms, "
+                f"{c['per_query_streams']['queries_per_s']:.1f} q/s over streams, "
+                f"exact {c['bit_exact']}")
 
     cpu = None
     if not args.no_cpu:
