@@ -600,18 +600,24 @@ __device__ __forceinline__ bool wait_count(const uint32_t* ctr, uint32_t want) {
   return false;
 }
 
+// Stack slots of a key that the split kernel's DFS wave holds in registers:
+// the top one (depth 1, used once per tile) and the R - 1 bottom ones (depths
+// kl - 1, kl - 2, ..., pushed and popped within a few steps). Groups of four
+// keys hold two more bottom levels in registers, which leaves their four pairs
+// of a block the LDS for a ring of four mask steps at kl = 7, or three at kl = 9.
+__host__ __device__ constexpr int split_reg_slots(int G) { return G >= 4 ? 5 : 3; }
+
 // Stack levels of a depth-kl lane DFS kept in LDS by the split kernel: the
-// kl - 1 levels minus the top one (depth 1, used once per tile) and the two
-// bottom ones (depths kl - 1 and kl - 2, pushed and popped within one to
-// three steps), which the DFS wave holds in registers.
-__host__ __device__ __forceinline__ uint32_t split_lds_levels(uint32_t kl) {
-  return kl > 4 ? kl - 4 : 0;
+// kl - 1 levels minus the register slots.
+__host__ __device__ __forceinline__ uint32_t split_lds_levels(int G, uint32_t kl) {
+  const uint32_t r = (uint32_t)split_reg_slots(G);
+  return kl > r + 1 ? kl - r - 1 : 0;
 }
 
 // LDS of one pair, in uint4 units: G stacks of split_lds_levels x 64 seeds,
 // D mask buffers of G x 128 masks, and one uint4 holding the two counters.
 __host__ __device__ __forceinline__ size_t split_pair_u4(int G, uint32_t kl, uint32_t D) {
-  return (size_t)G * split_lds_levels(kl) * 64 + (size_t)D * G * 128 + 1;
+  return (size_t)G * split_lds_levels(G, kl) * 64 + (size_t)D * G * 128 + 1;
 }
 
 template <int S, int G, int NP>
@@ -649,11 +655,13 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
   const uint32_t D = p.mask_depth;
   // Right children wait at depths 1 .. kl-1 (depth 0 is the lane root,
   // never pushed): a key's stack has kl - 1 slots, slot l holding depth l + 1.
-  // Slots 0, kl - 3 and kl - 2 sit in the DFS wave's registers, the others in
-  // LDS ([G][kl - 4][64] seeds, slot l at level l - 1); the control bytes are
-  // kept as the bit t == 1 in a register per key.
+  // Slot 0 and the R - 1 bottom slots kl - 2, kl - 3, ... sit in the DFS
+  // wave's registers, the others in LDS ([G][kl - 1 - R][64] seeds, slot l at
+  // level l - 1); the control bytes are kept as the bit t == 1 in a register
+  // per key.
+  constexpr int R = split_reg_slots(G);
   const int slots = kl - 1;
-  const int lds_slots = (int)split_lds_levels(p.kl);
+  const int lds_slots = (int)split_lds_levels(G, p.kl);
   uint4* pair_lds = lds + (size_t)pi * split_pair_u4(G, p.kl, D);
   uint4* stacks = pair_lds;                                  // [G][lds_slots][64]
   uint4* masks = pair_lds + (size_t)G * lds_slots * 64;      // [D][G][128]
@@ -683,9 +691,9 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
     const int leaf_level = d - kl;
     Node node[G];
     uint32_t st_tbits[G];
-    // Register slots as plain words: [0] slot 0, [1] slot kl - 2, [2] slot kl - 3 (for
+    // Register slots as plain words: [0] slot 0, [r] slot kl - 1 - r for r >= 1 (for
     // small kl they may name the same slot; each holds what was last pushed there).
-    uint32_t st_reg[G][3][4];
+    uint32_t st_reg[G][R][4];
 #pragma unroll
     for (int g = 0; g < G; g++) st_tbits[g] = 0;
     // Seed of stack slot `slot` (uniform) for key g. The register slots are
@@ -693,18 +701,21 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
     // stay in VGPRs.
     auto put_seed = [&](int g, int slot, const Node& n) __attribute__((always_inline)) {
       const uint32_t v[4] = {n.s0, n.s1, n.s2, n.s3};
-      const bool in_reg[3] = {slot == 0, slot == slots - 1, slot == slots - 2};
+      bool any = false;
 #pragma unroll
-      for (int r = 0; r < 3; r++)
+      for (int r = 0; r < R; r++) {
+        const bool in_reg = slot == (r == 0 ? 0 : slots - r);
+        any |= in_reg;
 #pragma unroll
-        for (int i = 0; i < 4; i++) st_reg[g][r][i] = in_reg[r] ? v[i] : st_reg[g][r][i];
-      if (!in_reg[0] && !in_reg[1] && !in_reg[2])
+        for (int i = 0; i < 4; i++) st_reg[g][r][i] = in_reg ? v[i] : st_reg[g][r][i];
+      }
+      if (!any)
         stacks[((size_t)g * lds_slots + (slot - 1)) * 64 + lane] = make_uint4(v[0], v[1], v[2], v[3]);
     };
     auto get_seed = [&](int g, int slot) __attribute__((always_inline)) -> uint4 {
-      uint32_t a[3][4];
+      uint32_t a[R][4];
 #pragma unroll
-      for (int r = 0; r < 3; r++)
+      for (int r = 0; r < R; r++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           a[r][i] = st_reg[g][r][i];
@@ -712,13 +723,18 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
           // through a selected address would keep the slots in scratch memory.
           asm("" : "+v"(a[r][i]));
         }
-      const int r = slot == 0 ? 0 : (slot == slots - 1 ? 1 : 2);
-      uint4 v = make_uint4(r == 0 ? a[0][0] : (r == 1 ? a[1][0] : a[2][0]),
-                           r == 0 ? a[0][1] : (r == 1 ? a[1][1] : a[2][1]),
-                           r == 0 ? a[0][2] : (r == 1 ? a[1][2] : a[2][2]),
-                           r == 0 ? a[0][3] : (r == 1 ? a[1][3] : a[2][3]));
-      if (slot != 0 && slot != slots - 1 && slot != slots - 2)
-        v = stacks[((size_t)g * lds_slots + (slot - 1)) * 64 + lane];
+      // The register slot holding `slot`, counting from the last one (R - 1 when none).
+      uint32_t w[4] = {a[R - 1][0], a[R - 1][1], a[R - 1][2], a[R - 1][3]};
+      bool any = slot == slots - (R - 1);
+#pragma unroll
+      for (int r = R - 2; r >= 0; r--) {
+        const bool hit = slot == (r == 0 ? 0 : slots - r);
+        any |= hit;
+#pragma unroll
+        for (int i = 0; i < 4; i++) w[i] = hit ? a[r][i] : w[i];
+      }
+      uint4 v = make_uint4(w[0], w[1], w[2], w[3]);
+      if (!any) v = stacks[((size_t)g * lds_slots + (slot - 1)) * 64 + lane];
       return v;
     };
     // The next tile's lane roots, walked one tree level per step while the

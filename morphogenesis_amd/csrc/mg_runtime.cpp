@@ -539,13 +539,16 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     // Split kernel (kernel time, one box, the default mask ring): rows of up to 64 chunks
     // want deep lane subtrees, fewer as a group's keys add stack LDS: 78,643,200 x 1 KiB,
     // 1 key 11.07 / 11.15 / 11.27 ms at kl = 9 / 8 / 7; 2 keys 11.29 / 11.43 ms at kl = 8 / 7;
-    // 3 keys 11.46 / 11.54 / 11.88 ms at kl = 8 / 9 / 6; 4 keys 13.06 / 13.09 ms at kl = 7 / 6.
+    // 3 keys 11.46 / 11.54 / 11.88 ms at kl = 8 / 9 / 6. Four-key groups hold five stack
+    // levels in registers, which leaves room at kl = 9 for a 3-step mask ring: 12.88 /
+    // 13.10 / 13.15 / 13.35 ms at kl = 9 / 8 / 7 / 6 (rings of 3 / 3 / 4 / 4 steps; kl = 9
+    // with 2 steps 13.68 ms).
     // Wider rows stream best at kl = 7 (19,660,800 x 4 KiB, 1 key 12.28 against 12.38 ms at
     // kl = 8; 3 keys 12.27 against 12.30 / 12.37 ms at kl = 6 / 8; 39,321,600 x 2 KiB, 3 keys
     // 12.44 against 12.56 / 12.58 ms at kl = 6 / 8).
     const int S = split_s(pl);
     const int g = o->keys_per_wave ? (int)o->keys_per_wave : default_keys_per_wave(kernel, S, n_keys);
-    kl_default = S > 1 ? 7 : (g <= 2 ? 9 : (g == 3 ? 8 : 7));
+    kl_default = S > 1 ? 7 : (g == 3 ? 8 : 9);
     if (d > 6) kl_default = std::min(kl_default, d - 6);
   } else {
     const int g = o->keys_per_wave ? (int)o->keys_per_wave

@@ -8,7 +8,7 @@ checked without a host copy; the delta contribution follows scan.rs:434-485
 (oracle.scan_delta), applied on top as the reference applies it.
 
 Workloads:
-  78,643,200 rows x 1 KiB, 27-bit keys, 1 key and 3 cuckoo keys per pass;
+  78,643,200 rows x 1 KiB, 27-bit keys, 1 key, 3 cuckoo keys and 4 keys per pass;
   2^31 rows x 32 B plus a 2^16-row epoch delta, 31-bit keys, 3 keys per pass
   and 384 keys (192 pairs) in one pass;
   serving on the 2^31 x 32 B matrix: a Poisson burst of uneven requests with
@@ -66,7 +66,7 @@ def c3_matrix(mg, scanner):
 
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("n_keys", [1, 3])
+@pytest.mark.parametrize("n_keys", [1, 3, 4])
 def test_page_rows_full_size_each_server_equals_oracle(mg, scanner, c3_matrix, n_keys):
     rng = np.random.default_rng(27 + n_keys)
     targets = [int(x) for x in rng.integers(0, C3_ROWS, size=n_keys)]

@@ -149,9 +149,11 @@ def test_stream_kernel_matches_oracle(mg, scanner, n, w, d):
 
 def split_pair_bytes(g, kl):
     """LDS of one split-kernel pair at the smallest mask ring: g stacks of the kl - 1
-    DFS levels less the three held in registers (64 x 16 B each), two 128-mask buffers
-    per key and 16 B of hand-over counters. A block gets 160 KiB of LDS."""
-    return (g * max(0, kl - 4) * 64 + 256 * g + 1) * 16
+    DFS levels less those held in registers (three, five for groups of four keys;
+    64 x 16 B each), two 128-mask buffers per key and 16 B of hand-over counters. A
+    block gets 160 KiB of LDS."""
+    reg = 5 if g >= 4 else 3
+    return (g * max(0, kl - 1 - reg) * 64 + 256 * g + 1) * 16
 
 
 @pytest.mark.parametrize("n,w,d", [(3000, 1024, 12), (5000, 2048, 13), (777, 4096, 10),
