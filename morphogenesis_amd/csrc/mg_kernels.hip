@@ -854,71 +854,100 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
     for (int j = 0; j < S; j++)
       row_off[u][j] =
           (uint32_t)(((((uint64_t)(u >> 1) << kl) + (u & 1)) * C + lane_off[j]) * 16);
-  auto load = [&](uint64_t s, uint32_t b, uint4 (&v)[U][S]) {
-    const uint64_t gs = gs0 + s;
-    const uint64_t tile_base = (p.tile_begin + (gs >> (kl - 1))) << (kl + lb);
-    const uint32_t pair = (uint32_t)(gs & (n_pairs - 1));
-    const uint32_t r0 = b * U;
-    if (tile_base + (1ull << (kl + lb)) <= p.n_rows) {
-      const char* base = reinterpret_cast<const char*>(
-          p.rows + (tile_base + ((uint64_t)(r0 >> 1) << kl) + 2 * (uint64_t)pair) * C);
-#pragma unroll
-      for (int u = 0; u < U; u++)
-#pragma unroll
-        for (int j = 0; j < S; j++)
-          v[u][j] = load_nt(reinterpret_cast<const uint4*>(base + row_off[u][j]));
-      return;
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const uint32_t r = r0 + u;
-      const uint64_t x = tile_base + ((uint64_t)(r >> 1) << kl) + 2 * (uint64_t)pair + (r & 1);
-      const uint4* src = x < p.n_rows ? p.rows + x * C : p.zero_rows;
-#pragma unroll
-      for (int j = 0; j < S; j++) v[u][j] = load_nt(src + lane_off[j]);
-    }
-  };
-
   uint4 acc[G][S];
 #pragma unroll
   for (int g = 0; g < G; g++)
 #pragma unroll
     for (int j = 0; j < S; j++) acc[g][j] = make_uint4(0, 0, 0, 0);
-  uint4 ring[NB][U][S];
-  if (n_steps) {
-#pragma unroll
-    for (int k = 0; k < NB; k++) load(0, (uint32_t)k, ring[k]);
-  }
-  for (uint64_t s = 0; s < n_steps; ++s) {
-    if (!wait_count(published, (uint32_t)(s + 1))) {  // step s's masks are in buffer s % D
-      atomicOr(p.fault, 1u);
-      return;
-    }
-    const uint4* mbuf = masks + (size_t)(s % D) * G * 128;
-    for (uint32_t b0 = 0; b0 < n_batches; b0 += NB) {
-#pragma unroll
-      for (int k = 0; k < NB; k++) {
-        const uint32_t b = b0 + (uint32_t)k;
+
+  // The streaming loop, in two instances: kWhole for runs whose steps all lie
+  // in tiles wholly inside the matrix (every run but one that reaches a ragged
+  // last tile), where a batch's rows sit at fixed distances from one uniform
+  // base; otherwise each row is checked against n_rows and a row past it is
+  // read from the zero buffer. Every batch slot is refilled unconditionally,
+  // past the run's end with the run's last step again (rows in bounds, never
+  // folded), so every path through the loop issues the same loads and the
+  // compiler's wait counts keep all NB batches in flight: with the refill
+  // skipped at the end and the per-row check in the same loop they came out as
+  // waits for all but one or two of the batch's loads.
+  auto consume = [&](auto whole_tag) -> bool {
+    constexpr bool kWhole = decltype(whole_tag)::value;
+    auto load = [&](uint64_t s, uint32_t b, uint4 (&v)[U][S]) {
+      const uint64_t gs = gs0 + s;
+      const uint64_t tile_base = (p.tile_begin + (gs >> (kl - 1))) << (kl + lb);
+      const uint32_t pair = (uint32_t)(gs & (n_pairs - 1));
+      const uint32_t r0 = b * U;
+      if constexpr (kWhole) {
+        const char* base = reinterpret_cast<const char*>(
+            p.rows + (tile_base + ((uint64_t)(r0 >> 1) << kl) + 2 * (uint64_t)pair) * C);
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
-          for (int g = 0; g < G; g++) {
-            const uint4 m = mbuf[g * 128 + b * U + u];
+          for (int j = 0; j < S; j++)
+            v[u][j] = load_nt(reinterpret_cast<const uint4*>(base + row_off[u][j]));
+      } else {
 #pragma unroll
-            for (int j = 0; j < S; j++) xor_masked3(acc[g][j], ring[k][u][j], m);
-          }
-        // Batch b + NB goes into the freed slot: later in this step, or the
-        // start of the next one (batch indices stay congruent mod NB).
-        uint32_t nb = b + NB;
-        uint64_t ns = s;
-        if (nb >= n_batches) {
-          nb -= n_batches;
-          ++ns;
+        for (int u = 0; u < U; u++) {
+          const uint32_t r = r0 + u;
+          const uint64_t x =
+              tile_base + ((uint64_t)(r >> 1) << kl) + 2 * (uint64_t)pair + (r & 1);
+          const uint4* src = x < p.n_rows ? p.rows + x * C : p.zero_rows;
+#pragma unroll
+          for (int j = 0; j < S; j++) v[u][j] = load_nt(src + lane_off[j]);
         }
-        if (ns < n_steps) load(ns, nb, ring[k]);
       }
+    };
+    // The first NB batches go out in batch order, as the loop refills them
+    // (the scheduler would otherwise be free to issue batch 0 last, and the wait
+    // for it at the top of every step would become a wait for every load).
+    uint4 ring[NB][U][S];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      load(0, (uint32_t)k, ring[k]);
+      __builtin_amdgcn_sched_barrier(0);
     }
-    lds_store_release(folded, (uint32_t)(s + 1));  // after this step's mask reads
+    uint32_t buf = 0;  // step s's masks are in buffer s mod D
+    for (uint64_t s = 0; s < n_steps; ++s) {
+      if (!wait_count(published, (uint32_t)(s + 1))) return false;
+      const uint4* mbuf = masks + (size_t)buf * G * 128;
+#pragma unroll
+      for (uint32_t b0 = 0; b0 < n_batches; b0 += NB) {
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+          const uint32_t b = b0 + (uint32_t)k;
+#pragma unroll
+          for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+              const uint4 m = mbuf[g * 128 + b * U + u];
+#pragma unroll
+              for (int j = 0; j < S; j++) xor_masked3(acc[g][j], ring[k][u][j], m);
+            }
+          // Batch b + NB goes into the freed slot: later in this step, or the
+          // start of the next one (batch indices stay congruent mod NB).
+          uint32_t nb = b + NB;
+          uint64_t ns = s;
+          if (nb >= n_batches) {
+            nb -= n_batches;
+            ++ns;
+          }
+          load(ns < n_steps ? ns : n_steps - 1, nb, ring[k]);
+        }
+      }
+      lds_store_release(folded, (uint32_t)(s + 1));  // after this step's mask reads
+      buf = buf + 1 == D ? 0 : buf + 1;
+    }
+    return true;
+  };
+  if (n_steps) {
+    const uint64_t last_tile = p.tile_begin + ((gs0 + n_steps - 1) >> (kl - 1));
+    const bool ok = ((last_tile + 1) << (kl + lb)) <= p.n_rows
+                        ? consume(std::integral_constant<bool, true>{})
+                        : consume(std::integral_constant<bool, false>{});
+    if (!ok) {
+      atomicOr(p.fault, 1u);
+      return;
+    }
   }
 
   if (!live) return;
