@@ -167,12 +167,14 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
 
   // This wave's leaf-pair steps: global steps gs0 .. gs0 + n_steps - 1, step gs
   // being pair gs mod 2^(kl-1) of tile tile_begin + gs / 2^(kl-1).
+  // (A key's steps number N / 2^(lb+1) <= 2^25, so a wave's counts fit 32 bits.)
   const uint32_t n_pairs = 1u << (kl - 1);
   const uint64_t total_steps = (p.tile_end - p.tile_begin) << (kl - 1);
-  const uint64_t gs0 = (uint64_t)part * p.steps_per_wave;
-  uint64_t n_steps = 0;
+  const uint32_t spw = (uint32_t)p.steps_per_wave;
+  const uint64_t gs0 = (uint64_t)part * spw;
+  uint32_t n_steps = 0;
   if (live && gs0 < total_steps)
-    n_steps = total_steps - gs0 < p.steps_per_wave ? total_steps - gs0 : p.steps_per_wave;
+    n_steps = total_steps - gs0 < spw ? (uint32_t)(total_steps - gs0) : spw;
   const bool lane_active = lane < (1 << lb);
   const int leaf_level = d - kl;
 
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   // start at tile boundaries), so they stay in step and their identical row
   // reads hit in L2 instead of each fetching the rows again. Waves past the
   // work, and waves with shorter runs, take the same barriers.
-  for (uint64_t s = 0; s < p.steps_per_wave; ++s) {
+  for (uint32_t s = 0; s < spw; ++s) {
     if (s && (s & (n_pairs - 1)) == 0) __syncthreads();
     if (s >= n_steps) continue;
     const uint64_t step = gs0 + s;  // (gs is the count of global stack levels)
@@ -268,9 +270,9 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
     if constexpr (kLateLoads) load_rows();
     const uint4 m0 = opaque4(leaf_mask(k, L)), m1 = opaque4(leaf_mask(k, R));
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-      xor_masked(acc[c], r0[c], m0);
-      xor_masked(acc[c], r1[c], m1);
+    for (int c = 0; c < C; c++) {  // one v_bitop3_b32 per word and row
+      xor_masked3(acc[c], r0[c], m0);
+      xor_masked3(acc[c], r1[c], m1);
     }
   }
 
