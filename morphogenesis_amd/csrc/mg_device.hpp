@@ -361,6 +361,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // 16-byte words in LDS / global memory, typed by address space.
 typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 typedef __attribute__((address_space(1))) u32x4 glob_u32x4;
+typedef __attribute__((address_space(1))) uint32_t glob_u32;
 
 // Key reads go through the constant address space: no kernel writes a key,
 // and a wave-uniform read then becomes a scalar load into SGPRs (a divergent

@@ -92,7 +92,7 @@ struct ScanBuffers {
   uint4* out = nullptr;
   const uint32_t* out_idx = nullptr;  // launch key i -> out row out_idx[i]
   bool accumulate = false;            // out ^= scan instead of out = scan
-  uint4* gstack = nullptr;            // >= plan.parts * n_keys * plan.gslots * 1 KiB
+  uint4* gstack = nullptr;            // >= lane_gstack_bytes(plan, n_keys)
 };
 
 // One scan workspace: a stream of its own with the device buffers, pinned
@@ -213,9 +213,10 @@ int parse_keys(const uint8_t* keys, const size_t* key_lens, uint32_t n_keys, Dev
 int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_bytes,
               uint32_t n_keys, uint32_t d, const mg_scan_opts* o, Plan* pl);
 
-// Bytes of the lane-DFS global stack levels a plan's launch needs (0 if none).
+// Bytes of the lane-DFS global stack levels and tile-path caches a plan's launch
+// needs (0 if none).
 inline size_t lane_gstack_bytes(const Plan& pl, uint32_t n_keys) {
-  return (size_t)pl.parts * n_keys * pl.gslots * 64 * 16;
+  return pl.gslots ? (size_t)pl.parts * n_keys * ((size_t)pl.gslots * 64 + kLanePathU4) * 16 : 0;
 }
 
 // Where a launch sequence goes: the stream, and the word a split-kernel launch on it

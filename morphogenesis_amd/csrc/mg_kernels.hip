@@ -127,6 +127,38 @@ __device__ __forceinline__ Node tile_lane_root(const DevKey* __restrict__ k,
   return n;
 }
 
+// The same walk with the path kept in global memory instead of registers:
+// pseed[l] / pt[l] hold the wave-uniform node at level l (0 .. tb). Every lane
+// writes the same words and later reads them back itself, so program order
+// alone orders each lane's read after its write. The path is touched only at
+// tile starts, so it costs no registers through the ChaCha rounds.
+__device__ __forceinline__ Node tile_lane_root_cached(const DevKey* __restrict__ k,
+                                                      glob_u32x4* pseed, glob_u32* pt, int lane,
+                                                      uint64_t tile, uint64_t prev, bool first,
+                                                      int tb, int lb) {
+  int start = 0;
+  if (!first) {
+    const uint64_t diff = tile ^ prev;  // non-zero, below bit tb
+    start = tb - (64 - __builtin_clzll(diff));
+  }
+  Node n;
+  if (start == 0) {
+    n = key_root(k);
+  } else {
+    const u32x4 v = pseed[start];
+    n.s0 = v.x; n.s1 = v.y; n.s2 = v.z; n.s3 = v.w;
+    n.t = pt[start];
+  }
+  for (int lvl = start; lvl < tb; ++lvl) {
+    n = ggm_step(k, lvl, n, (uint32_t)(tile >> (tb - 1 - lvl)) & 1u);
+    pseed[lvl + 1] = u32x4{n.s0, n.s1, n.s2, n.s3};
+    pt[lvl + 1] = n.t;
+  }
+  for (int j = 0; j < lb; ++j)
+    n = ggm_step(k, tb + j, n, ((uint32_t)lane >> (lb - 1 - j)) & 1u);
+  return n;
+}
+
 // -------------------------------------------------------- lane-DFS scan ----
 
 template <int C, bool kLateLoads, bool kGlobalStack>
@@ -155,10 +187,17 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   // The two parts are typed by address space, so a pop stays a ds_read or a
   // global load: through one generic pointer it would be a flat load, slower
   // for the LDS slots that nearly every pop reads.
+  // With global levels, the wave's region of p.gstack also caches the
+  // wave-uniform tile-prefix path (kLanePathU4 uint4s after the levels), which
+  // the LDS-only instances keep in registers instead.
   const int gs = kGlobalStack ? (int)p.gslots : 0;
   lds_u32x4* st_seed = (lds_u32x4*)(lds + (size_t)wib * (kl - 1 - gs) * 64);  // slot s >= gs
   glob_u32x4* st_glob =
-      kGlobalStack ? (glob_u32x4*)(p.gstack + (size_t)(live ? w : 0) * gs * 64) : nullptr;
+      kGlobalStack
+          ? (glob_u32x4*)(p.gstack + (size_t)(live ? w : 0) * ((size_t)gs * 64 + kLanePathU4))
+          : nullptr;
+  glob_u32x4* path_seed = kGlobalStack ? st_glob + gs * 64 : nullptr;
+  glob_u32* path_t = kGlobalStack ? (glob_u32*)(path_seed + 64) : nullptr;
   uint32_t st_tbits = 0;
 
   uint4 acc[C];
@@ -188,13 +227,13 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
     st_tbits = R.t == 1u ? (st_tbits | bit) : (st_tbits & ~bit);
   };
 
-  LanePath path;
-  {
+  LanePath path;  // (LDS-only instances)
+  if constexpr (!kGlobalStack) {
     const Node r = key_root(k);
     path.s0 = r.s0; path.s1 = r.s1; path.s2 = r.s2; path.s3 = r.s3; path.t = r.t;
   }
   Node node;
-  uint64_t base = 0;
+  uint64_t tile_row0 = 0;  // the tile's first row; lane l's subtree starts 2^kl * l rows on
   // One iteration per leaf pair: pop the pending right child (if any), walk
   // down to the bottom node pushing right children, expand it into the two
   // leaves and fold the pair's rows into the accumulators. The rows are
@@ -217,8 +256,11 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
     const uint32_t pair = (uint32_t)(step & (n_pairs - 1));
     int lvl;
     if (s == 0 || pair == 0) {  // the run's first step, or a new tile
-      node = tile_lane_root(k, path, lane, tile, tile - 1, s == 0, tb, lb);
-      base = (tile << (kl + lb)) | ((uint64_t)lane << kl);
+      if constexpr (kGlobalStack)
+        node = tile_lane_root_cached(k, path_seed, path_t, lane, tile, tile - 1, s == 0, tb, lb);
+      else
+        node = tile_lane_root(k, path, lane, tile, tile - 1, s == 0, tb, lb);
+      tile_row0 = tile << (kl + lb);
       lvl = 0;
       if (pair != 0) {
         // The run starts inside a tile: walk from the lane root down to leaf
@@ -248,7 +290,8 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
     }
     uint4 r0[C], r1[C];
     auto load_rows = [&]() {
-      const uint64_t x = base + 2 * (uint64_t)pair;
+      // (the lane bits are added here rather than kept in a register pair)
+      const uint64_t x = (tile_row0 | ((uint64_t)__lane_id() << kl)) + 2 * (uint64_t)pair;
       const uint4* src0 = (lane_active && x < p.n_rows) ? p.rows + x * C : p.zero_rows;
       const uint4* src1 =
           (lane_active && x + 1 < p.n_rows) ? p.rows + (x + 1) * C : p.zero_rows;

@@ -15,6 +15,9 @@ struct DevKey;
 //   lb  per-lane walk to the lane's subtree root (lanes >= 2^lb idle),
 //   kl  depth of each lane's subtree, expanded depth-first (kl >= 1).
 // A tile is 2^(lb + kl) consecutive rows.
+// uint4s per wave of a lane-DFS launch's global region beyond its stack levels.
+constexpr uint32_t kLanePathU4 = 64 + 16;
+
 struct LaneDfsParams {
   const uint4* rows;
   uint64_t n_rows;
@@ -32,7 +35,9 @@ struct LaneDfsParams {
   uint32_t parts;  // waves per key; wave w serves key w % n_keys
   uint4* partial;  // [parts][n_keys][chunks]
   // The top gslots levels of each lane's DFS stack (depths 1 .. gslots) live in
-  // global memory, [waves][gslots][64] seeds; the rest in LDS.
+  // global memory, the rest in LDS. Wave w's region of gstack is gslots * 64 +
+  // kLanePathU4 uint4s: [gslots][64] seeds, then the wave-uniform tile-prefix
+  // path ([64] seeds, [64] control words) cached at tile starts.
   uint4* gstack;
   uint32_t gslots;
   uint32_t late_loads;  // rows loaded after a step's expansions (else before)
