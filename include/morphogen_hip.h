@@ -80,7 +80,7 @@ typedef struct {
   uint32_t keys_per_wave;
   uint32_t waves;        /* waves launched for the fused scan              */
   uint32_t stack_global; /* lane-DFS stack levels kept in global memory    */
-  uint32_t reserved;
+  uint32_t dfs_waves;    /* split kernel: DFS waves per pair (else 0)     */
 } mg_timing;
 
 enum {

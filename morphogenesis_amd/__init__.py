@@ -77,7 +77,7 @@ class Timing(ctypes.Structure):
                 ("scan_kernel_ms", ctypes.c_double), ("kernel", ctypes.c_uint32),
                 ("lane_depth", ctypes.c_uint32), ("keys_per_wave", ctypes.c_uint32),
                 ("waves", ctypes.c_uint32), ("stack_global", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("dfs_waves", ctypes.c_uint32)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}

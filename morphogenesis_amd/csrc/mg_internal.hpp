@@ -66,6 +66,7 @@ struct Plan {
   uint32_t late_loads;      // lane-DFS: rows loaded after a step's expansions
   uint32_t col_splits;      // split kernel: 1, or 2 pairs per row, one per 128-chunk half
   uint32_t mask_depth;      // split kernel: steps of masks a DFS wave may run ahead
+  uint32_t dfs_waves;       // split kernel: DFS waves per pair (1 or 2)
   uint64_t row_begin, row_end;
   uint64_t n_delta;
   const void* rows;

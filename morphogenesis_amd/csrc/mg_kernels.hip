@@ -12,9 +12,9 @@
 //                    per wave; leaf masks go through LDS and each row is read
 //                    once, coalesced, for all G keys (PK rows per load when a
 //                    row has at most 64 / PK chunks).
-// scan_split<S,G,NP> the stream kernel's work split between NP pairs of
-//                    waves per block: a DFS wave makes the masks, a streaming
-//                    wave folds the rows (528 B - 1 KiB rows, several keys).
+// scan_split<S,G,NP,DW> the stream kernel's work split between NP pairs of
+//                    waves per block: DW DFS waves make the masks, a streaming
+//                    wave folds the rows (528 B - 4 KiB rows, several keys).
 // scan_point         point-eval fallback (d = 0, tiny inputs) and the delta
 //                    scan: one lane per row, full path walk per row.
 // reduce_partials    XOR of per-wave partial answers into the output.
@@ -660,18 +660,28 @@ __host__ __device__ __forceinline__ uint32_t split_lds_levels(int G, uint32_t kl
 }
 
 // LDS of one pair, in uint4 units: G stacks of split_lds_levels x 64 seeds,
-// D mask buffers of G x 128 masks, and one uint4 holding the two counters.
+// D mask buffers of G x 128 masks, and one uint4 holding the counters (steps
+// published by the first DFS wave, steps folded, steps published by the second).
 __host__ __device__ __forceinline__ size_t split_pair_u4(int G, uint32_t kl, uint32_t D) {
   return (size_t)G * split_lds_levels(G, kl) * 64 + (size_t)D * G * 128 + 1;
 }
 
-template <int S, int G, int NP>
-__global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
+// DW DFS waves per pair (1 or 2): with two, each makes the masks of G / DW of
+// the pair's keys, so the keys' ChaCha work runs on two waves of a SIMD, and
+// the streaming wave waits for both waves' counters. Blocks of DW = 2 hold
+// 3 NP waves and keep two row batches in flight instead of four, which leaves
+// the registers for three waves per SIMD.
+template <int S, int G, int NP, int DW>
+__global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p) {
+  static_assert(DW == 1 || DW == 2, "one or two DFS waves per pair");
+  static_assert(G % DW == 0, "a pair's keys split evenly over its DFS waves");
+  constexpr int GW = G / DW;  // keys per DFS wave
   extern __shared__ uint4 lds[];
   const int lane = threadIdx.x & 63;
   const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const bool producer = wib < NP;
-  const int pi = wib % NP;  // this wave's pair in the block
+  const bool producer = wib < NP * DW;
+  const int pi = producer ? wib / DW : wib - NP * DW;  // this wave's pair in the block
+  const int sub = producer ? wib % DW : 0;            // a DFS wave's share of the keys
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
   const uint32_t halves = p.col_splits;
   const uint32_t total = p.parts * n_groups * halves;  // pairs with work
@@ -711,36 +721,39 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
   uint4* stacks = pair_lds;                                  // [G][lds_slots][64]
   uint4* masks = pair_lds + (size_t)G * lds_slots * 64;      // [D][G][128]
   uint32_t* ctr = reinterpret_cast<uint32_t*>(masks + (size_t)D * G * 128);
-  uint32_t* published = ctr;  // steps whose masks are in the ring
+  uint32_t* published = ctr + 2 * sub;  // steps whose masks (this DFS wave's keys) are in the ring
   uint32_t* folded = ctr + 1;  // steps whose masks the streaming wave is done with
   if (producer && lane == 0) {
     *published = 0;
-    *folded = 0;
+    if (sub == 0) *folded = 0;
   }
   __syncthreads();  // the only block barrier: counters are set before any wave reads them
 
   if (producer) {
-    const DevKey* kp[G];
+    // This wave's keys are the pair's keys sub * GW .. sub * GW + GW - 1; arrays
+    // below are indexed by g < GW, the pair's LDS stacks and masks by gk = sub * GW + g.
+    const int gk0 = sub * GW;
+    const DevKey* kp[GW];
 #pragma unroll
-    for (int g = 0; g < G; g++) {
-      const uint32_t ki = key0 + g < p.n_keys ? key0 + g : key0;
+    for (int g = 0; g < GW; g++) {
+      const uint32_t ki = key0 + gk0 + g < p.n_keys ? key0 + gk0 + g : key0;
       kp[g] = p.keys + (p.key_idx ? p.key_idx[ki] : ki);
     }
-    LanePath path[G];
+    LanePath path[GW];
 #pragma unroll
-    for (int g = 0; g < G; g++) {
+    for (int g = 0; g < GW; g++) {
       const Node r = key_root(kp[g]);
       path[g].s0 = r.s0; path[g].s1 = r.s1; path[g].s2 = r.s2;
       path[g].s3 = r.s3; path[g].t = r.t;
     }
     const int leaf_level = d - kl;
-    Node node[G];
-    uint32_t st_tbits[G];
+    Node node[GW];
+    uint32_t st_tbits[GW];
     // Register slots as plain words: [0] slot 0, [r] slot kl - 1 - r for r >= 1 (for
     // small kl they may name the same slot; each holds what was last pushed there).
-    uint32_t st_reg[G][R][4];
+    uint32_t st_reg[GW][R][4];
 #pragma unroll
-    for (int g = 0; g < G; g++) st_tbits[g] = 0;
+    for (int g = 0; g < GW; g++) st_tbits[g] = 0;
     // Seed of stack slot `slot` (uniform) for key g. The register slots are
     // written and read as values (selects), never through a pointer, so they
     // stay in VGPRs.
@@ -755,7 +768,8 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
         for (int i = 0; i < 4; i++) st_reg[g][r][i] = in_reg ? v[i] : st_reg[g][r][i];
       }
       if (!any)
-        stacks[((size_t)g * lds_slots + (slot - 1)) * 64 + lane] = make_uint4(v[0], v[1], v[2], v[3]);
+        stacks[((size_t)(gk0 + g) * lds_slots + (slot - 1)) * 64 + lane] =
+            make_uint4(v[0], v[1], v[2], v[3]);
     };
     auto get_seed = [&](int g, int slot) __attribute__((always_inline)) -> uint4 {
       uint32_t a[R][4];
@@ -779,7 +793,7 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
         for (int i = 0; i < 4; i++) w[i] = hit ? a[r][i] : w[i];
       }
       uint4 v = make_uint4(w[0], w[1], w[2], w[3]);
-      if (!any) v = stacks[((size_t)g * lds_slots + (slot - 1)) * 64 + lane];
+      if (!any) v = stacks[((size_t)(gk0 + g) * lds_slots + (slot - 1)) * 64 + lane];
       return v;
     };
     // The next tile's lane roots, walked one tree level per step while the
@@ -787,12 +801,12 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
     // tile's first step only descends its kl levels. Walking the changed
     // tile-prefix levels and the lb lane levels (about 8 node expansions per
     // key) all in that step held the streaming wave up once per tile.
-    Node nroot[G];
+    Node nroot[GW];
     int pre_lvl = -1;  // -1: no walk under way
     const int root_lvl = tb + lb;
     auto root_level = [&](uint64_t ntile) {  // one level of the walk, every key
 #pragma unroll
-      for (int g = 0; g < G; g++) {
+      for (int g = 0; g < GW; g++) {
         const uint32_t bit = pre_lvl < tb ? (uint32_t)(ntile >> (tb - 1 - pre_lvl)) & 1u
                                           : ((uint32_t)lane >> (root_lvl - 1 - pre_lvl)) & 1u;
         Node a, b;
@@ -816,10 +830,10 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
         if (s != 0 && pre_lvl >= 0) {
           while (pre_lvl < root_lvl) root_level(tile);  // finish the walk
 #pragma unroll
-          for (int g = 0; g < G; g++) node[g] = nroot[g];
+          for (int g = 0; g < GW; g++) node[g] = nroot[g];
         } else {
 #pragma unroll
-          for (int g = 0; g < G; g++)
+          for (int g = 0; g < GW; g++)
             node[g] = tile_lane_root(kp[g], path[g], lane, tile, tile - 1, s == 0, tb, lb);
         }
         pre_lvl = -1;
@@ -830,7 +844,7 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
         // pair `pair`, keeping the right children that lie ahead of it, as the
         // depth-first walk from pair 0 would have left them.
 #pragma unroll
-        for (int g = 0; g < G; g++) {
+        for (int g = 0; g < GW; g++) {
           for (int l = 0; l < kl - 1; ++l) {
             Node L, R;
             expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
@@ -848,7 +862,7 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
       uint4* mbuf = masks + (size_t)(s % D) * G * 128;
       const int lvl = seek ? kl - 1 : from_root ? 0 : kl - 1 - __builtin_ctz(pair);
 #pragma unroll
-      for (int g = 0; g < G; g++) {
+      for (int g = 0; g < GW; g++) {
         if (!from_root) {  // the right child at depth lvl >= 1 sits in slot lvl - 1
           const uint4 sv = get_seed(g, lvl - 1);
           node[g].s0 = sv.x; node[g].s1 = sv.y; node[g].s2 = sv.z; node[g].s3 = sv.w;
@@ -863,8 +877,8 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
           node[g] = L;
         }
         expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + kl - 1, node[g], L, R);
-        mbuf[g * 128 + 2 * lane] = leaf_mask<false>(kp[g], L);
-        mbuf[g * 128 + 2 * lane + 1] = leaf_mask<false>(kp[g], R);
+        mbuf[(gk0 + g) * 128 + 2 * lane] = leaf_mask<false>(kp[g], L);
+        mbuf[(gk0 + g) * 128 + 2 * lane + 1] = leaf_mask<false>(kp[g], R);
       }
       lds_store_release(published, (uint32_t)(s + 1));  // after this wave's mask stores
       // One level of the next tile's lane-root walk, if this run reaches that tile.
@@ -873,7 +887,7 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
           const uint64_t diff = (tile + 1) ^ tile;
           pre_lvl = tb - (64 - __builtin_clzll(diff));
 #pragma unroll
-          for (int g = 0; g < G; g++) nroot[g] = path[g].get(pre_lvl);
+          for (int g = 0; g < GW; g++) nroot[g] = path[g].get(pre_lvl);
         }
         if (pre_lvl < root_lvl) root_level(tile + 1);
       }
@@ -884,7 +898,7 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
   // Consumer: rows of step s are tile_base + j*2^kl + 2*pair + {0,1} for
   // source lane j < 64; row r of the step uses mask slot r.
   constexpr int U = 8 / S;
-  constexpr int NB = kSplitNB;
+  constexpr int NB = DW == 2 ? 2 : kSplitNB;
   const uint32_t n_batches = 128 / U;  // a multiple of NB
   uint32_t lane_off[S];
 #pragma unroll
@@ -953,7 +967,9 @@ __global__ __launch_bounds__(128 * NP) void scan_split(StreamParams p) {
     }
     uint32_t buf = 0;  // step s's masks are in buffer s mod D
     for (uint64_t s = 0; s < n_steps; ++s) {
-      if (!wait_count(published, (uint32_t)(s + 1))) return false;
+      // Step s's masks are in the ring once every DFS wave of the pair has published it.
+      if (!wait_count(ctr, (uint32_t)(s + 1))) return false;
+      if (DW == 2 && !wait_count(ctr + 2, (uint32_t)(s + 1))) return false;
       const uint4* mbuf = masks + (size_t)buf * G * 128;
 #pragma unroll
       for (uint32_t b0 = 0; b0 < n_batches; b0 += NB) {
@@ -1257,6 +1273,9 @@ hipError_t launch_suffix(int chunks, const SuffixParams& p, hipStream_t s) {
   MG_SG(2, 1, NN) MG_SG(2, 2, NN) MG_SG(2, 3, NN) MG_SG(2, 4, NN)             \
   MG_SG(4, 1, NN) MG_SG(4, 2, NN) MG_SG(4, 3, NN)
 #define MG_SPLIT_ALL MG_SPLIT_SHAPES(1) MG_SPLIT_SHAPES(2) MG_SPLIT_SHAPES(4)
+// Two DFS waves per pair: 1 KiB rows, 2 or 4 keys.
+#define MG_SPLIT_DW2 MG_SG2(1, 2, 1) MG_SG2(1, 2, 2) MG_SG2(1, 2, 4) \
+  MG_SG2(1, 4, 1) MG_SG2(1, 4, 2) MG_SG2(1, 4, 4)
 
 hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t s) {
   if (p.col_splits != 1 &&
@@ -1265,17 +1284,25 @@ hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t
   const uint32_t n_groups = (p.n_keys + G - 1) / G;
   const uint32_t total = p.parts * n_groups * p.col_splits;
   const uint32_t blocks = (total + NP - 1) / NP;
-  const dim3 block(128 * NP);
+  const uint32_t DW = p.dfs_waves;
+  const dim3 block(64 * (DW + 1) * NP);
   const dim3 grid((blocks + kXcds - 1) / kXcds * kXcds);
   if (p.mask_depth < 2 || !p.fault) return hipErrorInvalidValue;
   const size_t lds = split_lds_bytes(G, p.kl, NP, p.mask_depth);
 #define MG_SG(SS, GG, NN)                                                     \
-  if (S == SS && G == GG && NP == NN) {                                       \
-    hipLaunchKernelGGL((scan_split<SS, GG, NN>), grid, block, lds, s, p);     \
+  if (DW == 1 && S == SS && G == GG && NP == NN) {                            \
+    hipLaunchKernelGGL((scan_split<SS, GG, NN, 1>), grid, block, lds, s, p);  \
+    return hipGetLastError();                                                 \
+  }
+#define MG_SG2(SS, GG, NN)                                                    \
+  if (DW == 2 && S == SS && G == GG && NP == NN) {                            \
+    hipLaunchKernelGGL((scan_split<SS, GG, NN, 2>), grid, block, lds, s, p);  \
     return hipGetLastError();                                                 \
   }
   MG_SPLIT_ALL
+  MG_SPLIT_DW2
 #undef MG_SG
+#undef MG_SG2
   return hipErrorInvalidValue;
 }
 
@@ -1283,13 +1310,19 @@ size_t split_lds_bytes(int G, uint32_t kl, int NP, uint32_t mask_depth) {
   return (size_t)NP * split_pair_u4(G, kl, mask_depth) * sizeof(uint4);
 }
 
-hipError_t split_blocks_per_cu(int S, int G, int NP, size_t lds, int* blocks) {
+hipError_t split_blocks_per_cu(int S, int G, int NP, int DW, size_t lds, int* blocks) {
 #define MG_SG(SS, GG, NN)                                                                   \
-  if (S == SS && G == GG && NP == NN)                                                       \
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_split<SS, GG, NN>,     \
+  if (DW == 1 && S == SS && G == GG && NP == NN)                                            \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_split<SS, GG, NN, 1>,  \
                                                         128 * NN, lds);
+#define MG_SG2(SS, GG, NN)                                                                  \
+  if (DW == 2 && S == SS && G == GG && NP == NN)                                            \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_split<SS, GG, NN, 2>,  \
+                                                        192 * NN, lds);
   MG_SPLIT_ALL
+  MG_SPLIT_DW2
 #undef MG_SG
+#undef MG_SG2
   return hipErrorInvalidValue;
 }
 

@@ -70,6 +70,8 @@ struct StreamParams {
   // runs out (a bug guard: every wait is bounded, so the kernel always ends).
   uint32_t mask_depth;
   uint32_t* fault;
+  // scan_split: DFS waves per pair (1, or 2 with the pair's keys split between them).
+  uint32_t dfs_waves;
 };
 
 struct PointParams {
@@ -110,7 +112,7 @@ size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block);
 // pairs of a DFS wave and a row-streaming wave; needs lb = 6.
 hipError_t launch_split(int S, int G, int NP, const StreamParams& p, hipStream_t s);
 size_t split_lds_bytes(int G, uint32_t kl, int NP, uint32_t mask_depth);
-hipError_t split_blocks_per_cu(int S, int G, int NP, size_t lds, int* blocks);
+hipError_t split_blocks_per_cu(int S, int G, int NP, int DW, size_t lds, int* blocks);
 // Resident blocks per CU for the given block size and dynamic LDS.
 hipError_t lane_dfs_blocks_per_cu(int chunks, bool late_loads, bool global_stack,
                                   int block_threads, size_t lds, int* blocks);

@@ -218,7 +218,8 @@ namespace {
 
 // Resident waves the device holds for a kernel variant (cached per variant).
 // Lane-DFS kinds pass G = late loads (1) + global stack levels (2): the four
-// kernels of one row width.
+// kernels of one row width. Split kinds pass PK = DFS waves per pair and count
+// two waves per pair, whatever the block's real size.
 int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chunks,
                    int waves_per_block, size_t lds, int* out) {
   auto key = std::make_pair(((kind * 8 + (uint32_t)PK) * 8 + (uint32_t)G) * 64 +
@@ -234,7 +235,7 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chun
       MG_HIP(lane_dfs_blocks_per_cu(chunks, (G & 1) != 0, (G & 2) != 0, 64 * waves_per_block,
                                     lds, &blocks));
     else if (kind / 100 == MG_KERNEL_STREAM_SPLIT)
-      MG_HIP(split_blocks_per_cu(S, G, waves_per_block / 2, lds, &blocks));
+      MG_HIP(split_blocks_per_cu(S, G, waves_per_block / 2, PK, lds, &blocks));
     else
       MG_HIP(stream_blocks_per_cu(S, G, PK, 64 * waves_per_block, lds, &blocks));
     dev->occupancy[key] = blocks;
@@ -635,6 +636,12 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
     }
     pl->waves_per_block = wpb;
     const int NP = (int)wpb / 2;
+    // One group of two or four keys over rows of up to 64 chunks: two DFS waves per pair,
+    // each with half the keys (78,643,200 x 1 KiB, one process, two placements: 4 keys
+    // 12.55 / 12.74 against 13.22 / 13.35 ms with one DFS wave, 2 keys 11.64 / 12.05 against
+    // 11.92 / 12.16 ms). With several groups one DFS wave stays faster (8 keys as 4 + 4:
+    // 23.29 against 24.92 ms).
+    pl->dfs_waves = (S == 1 && (G == 2 || G == 4) && groups == 1 && pl->col_splits == 1) ? 2 : 1;
     // Mask ring: as many steps as the block's LDS holds, up to kSplitDefaultDepth, so a DFS
     // wave can run ahead through the cheap steps of a tile and cover the steps that descend
     // a whole subtree.
@@ -649,8 +656,8 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
       return fail(MG_ERR_INVALID_ARG, "split plan needs %zu B of LDS per block (max %zu)", lds,
                   dev->max_lds);
     int res_waves = 0;
-    rc = resident_waves(dev, MG_KERNEL_STREAM_SPLIT * 100 + (uint32_t)(S * 8 + G), S, G, 1,
-                        (int)pl->chunks, (int)wpb, lds, &res_waves);
+    rc = resident_waves(dev, MG_KERNEL_STREAM_SPLIT * 100 + (uint32_t)(S * 8 + G), S, G,
+                        (int)pl->dfs_waves, (int)pl->chunks, (int)wpb, lds, &res_waves);
     // Three pairs per CU (768 on 256 CUs) stream fastest with single-pair blocks; a fourth
     // block on some CUs unbalances them (four keys: 14.9 ms at 768 blocks, 26.5 ms at 832).
     // Four-pair blocks: one per CU.
@@ -733,6 +740,7 @@ int enqueue_scan(mg_device* dev, const Exec& x, const Plan& pl, const ScanBuffer
       if (pl.kernel == MG_KERNEL_STREAM_SPLIT) {
         p.col_splits = pl.col_splits;
         p.mask_depth = pl.mask_depth;
+        p.dfs_waves = pl.dfs_waves;
         p.fault = x.fault;
         MG_HIP(launch_split(split_s(&pl), (int)pl.keys_per_wave,
                             (int)pl.waves_per_block / 2, p, s));
@@ -833,6 +841,7 @@ int fill_timing(const hipEvent_t ev[5], const Plan& pl, double t0, mg_timing* t)
   t->kernel = pl.kernel;
   t->lane_depth = pl.kl;
   t->stack_global = pl.gslots;
+  t->dfs_waves = pl.kernel == MG_KERNEL_STREAM_SPLIT ? pl.dfs_waves : 0;
   const uint32_t n_keys = pl.n_keys;
   const bool stream = pl.kernel == MG_KERNEL_STREAM || pl.kernel == MG_KERNEL_STREAM_SPLIT;
   t->keys_per_wave = stream ? pl.keys_per_wave : 1;

@@ -246,6 +246,44 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
     m.close()
 
 
+@pytest.mark.parametrize("n,w,d", [(3000, 1024, 12), (70_001, 1024, 17), (2100, 528, 12),
+                                   (5000, 1008, 13)])
+@pytest.mark.parametrize("n_keys", [2, 4])
+def test_split_two_dfs_waves_match_oracle(mg, scanner, n, w, d, n_keys):
+    # One group of 2 or 4 keys over rows of up to 64 chunks: each pair has two DFS
+    # waves with half the keys each, and the streaming wave waits for both.
+    print(f"case split-dw2 n={n} w={w} d={d} keys={n_keys}", flush=True)
+    rows = oracle.fill_rows(seed=n * 7 + w, first_row=0, n_rows=n, row_bytes=w)
+    m = upload(mg, scanner, rows)
+    targets = [0, n - 1, n // 3, n // 2][:n_keys]
+    keys = [p[i % 2] for i, p in enumerate(pairs_for(d, targets, seed=w + 3))]
+    ref = oracle.scan_tiled(keys, rows, THREADS)
+    for kl in sorted({1, 2, 3, 5, d - 6}):
+        for wpb in (0, 2, 4, 8):
+            for wpk in (0, 3, 1000):  # runs that start and end inside tiles
+                opts = mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT, lane_depth=kl,
+                                    waves_per_block=wpb, waves_per_key=wpk)
+                if (wpb or 2) // 2 * split_pair_bytes(n_keys, kl) > 160 * 1024:
+                    continue
+                got, t = scanner.scan_keys(m, keys, opts, with_timing=True)
+                assert t["kernel"] == "stream_split" and t["dfs_waves"] == 2, t
+                assert np.array_equal(got, ref), (kl, wpb, wpk)
+    for depth in (2, 3, 4):
+        try:
+            got = scanner.scan_keys(m, keys, mg.make_opts(kernel=mg.KERNEL_STREAM_SPLIT,
+                                                          lane_depth=d - 6, mask_depth=depth))
+        except mg.MgError as e:
+            assert "LDS" in str(e), e
+            continue
+        assert np.array_equal(got, ref), depth
+    # Several groups keep one DFS wave per pair.
+    more = keys + [pairs_for(d, [n // 7], seed=w + 9)[0][0]] * 5
+    got, t = scanner.scan_keys(m, more, None, with_timing=True)
+    assert t["kernel"] == "stream_split" and t["dfs_waves"] == 1, t
+    assert np.array_equal(got, oracle.scan_tiled(more, rows, THREADS))
+    m.close()
+
+
 # Kernel AUTO picks: lane-DFS up to 128-byte rows; the split kernel for
 # 528-1008-byte rows with >= 3 keys and 1024-4096-byte rows, when d >= 11
 # (rows over 2 KiB in column halves from 4 keys); the stream kernel otherwise. Small domains shrink the split kernel's lane depth.
