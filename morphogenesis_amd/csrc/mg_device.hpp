@@ -343,11 +343,13 @@ __device__ __forceinline__ uint32_t t_mask(uint32_t t) {
 // expansions per second than kPrgOrdered at 4 waves per SIMD, 6.5 % at 5,
 // 9.7 % at 8, and in the 2^31 x 32 B, 24-key lane-DFS pass 800.9-802.5 ms
 // against 817.3-818.2 ms with six wait states before rotates (twelve:
-// 804.9-805.6, fourteen: 831.1-832.0 ms). kPrgOrdered is the routine of the
-// stream and split kernels' DFS waves: they run one to three DFS waves per
-// SIMD, where the spacing s_nops cost more than they save (78,643,200 x 1 KiB,
-// 4 keys: 14.40-14.61 against 14.87-15.45 ms; 238,609,294 x 144 B, 1 key:
-// 9.57-9.61 against 9.83-9.89 ms).
+// 804.9-805.6, fourteen: 831.1-832.0 ms; at eight waves per SIMD ten still wins:
+// 790.3-793.7 ms against 792.1-793.9 for twelve and 807-814 for six, eight and
+// fourteen). kPrgOrdered is the routine of the stream kernel's waves and of
+// split pairs with one DFS wave: one to three DFS waves per SIMD, where the
+// spacing s_nops cost more than they save (78,643,200 x 1 KiB, 4 keys:
+// 14.40-14.61 against 14.87-15.45 ms; 238,609,294 x 144 B, 1 key: 9.57-9.61
+// against 9.83-9.89 ms). Split pairs with two DFS waves use kPrgSpaced.
 enum PrgRoutine : int { kPrgOrdered = 1, kPrgSpaced = 2 };
 constexpr int kSpaceAdd = 2, kSpaceXor = 2, kSpaceRot = 10;
 constexpr int kPrgVariant = kPrgSpaced;

@@ -676,6 +676,10 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
   static_assert(DW == 1 || DW == 2, "one or two DFS waves per pair");
   static_assert(G % DW == 0, "a pair's keys split evenly over its DFS waves");
   constexpr int GW = G / DW;  // keys per DFS wave
+  // Two DFS waves share a SIMD, and the stage-spaced node routine pays there (4 keys of
+  // 78,643,200 x 1 KiB, one process: 12.13 / 12.29 against 12.43 / 12.54 ms); one DFS
+  // wave keeps the unspaced one.
+  constexpr int kVar = DW == 2 ? kPrgSpaced : kSplitPrgVariant;
   extern __shared__ uint4 lds[];
   const int lane = threadIdx.x & 63;
   const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -810,7 +814,7 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
         const uint32_t bit = pre_lvl < tb ? (uint32_t)(ntile >> (tb - 1 - pre_lvl)) & 1u
                                           : ((uint32_t)lane >> (root_lvl - 1 - pre_lvl)) & 1u;
         Node a, b;
-        expand_node<kSplitPrgVariant, false>(kp[g], pre_lvl, nroot[g], a, b);
+        expand_node<kVar, false>(kp[g], pre_lvl, nroot[g], a, b);
         nroot[g] = bit ? b : a;
         if (pre_lvl < tb) path[g].put(lane, pre_lvl + 1, nroot[g]);
       }
@@ -847,7 +851,7 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
         for (int g = 0; g < GW; g++) {
           for (int l = 0; l < kl - 1; ++l) {
             Node L, R;
-            expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
+            expand_node<kVar, false>(kp[g], leaf_level + l, node[g], L, R);
             if ((pair >> (kl - 2 - l)) & 1u) {
               node[g] = R;
             } else {
@@ -870,13 +874,13 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
         }
         Node L, R;
         for (int l = lvl; l < kl - 1; ++l) {
-          expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + l, node[g], L, R);
+          expand_node<kVar, false>(kp[g], leaf_level + l, node[g], L, R);
           put_seed(g, l, R);  // depth l + 1
           const uint32_t bit = 1u << (l + 1);
           st_tbits[g] = R.t == 1u ? (st_tbits[g] | bit) : (st_tbits[g] & ~bit);
           node[g] = L;
         }
-        expand_node<kSplitPrgVariant, false>(kp[g], leaf_level + kl - 1, node[g], L, R);
+        expand_node<kVar, false>(kp[g], leaf_level + kl - 1, node[g], L, R);
         mbuf[(gk0 + g) * 128 + 2 * lane] = leaf_mask<false>(kp[g], L);
         mbuf[(gk0 + g) * 128 + 2 * lane + 1] = leaf_mask<false>(kp[g], R);
       }
