@@ -214,10 +214,25 @@ int parse_keys(const uint8_t* keys, const size_t* key_lens, uint32_t n_keys, Dev
 int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_bytes,
               uint32_t n_keys, uint32_t d, const mg_scan_opts* o, Plan* pl);
 
-// Bytes of the lane-DFS global stack levels and tile-path caches a plan's launch
-// needs (0 if none).
+// Tile roots a lane-DFS launch with global stack levels precomputes (0: none, each
+// wave walks its tile prefixes): one per key and tile, when the table is small.
+inline uint64_t lane_root_entries(const Plan& pl, uint32_t n_keys) {
+  const uint64_t e = (uint64_t)n_keys * (pl.tile_end - pl.tile_begin);
+  return pl.gslots && n_keys <= 65535 && e * 20 <= kLaneRootTableMax ? e : 0;
+}
+
+// uint4s of the lane-DFS global region before the tile-root table: per wave, the
+// stack levels and the tile-path cache.
+inline size_t lane_wave_region_u4(const Plan& pl, uint32_t n_keys) {
+  return (size_t)pl.parts * n_keys * ((size_t)pl.gslots * 64 + kLanePathU4);
+}
+
+// Bytes of the lane-DFS global region a plan's launch needs (0 if none): the waves'
+// stack levels and tile-path caches, then the tile roots (seeds, then control words).
 inline size_t lane_gstack_bytes(const Plan& pl, uint32_t n_keys) {
-  return pl.gslots ? (size_t)pl.parts * n_keys * ((size_t)pl.gslots * 64 + kLanePathU4) * 16 : 0;
+  if (!pl.gslots) return 0;
+  const uint64_t e = lane_root_entries(pl, n_keys);
+  return lane_wave_region_u4(pl, n_keys) * 16 + e * 16 + (e * 4 + 15) / 16 * 16;
 }
 
 // Where a launch sequence goes: the stream, and the word a split-kernel launch on it

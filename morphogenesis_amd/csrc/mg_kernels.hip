@@ -159,10 +159,32 @@ __device__ __forceinline__ Node tile_lane_root_cached(const DevKey* __restrict__
   return n;
 }
 
+// Tile roots of a lane-DFS launch: thread (key, tile) walks the key's tb prefix
+// levels to the tile's root. One key per block row (blockIdx.y), so the key is
+// wave-uniform and its correction words come through scalar loads.
+__global__ __launch_bounds__(256) void lane_tile_roots(LaneDfsParams p, uint4* __restrict__ seeds,
+                                                       uint32_t* __restrict__ ts) {
+  const uint64_t n_tiles = p.tile_end - p.tile_begin;
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n_tiles) return;
+  const uint32_t key_i = blockIdx.y;
+  const DevKey* __restrict__ k = p.keys + (p.key_idx ? p.key_idx[key_i] : key_i);
+  const uint64_t tile = p.tile_begin + j;
+  const int tb = (int)p.tb;
+  Node n = key_root(k);
+  for (int lvl = 0; lvl < tb; ++lvl) n = ggm_step(k, lvl, n, (uint32_t)(tile >> (tb - 1 - lvl)) & 1u);
+  const size_t e = (size_t)key_i * n_tiles + j;
+  seeds[e] = make_uint4(n.s0, n.s1, n.s2, n.s3);
+  ts[e] = n.t;
+}
+
 // -------------------------------------------------------- lane-DFS scan ----
 
-template <int C, bool kLateLoads, bool kGlobalStack>
+template <int C, bool kLateLoads, bool kGlobalStack, bool kRootTable>
 __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
+  // kRootTable (with global stack levels only): tile roots come from the launch's table
+  // instead of a walk; its own instance, so each instance has one way to a tile root.
+  static_assert(!kRootTable || kGlobalStack, "the tile-root table goes with global levels");
   extern __shared__ uint4 lds[];
   const int lane = threadIdx.x & 63;
   const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -198,6 +220,14 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
           : nullptr;
   glob_u32x4* path_seed = kGlobalStack ? st_glob + gs * 64 : nullptr;
   glob_u32* path_t = kGlobalStack ? (glob_u32*)(path_seed + 64) : nullptr;
+  // This key's row of the launch's tile-root table.
+  const glob_u32x4* root_seed = nullptr;
+  const glob_u32* root_t = nullptr;
+  if constexpr (kRootTable) {
+    const size_t row = (size_t)key_i * (p.tile_end - p.tile_begin);
+    root_seed = (const glob_u32x4*)p.root_seed + row;
+    root_t = (const glob_u32*)p.root_t + row;
+  }
   uint32_t st_tbits = 0;
 
   uint4 acc[C];
@@ -256,9 +286,20 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
     const uint32_t pair = (uint32_t)(step & (n_pairs - 1));
     int lvl;
     if (s == 0 || pair == 0) {  // the run's first step, or a new tile
-      if constexpr (kGlobalStack)
+      if constexpr (kRootTable) {
+        {  // the tile's root from the launch's table, then the lane's lb levels
+          const uint64_t j = tile - p.tile_begin;
+          const u32x4 v = root_seed[j];
+          Node n;
+          n.s0 = v.x; n.s1 = v.y; n.s2 = v.z; n.s3 = v.w;
+          n.t = root_t[j];
+          for (int j = 0; j < lb; ++j)
+            n = ggm_step(k, tb + j, n, ((uint32_t)lane >> (lb - 1 - j)) & 1u);
+          node = n;
+        }
+      } else if constexpr (kGlobalStack) {
         node = tile_lane_root_cached(k, path_seed, path_t, lane, tile, tile - 1, s == 0, tb, lb);
-      else
+      } else
         node = tile_lane_root(k, path, lane, tile, tile - 1, s == 0, tb, lb);
       tile_row0 = tile << (kl + lb);
       lvl = 0;
@@ -1223,26 +1264,43 @@ size_t lane_dfs_lds_bytes(int waves_per_block, uint32_t kl, uint32_t gslots) {
   return (size_t)waves_per_block * (kl - 1 - gslots) * 64 * sizeof(uint4);
 }
 
+hipError_t launch_lane_tile_roots(const LaneDfsParams& p, uint4* seeds, uint32_t* ts,
+                                  hipStream_t s) {
+  const uint64_t n_tiles = p.tile_end - p.tile_begin;
+  if (!n_tiles || !p.n_keys) return hipSuccess;
+  if (p.n_keys > 65535 || (n_tiles + 255) / 256 > 0x7fffffffu) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(lane_tile_roots, dim3((uint32_t)((n_tiles + 255) / 256), p.n_keys),
+                     dim3(256), 0, s, p, seeds, ts);
+  return hipGetLastError();
+}
+
 hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_block,
                            hipStream_t s) {
   const uint32_t total = p.parts * p.n_keys;
   const dim3 block(64 * waves_per_block);
   const dim3 grid(xcd_grid(total, waves_per_block));
   if (p.gslots > p.kl - 1 || (p.gslots && !p.gstack)) return hipErrorInvalidValue;
+  if ((p.root_seed != nullptr) != (p.root_t != nullptr) || (p.root_seed && !p.gslots))
+    return hipErrorInvalidValue;
+  const bool table = p.root_seed != nullptr;
   const size_t lds = lane_dfs_lds_bytes(waves_per_block, p.kl, p.gslots);
   switch (chunks) {
-#define MG_LAUNCH(C, L, G)                                                          \
-  hipLaunchKernelGGL((scan_lane_dfs<C, L, G>), grid, block, lds, s, p)
+#define MG_LAUNCH(C, L, G, T)                                                       \
+  hipLaunchKernelGGL((scan_lane_dfs<C, L, G, T>), grid, block, lds, s, p)
 #define MG_CASE(C)                                                                  \
   case C:                                                                           \
-    if (p.late_loads && p.gslots)                                                   \
-      MG_LAUNCH(C, true, true);                                                     \
+    if (p.late_loads && p.gslots && table)                                          \
+      MG_LAUNCH(C, true, true, true);                                               \
+    else if (p.late_loads && p.gslots)                                              \
+      MG_LAUNCH(C, true, true, false);                                              \
     else if (p.late_loads)                                                          \
-      MG_LAUNCH(C, true, false);                                                    \
+      MG_LAUNCH(C, true, false, false);                                             \
+    else if (p.gslots && table)                                                     \
+      MG_LAUNCH(C, false, true, true);                                              \
     else if (p.gslots)                                                              \
-      MG_LAUNCH(C, false, true);                                                    \
+      MG_LAUNCH(C, false, true, false);                                             \
     else                                                                            \
-      MG_LAUNCH(C, false, false);                                                   \
+      MG_LAUNCH(C, false, false, false);                                            \
     break;
     MG_CASE(1) MG_CASE(2) MG_CASE(3) MG_CASE(4) MG_CASE(5) MG_CASE(6) MG_CASE(7) MG_CASE(8)
 #undef MG_CASE
@@ -1360,14 +1418,21 @@ hipError_t launch_stream(int S, int G, int PK, const StreamParams& p, int waves_
 }
 
 hipError_t lane_dfs_blocks_per_cu(int chunks, bool late_loads, bool global_stack,
-                                  int block_threads, size_t lds, int* blocks) {
+                                  bool root_table, int block_threads, size_t lds, int* blocks) {
+  if (root_table && !global_stack) return hipErrorInvalidValue;
   switch (chunks) {
-#define MG_OCC(C, L, G) \
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_lane_dfs<C, L, G>, block_threads, lds)
+#define MG_OCC(C, L, G, T)                                                                 \
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, scan_lane_dfs<C, L, G, T>,          \
+                                               block_threads, lds)
 #define MG_CASE(C)                                                                          \
   case C:                                                                                   \
-    if (late_loads) return global_stack ? MG_OCC(C, true, true) : MG_OCC(C, true, false);   \
-    return global_stack ? MG_OCC(C, false, true) : MG_OCC(C, false, false);
+    if (late_loads)                                                                         \
+      return global_stack ? (root_table ? MG_OCC(C, true, true, true)                       \
+                                        : MG_OCC(C, true, true, false))                     \
+                          : MG_OCC(C, true, false, false);                                  \
+    return global_stack ? (root_table ? MG_OCC(C, false, true, true)                        \
+                                      : MG_OCC(C, false, true, false))                      \
+                        : MG_OCC(C, false, false, false);
     MG_CASE(1) MG_CASE(2) MG_CASE(3) MG_CASE(4) MG_CASE(5) MG_CASE(6) MG_CASE(7) MG_CASE(8)
 #undef MG_CASE
 #undef MG_OCC

@@ -41,7 +41,16 @@ struct LaneDfsParams {
   uint4* gstack;
   uint32_t gslots;
   uint32_t late_loads;  // rows loaded after a step's expansions (else before)
+  // Global-stack instances: each key's tile roots (the node at level tb of every tile
+  // of the range), [n_keys][tile_end - tile_begin] seeds and control words, from
+  // launch_lane_tile_roots; null when each wave walks its tile prefixes itself.
+  const uint4* root_seed;
+  const uint32_t* root_t;
 };
+
+// Largest tile-root table a lane-DFS launch computes (bytes); a larger one is
+// left out and each wave walks its tile prefixes itself.
+constexpr size_t kLaneRootTableMax = 16u << 20;
 
 struct StreamParams {
   const uint4* rows;
@@ -105,6 +114,9 @@ hipError_t launch_lane_dfs(int chunks, const LaneDfsParams& p, int waves_per_blo
                            hipStream_t s);
 // Dynamic LDS of a lane-DFS block: kl - 1 - gslots levels of 64 seeds per wave.
 size_t lane_dfs_lds_bytes(int waves_per_block, uint32_t kl, uint32_t gslots);
+// Writes the tile roots of p's keys and tile range into seeds / ts ([n_keys][tiles]).
+hipError_t launch_lane_tile_roots(const LaneDfsParams& p, uint4* seeds, uint32_t* ts,
+                                  hipStream_t s);
 hipError_t launch_stream(int S, int G, int PK, const StreamParams& p, int waves_per_block,
                          hipStream_t s);
 size_t stream_lds_bytes(int G, uint32_t kl, int waves_per_block);
@@ -115,7 +127,7 @@ size_t split_lds_bytes(int G, uint32_t kl, int NP, uint32_t mask_depth);
 hipError_t split_blocks_per_cu(int S, int G, int NP, int DW, size_t lds, int* blocks);
 // Resident blocks per CU for the given block size and dynamic LDS.
 hipError_t lane_dfs_blocks_per_cu(int chunks, bool late_loads, bool global_stack,
-                                  int block_threads, size_t lds, int* blocks);
+                                  bool root_table, int block_threads, size_t lds, int* blocks);
 hipError_t stream_blocks_per_cu(int S, int G, int PK, int block_threads, size_t lds,
                                 int* blocks);
 hipError_t launch_point(const PointParams& p, hipStream_t s);

@@ -217,8 +217,8 @@ void dedupe_delta(const uint64_t* idx, const uint8_t* diffs, uint64_t n, uint32_
 namespace {
 
 // Resident waves the device holds for a kernel variant (cached per variant).
-// Lane-DFS kinds pass G = late loads (1) + global stack levels (2): the four
-// kernels of one row width. Split kinds pass PK = DFS waves per pair and count
+// Lane-DFS kinds pass G = late loads (1) + global stack levels (2) + tile-root
+// table (4): the six kernels of one row width. Split kinds pass PK = DFS waves per pair and count
 // two waves per pair, whatever the block's real size.
 int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chunks,
                    int waves_per_block, size_t lds, int* out) {
@@ -232,8 +232,8 @@ int resident_waves(mg_device* dev, uint32_t kind, int S, int G, int PK, int chun
     blocks = it->second;
   } else {
     if (kind == MG_KERNEL_LANE_DFS * 100 + (uint32_t)chunks)
-      MG_HIP(lane_dfs_blocks_per_cu(chunks, (G & 1) != 0, (G & 2) != 0, 64 * waves_per_block,
-                                    lds, &blocks));
+      MG_HIP(lane_dfs_blocks_per_cu(chunks, (G & 1) != 0, (G & 2) != 0, (G & 4) != 0,
+                                    64 * waves_per_block, lds, &blocks));
     else if (kind / 100 == MG_KERNEL_STREAM_SPLIT)
       MG_HIP(split_blocks_per_cu(S, G, waves_per_block / 2, PK, lds, &blocks));
     else
@@ -364,9 +364,11 @@ int choose_lane_shape(mg_device* dev, const Plan& pl0, uint32_t d, uint32_t n_ke
         while (wpb > 1 && lane_dfs_lds_bytes((int)wpb, kl, gs) > dev->max_lds) wpb--;
         const size_t lds = lane_dfs_lds_bytes((int)wpb, kl, gs);
         if (lds > dev->max_lds) continue;
+        // With global levels the launch precomputes the tile roots when the table is small.
+        const bool table = gs && n_keys <= 65535 && n_keys * n_tiles * 20 <= kLaneRootTableMax;
         int slots = 0;
         int rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl0.chunks, 0,
-                                (int)late + (gs ? 2 : 0), 1,
+                                (int)late + (gs ? 2 : 0) + (table ? 4 : 0), 1,
                                 (int)pl0.chunks, (int)wpb, lds, &slots);
         if (rc) return rc;
         const uint64_t cap = std::max<uint64_t>((uint64_t)slots, 1);
@@ -396,11 +398,14 @@ int choose_lane_shape(mg_device* dev, const Plan& pl0, uint32_t d, uint32_t n_ke
           const uint64_t per_round = (waves + rounds - 1) / rounds;
           const uint64_t wps = (per_round + simds - 1) / simds;
           const double rate = kRate[std::min<uint64_t>(wps, 8)];
-          // Per lane: two expansions per step, 8 per tile touched, the seek when a
-          // run starts inside a tile.
+          // Per lane: two expansions per step, 8 per tile touched (6 with the tile-root
+          // table), the seek when a run starts inside a tile, and without the table the
+          // wave's first walk down the tile prefix.
           const bool whole_tiles = spw % n_pairs == 0;
           const double tiles_touched = (double)((spw + n_pairs - 1) / n_pairs + (whole_tiles ? 0 : 1));
-          const double work = 2.0 * (double)spw + 8.0 * tiles_touched + (whole_tiles ? 0.0 : kl - 1.0);
+          const double work = 2.0 * (double)spw + (table ? 6.0 : 8.0) * tiles_touched +
+                              (whole_tiles ? 0.0 : kl - 1.0) +
+                              (table ? 0.0 : (double)(d - kl - lb));
           // Rows loaded after a step's expansions wait in the open: 1 % dearer at
           // equal waves per SIMD (2^27 x 32 B, one key, four waves per SIMD: 2.87
           // against 2.83 ms), so they win only where they add waves.
@@ -589,8 +594,9 @@ int make_plan(mg_device* dev, const void* rows, uint64_t n_rows, uint32_t row_by
       return fail(MG_ERR_INVALID_ARG, "lane-DFS plan needs %zu B of LDS per block (max %zu)", lds,
                   dev->max_lds);
     rc = resident_waves(dev, MG_KERNEL_LANE_DFS * 100 + pl->chunks, 0,
-                        (int)pl->late_loads + (pl->gslots ? 2 : 0), 1,
-                        (int)pl->chunks, (int)pl->waves_per_block, lds, &resident);
+                        (int)pl->late_loads + (pl->gslots ? 2 : 0) +
+                            (lane_root_entries(*pl, n_keys) ? 4 : 0),
+                        1, (int)pl->chunks, (int)pl->waves_per_block, lds, &resident);
   } else {
     const int S = kernel == MG_KERNEL_STREAM_SPLIT ? split_s(pl) : stream_s(pl->chunks);
     int G = o->keys_per_wave ? (int)o->keys_per_wave : default_keys_per_wave(kernel, S, n_keys);
@@ -720,6 +726,15 @@ int enqueue_scan(mg_device* dev, const Exec& x, const Plan& pl, const ScanBuffer
       p.gslots = pl.gslots;
       p.late_loads = pl.late_loads;
       if (pl.gslots && !b.gstack) return fail(MG_ERR_INVALID_ARG, "lane-DFS plan needs a stack buffer");
+      if (const uint64_t e = lane_root_entries(pl, pl.n_keys)) {
+        // The keys' tile roots first, on the same stream: waves then start each tile at
+        // its root instead of walking the tb prefix levels.
+        uint4* seeds = b.gstack + lane_wave_region_u4(pl, pl.n_keys);
+        uint32_t* ts = reinterpret_cast<uint32_t*>(seeds + e);
+        MG_HIP(launch_lane_tile_roots(p, seeds, ts, s));
+        p.root_seed = seeds;
+        p.root_t = ts;
+      }
       MG_HIP(launch_lane_dfs((int)pl.chunks, p, (int)pl.waves_per_block, s));
     } else if (pl.kernel == MG_KERNEL_STREAM || pl.kernel == MG_KERNEL_STREAM_SPLIT) {
       StreamParams p{};
