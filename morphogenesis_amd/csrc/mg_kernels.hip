@@ -269,9 +269,10 @@ __global__ __launch_bounds__(256) void scan_lane_dfs(LaneDfsParams p) {
   // leaves and fold the pair's rows into the accumulators. The rows are
   // loaded at the start of the step (kLateLoads false: in flight during the
   // expansions) or after the expansions (true: the rows stay out of the
-  // registers the ChaCha rounds need, 69 VGPRs for 32-byte rows against 78,
-  // seven waves per SIMD against six, and the other waves on the SIMD run
-  // while a wave waits for its rows). Loads are unconditional: a row past
+  // registers the ChaCha rounds need, which with the global-stack levels and
+  // the tile-root table leaves 32-byte rows at 60 VGPRs, eight waves per SIMD,
+  // and the other waves on the SIMD run while a wave waits for its rows).
+  // Loads are unconditional: a row past
   // n_rows (or of an idle lane) is read from a zero buffer instead.
   // The block's waves (consecutive keys, and parts) take the same number of
   // steps and meet at a barrier every 2^(kl-1) of them (every tile when runs
