@@ -650,12 +650,13 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
 
 // ------------------------------------------- split producer/consumer scan --
 // Wide rows, several keys per pass. A block holds NP producer/consumer pairs
-// of waves: waves 0 .. NP-1 run the keys' lane DFS exactly as scan_stream
-// does and publish each leaf-pair step's 2 x 64 masks per key into a ring of
-// D mask buffers of their pair; waves NP .. 2NP-1 only stream their pair's
-// rows (NB batches of U rows in flight, including the next step's first
-// batches while they wait) and fold them. The two waves of a pair hand steps
-// over through two counters in LDS, steps published and steps folded: a DFS
+// of waves: waves 0 .. DW*NP-1 run the keys' lane DFS as scan_stream does (DW
+// per pair, each with G / DW of its keys) and publish each leaf-pair step's
+// 2 x 64 masks per key into a ring of D mask buffers of their pair; the last
+// NP waves only stream their pair's rows (NB batches of U rows in flight,
+// including the next step's first batches while they wait) and fold them. The
+// waves of a pair hand steps over through counters in LDS, steps published
+// (one per DFS wave) and steps folded: a DFS
 // wave runs up to D - 1 steps ahead and only waits when the ring is full, so
 // the steps that descend a whole subtree (kl expansions per key, against one
 // or two for most steps) no longer stall the streaming wave as a per-step
