@@ -98,6 +98,31 @@ def test_lane_depths_agree(mg, scanner, kl):
     m.close()
 
 
+def test_tile_root_table_and_prefix_walk_agree(mg, scanner):
+    # Launches with global stack levels start tiles from a precomputed table of the
+    # keys' tile roots when it fits 16 MiB, else each wave walks (and caches) its tile
+    # prefixes. 2^22 rows in 256-row tiles (kl = 2): 8 keys take the table (2.6 MB),
+    # 64 keys the walk (21 MB); the first 8 keys' answers must agree, and equal the
+    # oracle, over runs that start inside tiles too.
+    n, w, d = 1 << 22, 32, 22
+    rows = oracle.fill_rows(seed=22, first_row=0, n_rows=n, row_bytes=w)
+    m = upload(mg, scanner, rows)
+    targets = [int(x) for x in np.random.default_rng(22).integers(0, n, size=32)]
+    pairs = pairs_for(d, targets, seed=5)
+    keys = [p[0] for p in pairs] + [p[1] for p in pairs]
+    ref = oracle.scan_fast(keys[:8], rows, THREADS)
+    for wpk in (0, 333):
+        opts = mg.make_opts(kernel=mg.KERNEL_LANE_DFS, lane_depth=2, stack_global=1,
+                            late_loads=True, waves_per_key=wpk)
+        few = scanner.scan_keys(m, keys[:8], opts)
+        many = scanner.scan_keys(m, keys, opts)
+        assert np.array_equal(few, ref), wpk
+        assert np.array_equal(many[:8], ref), wpk
+        for q, tgt in enumerate(targets):
+            assert np.array_equal(many[q] ^ many[q + 32], rows[tgt]), (wpk, q)
+    m.close()
+
+
 @pytest.mark.parametrize("w", [16, 48, 128])
 def test_global_stack_levels_by_row_width(mg, scanner, w):
     # 2^17 rows at kl = 9: every stack split, and the planner's own choice.
