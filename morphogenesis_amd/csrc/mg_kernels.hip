@@ -1040,6 +1040,12 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
             ++ns;
           }
           load(ns < n_steps ? ns : n_steps - 1, nb, ring[k]);
+          // Rows of 2 or 4 chunks per lane: keep the refill ahead of the next batch's
+          // folds. Left free, the scheduler sinks these loads to just before their use and
+          // waits for each one (16,817,159 x 4 KiB, 1 key: 23.9 against 9.9 ms; 2 KiB rows,
+          // 1 / 2 / 3 keys: 40.7 / 20.2 / 11.0 against 10.5 / 10.6 / 10.6 ms, one process).
+          // Rows of one chunk per lane keep the same waits either way.
+          if constexpr (S > 1) __builtin_amdgcn_sched_barrier(0);
         }
       }
       lds_store_release(folded, (uint32_t)(s + 1));  // after this step's mask reads
