@@ -668,6 +668,9 @@ __global__ __launch_bounds__(256) void scan_stream(StreamParams p) {
 // Needs all 64 lanes live (lb = 6), so a step is always 128 rows.
 constexpr int kSplitNB = 4;
 constexpr uint32_t kSplitSpinLimit = 1u << 22;  // ~0.3 s of polls; a step takes microseconds
+// Ring turns (NB batches) a streaming wave may run ahead of a partner pair reading the same
+// rows for other keys.
+constexpr uint32_t kSplitPace = 2;
 
 __device__ __forceinline__ uint32_t lds_load_acquire(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -770,9 +773,13 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
   uint32_t* ctr = reinterpret_cast<uint32_t*>(masks + (size_t)D * G * 128);
   uint32_t* published = ctr + 2 * sub;  // steps whose masks (this DFS wave's keys) are in the ring
   uint32_t* folded = ctr + 1;  // steps whose masks the streaming wave is done with
+  uint32_t* turns = ctr + 3;  // ring turns the streaming wave has started
   if (producer && lane == 0) {
     *published = 0;
-    if (sub == 0) *folded = 0;
+    if (sub == 0) {
+      *folded = 0;
+      *turns = 0;
+    }
   }
   __syncthreads();  // the only block barrier: counters are set before any wave reads them
 
@@ -965,6 +972,32 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
   for (int g = 0; g < G; g++)
 #pragma unroll
     for (int j = 0; j < S; j++) acc[g][j] = make_uint4(0, 0, 0, 0);
+  // Partners: the block's other pairs of this part and column half (other key groups),
+  // which read the same rows. Their streaming waves keep within kSplitPace ring turns of
+  // each other, so that one pair's row reads still sit in L2 when the others ask for the
+  // same rows, and a pass of several groups reads the matrix from HBM about once (without
+  // the pacing, the pairs of a block drift apart: 1.55x the matrix at 8 keys over 2 KiB
+  // rows). The pair that is furthest behind never waits, so the pacing cannot deadlock.
+  // Only rows of 2 chunks per lane are paced (2 KiB rows, and the column halves of 4 KiB
+  // rows that passes of 4 or more keys use): a pass of several groups over 1 KiB rows is
+  // bound by its DFS waves, and tying the pairs together there cost 9-15 % (78,643,200 x
+  // 1 KiB, 8 keys: 26.3-27.8 against 24.0 ms); whole 4 KiB rows run one group per pass
+  // unless column halves are turned off. Pairs with two DFS waves run in passes of one
+  // group.
+  constexpr bool kPaced = NP > 1 && DW == 1 && S == 2;
+  uint32_t partners = 0;
+  if constexpr (kPaced) {
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+      const uint32_t pq = blk * NP + (uint32_t)q;
+      if (q != pi && live && pq < total && pq % halves == half && pq / (halves * n_groups) == part)
+        partners |= 1u << q;
+    }
+  }
+  auto pair_turns = [&](int q) -> const uint32_t* {
+    const uint4* ql = lds + (size_t)q * split_pair_u4(G, p.kl, D);
+    return reinterpret_cast<const uint32_t*>(ql + (size_t)G * lds_slots * 64 + (size_t)D * G * 128) + 3;
+  };
 
   // The streaming loop, in two instances: kWhole for runs whose steps all lie
   // in tiles wholly inside the matrix (every run but one that reaches a ragged
@@ -976,8 +1009,20 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
   // compiler's wait counts keep all NB batches in flight: with the refill
   // skipped at the end and the per-row check in the same loop they came out as
   // waits for all but one or two of the batch's loads.
-  auto consume = [&](auto whole_tag) -> bool {
+  // kShared: the pass has partners (see above), and the rows are loaded with the default
+  // cache policy, so that a partner's reads of the same rows hit L2; without partners they
+  // are loaded non-temporally, which streams faster alone (39,321,600 x 2 KiB, 3 keys in
+  // one group: 12.3 against 13.1 ms; 8 keys in two groups: 16.1 against 22.1 ms, 1.06x
+  // against 1.55-1.66x the matrix from HBM).
+  auto consume = [&](auto whole_tag, auto shared_tag) -> bool {
     constexpr bool kWhole = decltype(whole_tag)::value;
+    constexpr bool kShared = decltype(shared_tag)::value;
+    auto ld = [](const uint4* a) -> uint4 {
+      if constexpr (kShared)
+        return *a;
+      else
+        return load_nt(a);
+    };
     auto load = [&](uint64_t s, uint32_t b, uint4 (&v)[U][S]) {
       const uint64_t gs = gs0 + s;
       const uint64_t tile_base = (p.tile_begin + (gs >> (kl - 1))) << (kl + lb);
@@ -990,7 +1035,7 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
         for (int u = 0; u < U; u++)
 #pragma unroll
           for (int j = 0; j < S; j++)
-            v[u][j] = load_nt(reinterpret_cast<const uint4*>(base + row_off[u][j]));
+            v[u][j] = ld(reinterpret_cast<const uint4*>(base + row_off[u][j]));
       } else {
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -999,7 +1044,7 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
               tile_base + ((uint64_t)(r >> 1) << kl) + 2 * (uint64_t)pair + (r & 1);
           const uint4* src = x < p.n_rows ? p.rows + x * C : p.zero_rows;
 #pragma unroll
-          for (int j = 0; j < S; j++) v[u][j] = load_nt(src + lane_off[j]);
+          for (int j = 0; j < S; j++) v[u][j] = ld(src + lane_off[j]);
         }
       }
     };
@@ -1013,6 +1058,7 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
       __builtin_amdgcn_sched_barrier(0);
     }
     uint32_t buf = 0;  // step s's masks are in buffer s mod D
+    uint32_t turn = 0;
     for (uint64_t s = 0; s < n_steps; ++s) {
       // Step s's masks are in the ring once every DFS wave of the pair has published it.
       if (!wait_count(ctr, (uint32_t)(s + 1))) return false;
@@ -1020,6 +1066,23 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
       const uint4* mbuf = masks + (size_t)buf * G * 128;
 #pragma unroll
       for (uint32_t b0 = 0; b0 < n_batches; b0 += NB) {
+        if constexpr (kShared) {
+          // Publish this ring turn, then wait while a partner is more than kSplitPace
+          // turns behind.
+          ++turn;
+          __hip_atomic_store(turns, turn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+          for (int q = 0; q < NP; q++) {
+            if (!((partners >> q) & 1u)) continue;
+            const uint32_t* tq = pair_turns(q);
+            uint32_t spin = 0;
+            while ((int32_t)(__hip_atomic_load(tq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) +
+                             kSplitPace - turn) < 0) {
+              if (++spin == kSplitSpinLimit) return false;
+              __builtin_amdgcn_s_sleep(1);
+            }
+          }
+        }
 #pragma unroll
         for (int k = 0; k < NB; k++) {
           const uint32_t b = b0 + (uint32_t)k;
@@ -1044,8 +1107,9 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
           // folds. Left free, the scheduler sinks these loads to just before their use and
           // waits for each one (16,817,159 x 4 KiB, 1 key: 23.9 against 9.9 ms; 2 KiB rows,
           // 1 / 2 / 3 keys: 40.7 / 20.2 / 11.0 against 10.5 / 10.6 / 10.6 ms, one process).
-          // Rows of one chunk per lane keep the same waits either way.
-          if constexpr (S > 1) __builtin_amdgcn_sched_barrier(0);
+          // Rows of one chunk per lane keep the same waits either way, and 3 keys over
+          // 4 chunks per lane, whose streaming wave holds 256 VGPRs, would spill with it.
+          if constexpr (S > 1 && !(S == 4 && G == 3)) __builtin_amdgcn_sched_barrier(0);
         }
       }
       lds_store_release(folded, (uint32_t)(s + 1));  // after this step's mask reads
@@ -1055,9 +1119,15 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
   };
   if (n_steps) {
     const uint64_t last_tile = p.tile_begin + ((gs0 + n_steps - 1) >> (kl - 1));
-    const bool ok = ((last_tile + 1) << (kl + lb)) <= p.n_rows
-                        ? consume(std::integral_constant<bool, true>{})
-                        : consume(std::integral_constant<bool, false>{});
+    using yes = std::integral_constant<bool, true>;
+    using no = std::integral_constant<bool, false>;
+    const bool whole = ((last_tile + 1) << (kl + lb)) <= p.n_rows;
+    bool ok;
+    if constexpr (kPaced)
+      ok = partners ? (whole ? consume(yes{}, yes{}) : consume(no{}, yes{}))
+                    : (whole ? consume(yes{}, no{}) : consume(no{}, no{}));
+    else
+      ok = whole ? consume(yes{}, no{}) : consume(no{}, no{});
     if (!ok) {
       atomicOr(p.fault, 1u);
       return;

@@ -271,6 +271,32 @@ def test_split_stream_kernel_matches_oracle(mg, scanner, n, w, d):
     m.close()
 
 
+@pytest.mark.parametrize("n,w,d", [(20_000, 2048, 15), (9000, 4096, 14), (7001, 2048, 13)])
+def test_paced_key_groups_match_oracle(mg, scanner, n, w, d):
+    # Passes of several key groups over rows of two chunks per lane (2 KiB rows, and the
+    # column halves of 4 KiB rows) pace the streaming waves of the pairs that read the same
+    # rows and load the rows through L2. 5-12 keys make 2-3 groups: with three, some pairs
+    # find their partners in the next block and run unpaced. Step runs that start inside
+    # tiles (waves_per_key 7 and 333) and a ragged last tile (7001 rows) included.
+    print(f"case paced n={n} w={w} d={d}", flush=True)
+    rows = oracle.fill_rows(seed=n + 3 * w, first_row=0, n_rows=n, row_bytes=w)
+    m = upload(mg, scanner, rows)
+    targets = [0, n - 1, n // 7, n // 3, n // 2, (2 * n) // 3]
+    pairs = pairs_for(d, targets, seed=w + 5)
+    keys = [p[0] for p in pairs] + [p[1] for p in pairs]
+    ref = oracle.scan_tiled(keys, rows, THREADS)
+    for nk in (5, 8, 12):
+        for wpk in (0, 7, 333):
+            got, t = scanner.scan_keys(m, keys[:nk], mg.make_opts(waves_per_key=wpk),
+                                       with_timing=True)
+            assert t["kernel"] == "stream_split"
+            assert np.array_equal(got, ref[:nk]), (nk, wpk)
+    got = scanner.scan_keys(m, keys)
+    for q, tgt in enumerate(targets):
+        assert np.array_equal(got[q] ^ got[q + len(targets)], rows[tgt])
+    m.close()
+
+
 @pytest.mark.parametrize("n,w,d", [(3000, 1024, 12), (70_001, 1024, 17), (2100, 528, 12),
                                    (5000, 1008, 13)])
 @pytest.mark.parametrize("n_keys", [2, 4])
