@@ -299,6 +299,11 @@ int default_keys_per_wave(uint32_t kernel, int S, uint32_t n_keys) {
                                                     : std::min(3, max_keys_per_wave(kernel, S));
   const int nk = std::max(1, (int)n_keys);
   const int ng = (nk + gmax - 1) / gmax;
+  // Split passes of several groups over rows of two chunks per lane (2 KiB rows, column
+  // halves of 4 KiB rows) pace their pairs, and full groups run faster there than keys
+  // spread evenly (39,321,600 x 2 KiB, 5-6 keys: 15.5 ms as 4 + 1-2 against 17.9 ms as
+  // 3 + 2-3; 19,660,800 x 4 KiB: 16.1 against 18.2-19.6 ms).
+  if (kernel == MG_KERNEL_STREAM_SPLIT && S == 2 && ng > 1) return gmax;
   return (nk + ng - 1) / ng;
 }
 
