@@ -19,7 +19,10 @@ Reported on the same JSON line:
     bit-exact flag: 2^20 x 32 B (the CPU-runnable pair), 2^27 x 32 B (one
     key per server, and the pair in one pass), 78,643,200 x 1 KiB (1 and 3
     keys), and a bounded serving run on the 2^31 x 32 B matrix (Poisson
-    arrivals, an admission deadline, an epoch update every 12 s);
+    arrivals, an admission deadline, an epoch update every 12 s); the
+    reference's own 4 KiB page shapes (16,817,159 and 65,536 pages, 3 keys
+    per query): one query per pass, 32 queries over the scan workspaces'
+    streams, and the same 32 queries batched 2-32 per pass;
   - cpu_baseline: the CPU scan on the host cores of the same run, on a bounded
     sample of the headline workload, in three forms (the reference's tiled
     eval_fused_3dpf_cpu shape, its server scan's eight-rows-per-step shape,
