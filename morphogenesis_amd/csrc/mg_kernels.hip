@@ -738,15 +738,36 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
   const uint32_t per = (gridDim.x + kXcds - 1) / kXcds;
   const uint32_t blk = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;  // XCD-aware
   if ((uint64_t)blk * NP >= total) return;  // the whole block
-  // Group fastest, then column half, then part: the pairs of a block read the same rows, and
-  // in passes of several groups up to NP groups read the same chunks of them, so that one
-  // HBM read serves them all (16,817,159 x 4 KiB in column halves, 24 / 96 keys: 2.24x /
-  // 4.1x the matrix from HBM, against 2.6x / 8.6x with the half fastest, where a block held
-  // two groups of each half; 96 keys 169 against 182 ms a pass).
+  // Pairs that split rows into column halves (rows of 2 chunks per lane in each half, blocks
+  // of 2 or 4 pairs) share their DFS work: pair 2j + 1 of a block takes the other half of
+  // the rows of pair 2j's part and group, and its streaming wave folds them with pair 2j's
+  // masks, so one DFS wave makes the masks for both halves (the other DFS wave ends at
+  // once), and pair 2j's DFS wave waits for both streaming waves before it reuses a buffer
+  // (16,817,159 x 4 KiB, 24 / 96 keys: 40.9 / 150.6 ms a pass and 1.2-1.3x / 2.6-3.2x the
+  // matrix from HBM, against 50.8 / 169.3 ms and 2.24x / 4.1x with four groups of a half
+  // per block and the masks made twice).
+  constexpr bool kHalfShare = NP > 1 && DW == 1 && S == 2;
+  const bool share = kHalfShare && halves == 2;
+  // Pair order: column half fastest, then group, then part when halves share masks; else
+  // group fastest, then column half, then part. Either way the pairs of a block read the
+  // same rows, and in passes of several groups the block's pairs of one half are NP / 2
+  // (or NP) groups reading the same chunks of them, so that one HBM read serves them all.
   const uint32_t pid = blk * NP + (uint32_t)pi;
   const bool live = pid < total;
-  const uint32_t grp = live ? pid % n_groups : 0;
-  const uint32_t half = live ? (pid / n_groups) % halves : 0;
+  // Column half of pair index q (its group: (q >> 1) % n_groups when halves share masks,
+  // else q % n_groups).
+  auto half_of = [&](uint32_t q) -> uint32_t {
+    return share ? (q & 1u) : (q / n_groups) % halves;
+  };
+  uint32_t grp, half;
+  if constexpr (kHalfShare) {
+    grp = live ? (share ? (pid >> 1) % n_groups : pid % n_groups) : 0;
+    half = live ? half_of(pid) : 0;
+  } else {
+    // (Other instances: column half fastest, then group.)
+    half = live ? pid % halves : 0;
+    grp = live ? (pid / halves) % n_groups : 0;
+  }
   const uint32_t part = live ? pid / (halves * n_groups) : 0;
   const uint32_t key0 = grp * G;
   const int kl = (int)p.kl, lb = (int)p.lb, tb = (int)p.tb, d = (int)p.d;
@@ -775,6 +796,12 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
   uint4* stacks = pair_lds;                                  // [G][lds_slots][64]
   uint4* masks = pair_lds + (size_t)G * lds_slots * 64;      // [D][G][128]
   uint32_t* ctr = reinterpret_cast<uint32_t*>(masks + (size_t)D * G * 128);
+  // Pair q's counters: [0] published, [1] folded, [2] published by a second DFS wave,
+  // [3] ring turns of the streaming wave.
+  auto pair_ctr = [&](int q) -> uint32_t* {
+    uint4* ql = lds + (size_t)q * split_pair_u4(G, p.kl, D);
+    return reinterpret_cast<uint32_t*>(ql + (size_t)G * lds_slots * 64 + (size_t)D * G * 128);
+  };
   uint32_t* published = ctr + 2 * sub;  // steps whose masks (this DFS wave's keys) are in the ring
   uint32_t* folded = ctr + 1;  // steps whose masks the streaming wave is done with
   uint32_t* turns = ctr + 3;  // ring turns the streaming wave has started
@@ -787,6 +814,9 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
   }
   __syncthreads();  // the only block barrier: counters are set before any wave reads them
 
+  if constexpr (kHalfShare) {
+    if (producer && share && half == 1) return;  // pair 2j's DFS wave makes these masks
+  }
   if (producer) {
     // This wave's keys are the pair's keys sub * GW .. sub * GW + GW - 1; arrays
     // below are indexed by g < GW, the pair's LDS stacks and masks by gk = sub * GW + g.
@@ -875,10 +905,17 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
       ++pre_lvl;
     };
     for (uint64_t s = 0; s < n_steps; ++s) {
-      // Buffer s % D is free once the streaming wave has folded step s - D.
+      // Buffer s % D is free once the streaming wave has folded step s - D (both halves'
+      // streaming waves, when they share the masks).
       if (s >= D && !wait_count(folded, (uint32_t)(s - D + 1))) {
         atomicOr(p.fault, 1u);
         return;
+      }
+      if constexpr (kHalfShare) {
+        if (s >= D && share && !wait_count(pair_ctr(pi + 1) + 1, (uint32_t)(s - D + 1))) {
+          atomicOr(p.fault, 1u);
+          return;
+        }
       }
       const uint64_t gs = gs0 + s;
       const uint64_t tile = p.tile_begin + (gs >> (kl - 1));
@@ -994,15 +1031,21 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
 #pragma unroll
     for (int q = 0; q < NP; q++) {
       const uint32_t pq = blk * NP + (uint32_t)q;
-      if (q != pi && live && pq < total && (pq / n_groups) % halves == half &&
-          pq / (halves * n_groups) == part)
+      if (q != pi && live && pq < total && pq / (halves * n_groups) == part && half_of(pq) == half)
         partners |= 1u << q;
     }
   }
-  auto pair_turns = [&](int q) -> const uint32_t* {
-    const uint4* ql = lds + (size_t)q * split_pair_u4(G, p.kl, D);
-    return reinterpret_cast<const uint32_t*>(ql + (size_t)G * lds_slots * 64 + (size_t)D * G * 128) + 3;
-  };
+  auto pair_turns = [&](int q) -> const uint32_t* { return pair_ctr(q) + 3; };
+  // The ring this streaming wave folds from: its own pair's, or pair 2j's when it takes the
+  // second half of pair 2j's rows.
+  const uint4* fmasks = masks;
+  const uint32_t* fpub = ctr;
+  if constexpr (kHalfShare) {
+    if (share && half == 1) {
+      fmasks = lds + (size_t)(pi - 1) * split_pair_u4(G, p.kl, D) + (size_t)G * lds_slots * 64;
+      fpub = pair_ctr(pi - 1);
+    }
+  }
 
   // The streaming loop, in two instances: kWhole for runs whose steps all lie
   // in tiles wholly inside the matrix (every run but one that reaches a ragged
@@ -1066,9 +1109,9 @@ __global__ __launch_bounds__(64 * (DW + 1) * NP) void scan_split(StreamParams p)
     uint32_t turn = 0;
     for (uint64_t s = 0; s < n_steps; ++s) {
       // Step s's masks are in the ring once every DFS wave of the pair has published it.
-      if (!wait_count(ctr, (uint32_t)(s + 1))) return false;
+      if (!wait_count(fpub, (uint32_t)(s + 1))) return false;
       if (DW == 2 && !wait_count(ctr + 2, (uint32_t)(s + 1))) return false;
-      const uint4* mbuf = masks + (size_t)buf * G * 128;
+      const uint4* mbuf = fmasks + (size_t)buf * G * 128;
 #pragma unroll
       for (uint32_t b0 = 0; b0 < n_batches; b0 += NB) {
         if constexpr (kShared) {
