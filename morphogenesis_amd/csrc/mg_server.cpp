@@ -625,6 +625,11 @@ void server_loop(mg_server* s) {
         if (items.empty() || items.back().upd) items.emplace_back();
         items.back().admit.push_back(it->req);
         s->queued_keys -= it->req->n_keys;
+        // Out of the queue means admitted for mg_server_abandon: between here and the
+        // slots below `mu` is released, and an abandon that took the request for a queued
+        // one would free it under the loop (it now cancels it, and the next step releases
+        // its slots).
+        it->req->admitted = true;
         it = s->queue.erase(it);
       }
       if (items.empty() && s->active == 0 && !s->queue.empty() && s->queue.front().snap) {

@@ -372,6 +372,15 @@ def test_wide_pages_in_column_halves_through_the_server(mg, scanner):
     keys = make_keys(d, [0, 8999, 4500, 123, 7777], 1, 15)
     ans, _ = srv.wait(srv.submit(keys), 5)
     assert np.array_equal(ans, oracle.scan_tiled(keys, rows, THREADS))
+    # Requests of 9, 4 and 3 keys submitted together: steps of up to 16 keys run as several
+    # groups of 4, paced, with both column halves of a group folding one DFS wave's masks.
+    rng = np.random.default_rng(8)
+    reqs = [make_keys(d, [int(x) for x in rng.integers(0, n, size=k)], k % 2, 40 + 7 * k)
+            for k in (9, 4, 3)]
+    tickets = [srv.submit(r) for r in reqs]
+    for t, r in zip(tickets, reqs):
+        ans, _ = srv.wait(t, len(r))
+        assert np.array_equal(ans, oracle.scan_tiled(r, rows, THREADS))
     srv.close()
     m.close()
 
