@@ -290,6 +290,7 @@ def test_paced_key_groups_match_oracle(mg, scanner, n, w, d):
             got, t = scanner.scan_keys(m, keys[:nk], mg.make_opts(waves_per_key=wpk),
                                        with_timing=True)
             assert t["kernel"] == "stream_split"
+            assert t["keys_per_wave"] == 4, t  # full groups of 4 keys, the last one padded
             assert np.array_equal(got, ref[:nk]), (nk, wpk)
     got = scanner.scan_keys(m, keys)
     for q, tgt in enumerate(targets):
